@@ -1,0 +1,166 @@
+"""Headline benchmark: 2D Ising L=4096 Metropolis (checkerboard, both colours
+fused in one pass) on one MI355X, site-updates/s and fraction of the HBM
+roofline, with the reference's own sequential CPU driver timed beside it.
+
+One "step" is one full Metropolis sweep (every site updated once) over the
+resident int8 lattice.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+METRIC = "site-updates/s/GPU + %HBM roofline; SLMC vs plain indep. samples/s (τ_int)"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_SITE = 2.0  # int8 read + int8 write per site-update
+
+
+def square_csr(L):
+    n = L * L
+    idx = np.arange(n, dtype=np.int64)
+    x, y = idx % L, idx // L
+    right = y * L + (x + 1) % L
+    down = ((y + 1) % L) * L + x
+    left = y * L + (x - 1) % L
+    up = ((y - 1) % L) * L + x
+    # row order of square_lattice (lattice/square.hpp:30-38): right, down, then the
+    # reverse entries of the left and upper neighbours' edges
+    bonds = np.stack([right, down, left, up], axis=1).astype(np.uint32).reshape(-1)
+    off = (np.arange(n + 1, dtype=np.int64) * 4).astype(np.uint32)
+    return off, bonds
+
+
+def random_spins(n, seed):
+    from oracle_lib import random_spins as rs
+    return rs(n, seed, 0)
+
+
+def traffic_from_profile():
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
+    scaled by the copy-kernel calibration), if the summary has been committed."""
+    p = os.path.join(ROOT, "profiles", "ising_fused_4096_traffic.txt")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("hbm_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def cpu_baseline(L, seconds_target=12.0):
+    """The reference's Metropolis<Ising, Flip, Pcg32>::sweep (compiled from its
+    own headers into oracle/_ref) on one host core, over a bounded number of
+    sweeps of the same L x L lattice.  Falls back to the oracle restatement."""
+    from oracle_lib import REF_SO, oracle, ref, square_edges, csr_square
+    n = L * L
+    spins = random_spins(n, 7)
+    kind = "reference"
+    sweeps = 1
+    if os.path.exists(REF_SO):
+        r = ref()
+        ei, ej = square_edges(L, L)
+        def run(k):
+            s = spins.copy()
+            r.ref_ising_metropolis_seq(n, len(ei), ei, ej, s, 1, 0.44, 2026, 1, k, 0, None, None, None)
+    else:
+        kind = "port"
+        o = oracle()
+        off, bonds = csr_square(L, L)
+        def run(k):
+            s = spins.copy()
+            o.orc_ising_metropolis_seq(n, off, bonds, s, 1, 0.44, 2026, 1, k, 0, None, None, None)
+    t0 = time.perf_counter()
+    run(1)
+    t1 = time.perf_counter() - t0
+    sweeps = max(1, int(seconds_target / max(t1, 1e-6)))
+    t0 = time.perf_counter()
+    run(sweeps)
+    dt = time.perf_counter() - t0
+    return {"value": n * sweeps / dt, "unit": "site-updates/s", "cores": 1, "kind": kind,
+            "sample": f"{sweeps} sequential sweeps of the {L}x{L} lattice at beta=0.44 "
+                      f"(Metropolis<Ising,Flip,Pcg32>, one thread, {platform.processor() or 'host CPU'})",
+            "seconds": dt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--L", type=int, default=4096)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box here has one MI355X, and the lattice "
+              "of this workload is one GPU's share; run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from universal_classical_montecarlo_amd import abi
+    L = args.L
+    sysm = abi.System.ising_square(L, L, 1)
+    assert sysm.info.schedule == abi.SCHED_FUSED
+    sysm.upload(random_spins(L * L, 7))
+    beta = 0.44
+    # warmup (untimed)
+    sysm.metropolis(beta, args.warmup, 2026, sweep0=0)
+    # timed region: exactly `steps` sweeps, HIP events on the library's stream
+    ms, launches = sysm.bench(beta, args.steps, 2026, repeats=1)
+    sites = L * L
+    kernel_s = ms * 1e-3 / launches
+    value = sites * args.steps / (ms * 1e-3)
+    achieved = BYTES_PER_SITE * sites / kernel_s / 1e9
+    obs = sysm.observe()
+    sysm.close()
+
+    # out-of-cache size (16384^2 = 268 MB int8) for the HBM-bound regime
+    ooc = None
+    if L == 4096:
+        L2 = 16384
+        s2 = abi.System.ising_square(L2, L2, 1)
+        s2.upload(np.where(np.random.default_rng(7).random(L2 * L2) < 0.5, -1, 1).astype(np.int8))
+        s2.metropolis(beta, 5, 11)
+        ms2, n2 = s2.bench(beta, 100, 11)
+        ooc = {"L": L2, "site_updates_per_s": L2 * L2 * 100 / (ms2 * 1e-3),
+               "hbm_frac": BYTES_PER_SITE * L2 * L2 / (ms2 * 1e-3 / n2) / 1e9 / HBM_PEAK_GBS}
+        s2.close()
+
+    cpu = None if args.no_cpu else cpu_baseline(L)
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "site-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int8",
+        "data": "synthetic",
+        "config": {"workload": f"2D Ising square periodic L={L}, J=1, beta={beta}, Flip proposal, "
+                               "checkerboard Metropolis with both colours fused (2 B/site)",
+                   "L": L, "beta": beta, "sites": sites, "energy_per_site_end": obs.energy / sites},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(),
+                     "kernel": "ising_fused", "kernel_avg_us": kernel_s * 1e6,
+                     "bytes_per_site": BYTES_PER_SITE},
+        "cpu_baseline": None if cpu is None else {k: cpu[k] for k in
+                                                  ("value", "unit", "cores", "kind", "sample")},
+        "out_of_cache": ooc,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

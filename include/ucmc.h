@@ -1,0 +1,208 @@
+/*
+ * ucmc.h -- C ABI of the MI355X sampling hot path.
+ *
+ * The reference library (a header-only C++23 lattice Monte Carlo library) runs
+ * its Metropolis and Wolff drivers as sequential host loops:
+ *   Metropolis<M,Q,R>::sweep / run / sample   driver.hpp:61-112
+ *   Wolff<M,A,R>::flip / sweep / run / sample driver.hpp:136-254
+ * over the PairGraph / CellComplex tables (lattice/pair_graph.hpp,
+ * lattice/cell_complex.hpp, lattice/hypercubic.hpp) and the example models
+ * (example/ising.hpp, example/vector_spin.hpp, example/u1_gauge.hpp,
+ * example/sun_gauge.hpp).  This header is the device boundary that replaces
+ * those loops: the host keeps the reference's model and driver types, and the
+ * C++ layer in universal_classical_montecarlo_amd/include/ucmc/gpu.hpp (or any
+ * FFI caller) hands the tables and the field to a system object here, runs
+ * sweeps / flips on the GPU and reads the field or device-reduced observables
+ * back.
+ *
+ * Conventions
+ *  - Plain C types and pointers only; host pointers are owned by the caller and
+ *    are not retained after a call returns.
+ *  - Every function returns UCMC_OK (0) or a negative code; the code mirrors the
+ *    exception type the reference throws for the same condition
+ *    (pair_graph.hpp:49-83, cell_complex.hpp:36-63, square.hpp:15-38,
+ *    hypercubic.hpp:15-64).  ucmc_last_error() gives the message of the
+ *    calling thread's last failure.  Nothing throws across this boundary.
+ *  - One system object per host thread at a time; calls on one system are
+ *    ordered on the system's own HIP stream and are synchronous on return.
+ *  - Field layouts on the host side are exactly the reference models' data():
+ *      Ising        int8_t  [sites]               example/ising.hpp:28
+ *      VectorSpin   double  [sites * N] (AoS)     example/vector_spin.hpp:32
+ *      U1Gauge      double  [links] (theta)       example/u1_gauge.hpp:33, dof/u1.hpp:11
+ *      SUGauge<N>   double  [links * N * N * 2]   example/sun_gauge.hpp:31, dof/sun.hpp:14
+ *                   (row-major complex<double>, re/im interleaved)
+ *      XY angle     double  [sites] (radians)
+ *      plaquette Ising int8_t [L * L]
+ *    The device stores int8 (Ising) or fp32 (everything else).
+ *
+ * Random numbers.  Site (or link) i in sweep s draws, in the reference's order
+ * (proposal draws, then the accept draw only when log_p < 0, per hit), from
+ *   Pcg32(ucmc_key(run_seed, sweep0 + s), i)           -- rng.hpp:14-40
+ * and Wolff flip f draws its axis and seed site from Pcg32(k_f, 0) and the
+ * activation uniform of undirected edge e from Pcg32(k_f, 1 + e), with
+ * k_f = ucmc_key(run_seed, 2^62 + flip0 + f).  ucmc_key is the
+ * (counter+1)-th output of SplitMix64 seeded with run_seed.
+ */
+#ifndef UCMC_H
+#define UCMC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UCMC_ABI_VERSION 1
+
+/* Opaque device system: one model's field and tables on one GPU. */
+typedef struct ucmc_system ucmc_system;
+
+/* status codes */
+#define UCMC_OK                 0
+#define UCMC_E_INVALID_ARGUMENT (-1) /* std::invalid_argument in the reference */
+#define UCMC_E_OUT_OF_RANGE     (-2) /* std::out_of_range                      */
+#define UCMC_E_OVERFLOW         (-3) /* std::overflow_error                    */
+#define UCMC_E_UNSUPPORTED      (-4) /* valid for the reference, not offered on device */
+#define UCMC_E_DEVICE           (-5) /* HIP runtime failure                    */
+#define UCMC_E_NO_DEVICE        (-6) /* no usable gfx950 device                */
+
+/* model kinds */
+#define UCMC_ISING         1 /* example/ising.hpp:14-76        */
+#define UCMC_VECTOR        2 /* example/vector_spin.hpp:14-83  */
+#define UCMC_U1_GAUGE      3 /* example/u1_gauge.hpp:16-85     */
+#define UCMC_SUN_GAUGE     4 /* example/sun_gauge.hpp:16-86    */
+#define UCMC_XY_ANGLE      5 /* XY spin stored as an angle (no reference model) */
+#define UCMC_PLAQ_ISING    6 /* J1 + 4-spin plaquette Ising (no reference model) */
+
+/* proposals (dof/ising.hpp:39-62, dof/vector.hpp:35-122, dof/u1.hpp:35-50,
+ * dof/sun.hpp:237-278) */
+#define UCMC_PROP_FLIP      1 /* Flip                         */
+#define UCMC_PROP_SPHERE    2 /* UniformSphereN<N>            */
+#define UCMC_PROP_ROTATION  3 /* LocalRotation<N>{width}      */
+#define UCMC_PROP_U1_SHIFT  4 /* U1Shift{width}               */
+#define UCMC_PROP_SU_SHIFT  5 /* SUShift<N>{width}            */
+
+/* schedules reported by ucmc_system_info */
+#define UCMC_SCHED_CSR          1 /* colour classes from a greedy colouring, CSR rows */
+#define UCMC_SCHED_CHECKERBOARD 2 /* square lattice, (x + y) & 1, one launch per colour */
+#define UCMC_SCHED_FUSED        3 /* square lattice, both colours in one pass, 2 B/site */
+#define UCMC_SCHED_CUBIC        4 /* simple cubic, (x + y + z) & 1 */
+#define UCMC_SCHED_LINK8        5 /* 4D links, colour 2 mu + parity */
+#define UCMC_SCHED_FOUR         6 /* square lattice, (x & 1) + 2 (y & 1) */
+
+/* flags of ucmc_metropolis_params / ucmc_wolff_params */
+#define UCMC_RECORD_ENERGY   1u /* device FP64 energy after each sweep / flip   */
+#define UCMC_RECORD_ORDER    2u /* device FP64 order parameter after each step  */
+#define UCMC_RECORD_ACCEPTS  4u /* number of update() calls in each sweep       */
+#define UCMC_FORCE_SIMPLE    8u /* Ising: use the one-launch-per-colour kernel  */
+
+/* The reference's PairGraph in CSR form (lattice/pair_graph.hpp:96-107):
+ * row i is bonds[offsets[i] .. offsets[i+1]), bond.j only (NoPayload). */
+typedef struct {
+    size_t          sites;
+    const uint32_t* offsets; /* sites + 1 entries */
+    const uint32_t* bonds;   /* offsets[sites] entries */
+} ucmc_pair_graph;
+
+typedef struct {
+    int      kind;         /* UCMC_ISING ... */
+    int      components;   /* VectorSpin N, SU(N) N, else 1 */
+    size_t   sites;        /* sites() of the model (links for gauge models) */
+    int      schedule;     /* UCMC_SCHED_* used by Metropolis */
+    int      ncolours;
+    size_t   dims[4];      /* Lx, Ly (square), L (cubic / 4D), 0 otherwise */
+    size_t   device_bytes; /* device memory held by the system */
+} ucmc_info;
+
+/* One measurement, FP64-reduced on the device. */
+typedef struct {
+    double   energy;        /* model_check.cpp:45-51 total energy; gauge: total action */
+    double   order[4];      /* sum_i s_i (Ising), sum_i S_i (O(N)) */
+    double   plaquette;     /* mean plaquette (gauge models) */
+    uint64_t accepted;      /* update() calls in the sweep (0 if not recorded) */
+    uint64_t cluster;       /* Wolff cluster size (0 for Metropolis) */
+} ucmc_observables;
+
+typedef struct {
+    double   beta;      /* driver beta, Metropolis(model, q, rng, beta) driver.hpp:66-69 */
+    int      proposal;  /* UCMC_PROP_* */
+    double   width;     /* LocalRotation::delta / U1Shift::width / SUShift::width */
+    int      hits;      /* Metropolis hits per site per sweep, >= 1 */
+    uint64_t run_seed;
+    uint64_t sweep0;    /* counter of the first sweep */
+    uint64_t nsweeps;
+    uint32_t flags;
+} ucmc_metropolis_params;
+
+typedef struct {
+    double   beta;      /* Wolff(model, axis, rng, beta) driver.hpp:140-145 */
+    uint64_t run_seed;
+    uint64_t flip0;     /* counter of the first flip */
+    uint64_t nflips;
+    uint32_t flags;
+} ucmc_wolff_params;
+
+/* ---- version / errors ---- */
+int         ucmc_abi_version(void);
+const char* ucmc_last_error(void);
+int         ucmc_device_count(int* count);
+uint64_t    ucmc_key(uint64_t run_seed, uint64_t counter);
+
+/* ---- systems ---- */
+/* Ising(PairGraph<> graph, int32_t J), example/ising.hpp:23-25.  The field
+ * starts all +1, as in the reference. */
+int ucmc_ising_create(const ucmc_pair_graph* graph, int32_t J, int device, ucmc_system** out);
+/* Ising(square_lattice(Lx, Ly), J) without materialising the CSR table: the
+ * regular-lattice topology is generated on the fly on the device
+ * (README.md:86-88 of the reference lists this as not implemented there). */
+int ucmc_ising_square_create(size_t Lx, size_t Ly, int32_t J, int device, ucmc_system** out);
+/* VectorSpin<N>(PairGraph<> graph, double J), example/vector_spin.hpp:27-28;
+ * N in {2, 3, 4}; starts along the last axis. */
+int ucmc_vector_create(int N, const ucmc_pair_graph* graph, double J, int device,
+                       ucmc_system** out);
+/* U1Gauge(L, beta), example/u1_gauge.hpp:29-31, cold start (theta = 0). */
+int ucmc_u1_gauge_create(size_t L, double beta, int device, ucmc_system** out);
+/* SUGauge<N>(L, beta), example/sun_gauge.hpp:24-27, N == 3, cold start. */
+int ucmc_sun_gauge_create(int N, size_t L, double beta, int device, ucmc_system** out);
+/* XY spins stored as angles on a pair graph, E = -J sum cos(t_i - t_j). */
+int ucmc_xy_angle_create(const ucmc_pair_graph* graph, double J, int device,
+                         ucmc_system** out);
+/* Periodic L x L Ising with nearest-neighbour J1 and 4-spin plaquette K. */
+int ucmc_plaquette_ising_create(size_t L, double J1, double K, int device, ucmc_system** out);
+int ucmc_system_destroy(ucmc_system* sys);
+int ucmc_system_info(const ucmc_system* sys, ucmc_info* out);
+/* The colour of every site in the Metropolis schedule (sites bytes). */
+int ucmc_colouring(const ucmc_system* sys, uint8_t* colour);
+
+/* ---- field transfer (host layout = model data(), see above) ---- */
+int ucmc_field_upload(ucmc_system* sys, const void* host, size_t bytes);
+int ucmc_field_download(ucmc_system* sys, void* host, size_t bytes);
+
+/* ---- drivers ---- */
+/* nsweeps sweeps of Metropolis::sweep (driver.hpp:73-87) in colour order;
+ * series (nullable) receives one entry per sweep for the recorded flags. */
+int ucmc_metropolis_run(ucmc_system* sys, const ucmc_metropolis_params* p,
+                        ucmc_observables* series);
+/* One sweep with per-site tracing: dE (fp32, as computed on the device) and
+ * the accept decision of every site (hit `hits-1` for multi-hit). */
+int ucmc_metropolis_trace(ucmc_system* sys, const ucmc_metropolis_params* p, float* dE,
+                          uint8_t* accepted);
+/* nflips Wolff::flip (driver.hpp:148-206); cluster_sizes (nullable, nflips
+ * entries); series (nullable) as above. */
+int ucmc_wolff_run(ucmc_system* sys, const ucmc_wolff_params* p, uint64_t* cluster_sizes,
+                   ucmc_observables* series);
+/* Observables of the current field. */
+int ucmc_observe(ucmc_system* sys, ucmc_observables* out);
+
+/* ---- measurement ---- */
+/* Runs the same launches as ucmc_metropolis_run (no recording) `repeats`
+ * times and reports the HIP-event time of the timed region and the number of
+ * sweep-kernel launches in it. */
+int ucmc_metropolis_bench(ucmc_system* sys, const ucmc_metropolis_params* p, int repeats,
+                          double* elapsed_ms, uint64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,525 @@
+/*
+ * Gauge and angle models of the oracle (test infrastructure only; see
+ * oracle.h).  Restates dof/u1.hpp:19-50, dof/sun.hpp:14-204,
+ * example/u1_gauge.hpp:36-85, example/sun_gauge.hpp:37-86 and the Metropolis
+ * loop of driver.hpp:73-87.
+ */
+#define _GNU_SOURCE
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_PI 3.14159265358979323846
+#define NMAX 4
+#define MAT (2 * NMAX * NMAX)
+
+/* ------------------------------------------------------------------ U(1) -- */
+
+/* dof/u1.hpp:40-46 */
+double orc_u1_shift(double width, double cur, orc_pcg32* rng)
+{
+    const double two_pi = 2.0 * ORC_PI;
+    const double shift = width * (2.0 * orc_pcg32_uni(rng) - 1.0);
+    return remainder(cur + shift, two_pi);
+}
+
+/* dof/u1.hpp:19-22 */
+static double u1_plaq(double t0, double t1, double t2, double t3)
+{
+    return cos(t0 + t1 - t2 - t3);
+}
+
+/* example/u1_gauge.hpp:36-52: before/after recompute of every incident
+ * plaquette, summed in incidence order. */
+double orc_u1_energy_diff_at(const uint32_t* cells, const uint64_t* inc_off,
+                             const uint32_t* inc_cell, const uint8_t* inc_which,
+                             const double* theta, double beta, size_t i, double p)
+{
+    double d = 0.0;
+    for (uint64_t q = inc_off[i]; q < inc_off[i + 1]; ++q) {
+        const uint32_t* c = cells + 4 * (size_t)inc_cell[q];
+        double t[4] = {theta[c[0]], theta[c[1]], theta[c[2]], theta[c[3]]};
+        const double before = u1_plaq(t[0], t[1], t[2], t[3]);
+        t[inc_which[q]] = p;
+        const double after = u1_plaq(t[0], t[1], t[2], t[3]);
+        d += beta * (before - after);
+    }
+    return d;
+}
+
+/* example/u1_gauge.hpp:67-72 */
+double orc_u1_mean_plaquette(size_t ncells, const uint32_t* cells, const double* theta)
+{
+    double sum = 0.0;
+    for (size_t c = 0; c < ncells; ++c)
+        sum += u1_plaq(theta[cells[4 * c]], theta[cells[4 * c + 1]], theta[cells[4 * c + 2]],
+                       theta[cells[4 * c + 3]]);
+    return sum / (double)ncells;
+}
+
+typedef struct {
+    size_t L, sites, ncells;
+    uint32_t* cells;
+    uint64_t* inc_off;
+    uint32_t* inc_cell;
+    uint8_t* inc_which;
+} orc_hc;
+
+static void hc_build(orc_hc* h, size_t L)
+{
+    h->L = L;
+    h->sites = 4 * L * L * L * L;
+    h->ncells = 6 * L * L * L * L;
+    h->cells = (uint32_t*)malloc(4 * h->ncells * sizeof(uint32_t));
+    h->inc_off = (uint64_t*)malloc((h->sites + 1) * sizeof(uint64_t));
+    h->inc_cell = (uint32_t*)malloc(4 * h->ncells * sizeof(uint32_t));
+    h->inc_which = (uint8_t*)malloc(4 * h->ncells);
+    orc_hypercubic_plaquettes(L, h->cells, h->inc_off, h->inc_cell, h->inc_which);
+}
+
+static void hc_free(orc_hc* h)
+{
+    free(h->cells);
+    free(h->inc_off);
+    free(h->inc_cell);
+    free(h->inc_which);
+}
+
+/* Colour of link (mu, x): 2 mu + parity(t + z + y + x), with the link id
+ * mu L^4 + ((t L + z) L + y) L + x of lattice/hypercubic.hpp:37-39.  Links of
+ * one colour share no plaquette when L is even. */
+static int link_colour(size_t L, size_t link)
+{
+    const size_t L4 = L * L * L * L;
+    const size_t mu = link / L4;
+    size_t r = link % L4;
+    size_t par = 0;
+    for (int k = 0; k < 4; ++k) {
+        par += r % L;
+        r /= L;
+    }
+    return (int)(2 * mu + (par & 1));
+}
+
+void orc_u1_metropolis_seq(size_t L, double* theta, double model_beta, double driver_beta,
+                           double width, uint64_t seed, uint64_t stream, size_t burn,
+                           size_t sweeps, double* plaq_out)
+{
+    orc_hc h;
+    hc_build(&h, L);
+    orc_pcg32 rng;
+    orc_pcg32_init(&rng, seed, stream);
+    for (size_t k = 0; k < burn + sweeps; ++k) {
+        for (size_t i = 0; i < h.sites; ++i) {
+            const double cand = orc_u1_shift(width, theta[i], &rng);
+            const double dE = orc_u1_energy_diff_at(h.cells, h.inc_off, h.inc_cell, h.inc_which,
+                                                    theta, model_beta, i, cand);
+            const double log_p = -driver_beta * dE + 0.0;
+            if (log_p >= 0.0 || log(orc_pcg32_uni(&rng)) < log_p) theta[i] = cand;
+        }
+        if (k >= burn && plaq_out)
+            plaq_out[k - burn] = orc_u1_mean_plaquette(h.ncells, h.cells, theta);
+    }
+    hc_free(&h);
+}
+
+void orc_u1_metropolis_keyed(size_t L, double* theta, double model_beta, double driver_beta,
+                             double width, uint64_t run_seed, uint64_t sweep0, size_t nsweeps,
+                             uint64_t* accepts_out, double* trace_dE)
+{
+    orc_hc h;
+    hc_build(&h, L);
+    for (size_t k = 0; k < nsweeps; ++k) {
+        const uint64_t key = orc_key(run_seed, sweep0 + k);
+        uint64_t acc = 0;
+        for (int c = 0; c < 8; ++c)
+            for (size_t i = 0; i < h.sites; ++i) {
+                if (link_colour(L, i) != c) continue;
+                orc_pcg32 rng;
+                orc_pcg32_init(&rng, key, i);
+                const double cand = orc_u1_shift(width, theta[i], &rng);
+                const double dE = orc_u1_energy_diff_at(h.cells, h.inc_off, h.inc_cell,
+                                                        h.inc_which, theta, model_beta, i, cand);
+                const double log_p = -driver_beta * dE + 0.0;
+                if (log_p >= 0.0 || log(orc_pcg32_uni(&rng)) < log_p) {
+                    theta[i] = cand;
+                    ++acc;
+                }
+                if (trace_dE) trace_dE[k * h.sites + i] = dE;
+            }
+        if (accepts_out) accepts_out[k] = acc;
+    }
+    hc_free(&h);
+}
+
+/* ----------------------------------------------------------------- SU(N) -- */
+
+/* dof/sun.hpp:139-149 and 165-176: r_ij = sum_k a_ik b_kj, complex products
+ * (ar br - ai bi, ar bi + ai br) accumulated from 0 in k order. */
+void orc_sun_mul(int N, const double* a, const double* b, double* out)
+{
+    double r[MAT];
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) {
+            double sr = 0.0, si = 0.0;
+            for (int k = 0; k < N; ++k) {
+                const double ar = a[2 * (i * N + k)], ai = a[2 * (i * N + k) + 1];
+                const double br = b[2 * (k * N + j)], bi = b[2 * (k * N + j) + 1];
+                sr += ar * br - ai * bi;
+                si += ar * bi + ai * br;
+            }
+            r[2 * (i * N + j)] = sr;
+            r[2 * (i * N + j) + 1] = si;
+        }
+    memcpy(out, r, (size_t)(2 * N * N) * sizeof(double));
+}
+
+/* dof/sun.hpp:108-115 */
+void orc_sun_dagger(int N, const double* a, double* out)
+{
+    double r[MAT];
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) {
+            r[2 * (i * N + j)] = a[2 * (j * N + i)];
+            r[2 * (i * N + j) + 1] = -a[2 * (j * N + i) + 1];
+        }
+    memcpy(out, r, (size_t)(2 * N * N) * sizeof(double));
+}
+
+/* dof/sun.hpp:152-158: Re Tr(((U0 U1) U2^dag) U3^dag) / N. */
+double orc_sun_plaquette(int N, const double* u0, const double* u1, const double* u2,
+                         const double* u3)
+{
+    double a[MAT], d[MAT];
+    orc_sun_mul(N, u0, u1, a);
+    orc_sun_dagger(N, u2, d);
+    orc_sun_mul(N, a, d, a);
+    orc_sun_dagger(N, u3, d);
+    orc_sun_mul(N, a, d, a);
+    double t = 0.0;
+    for (int i = 0; i < N; ++i) t += a[2 * (i * N + i)];
+    return t / (double)N;
+}
+
+/* dof/sun.hpp:179-188: max row sum of |a_ij| (|z| = hypot). */
+static double sun_norm(int N, const double* a)
+{
+    double n = 0.0;
+    for (int i = 0; i < N; ++i) {
+        double row = 0.0;
+        for (int j = 0; j < N; ++j) row += hypot(a[2 * (i * N + j)], a[2 * (i * N + j) + 1]);
+        if (row > n) n = row;
+    }
+    return n;
+}
+
+/* dof/sun.hpp:192-220: halve while the norm exceeds 0.5, Taylor series up to 30
+ * terms stopping once the term norm is below 1e-16, square back. */
+void orc_sun_expm(int N, const double* a, double* out, int* squarings_out, int* terms_out)
+{
+    const int nn = N * N;
+    double n = sun_norm(N, a);
+    int squarings = 0;
+    while (n > 0.5) {
+        n *= 0.5;
+        ++squarings;
+    }
+    const double scale = ldexp(1.0, -squarings);
+    double b[MAT], result[MAT], term[MAT];
+    for (int k = 0; k < 2 * nn; ++k) b[k] = a[k] * scale;
+    memset(result, 0, sizeof(result));
+    memset(term, 0, sizeof(term));
+    for (int i = 0; i < N; ++i) result[2 * (i * N + i)] = 1.0;
+    for (int i = 0; i < N; ++i) term[2 * (i * N + i)] = 1.0;
+    int terms = 0;
+    for (int k = 1; k <= 30; ++k) {
+        orc_sun_mul(N, term, b, term);
+        for (int q = 0; q < 2 * nn; ++q) term[q] /= (double)k;
+        for (int q = 0; q < 2 * nn; ++q) result[q] += term[q];
+        terms = k;
+        if (sun_norm(N, term) < 1e-16) break;
+    }
+    for (int k = 0; k < squarings; ++k) orc_sun_mul(N, result, result, result);
+    memcpy(out, result, (size_t)(2 * nn) * sizeof(double));
+    if (squarings_out) *squarings_out = squarings;
+    if (terms_out) *terms_out = terms;
+}
+
+/* dof/sun.hpp:245-274: traceless Hermitian X from N diagonal draws, then
+ * (a, b) for every i < j in row order; U' = U exp(i X). */
+void orc_sun_shift(int N, double width, const double* cur, orc_pcg32* rng, double* out)
+{
+    double x[MAT];
+    memset(x, 0, sizeof(x));
+    double tr = 0.0;
+    for (int i = 0; i < N; ++i) {
+        const double d = width * (2.0 * orc_pcg32_uni(rng) - 1.0);
+        x[2 * (i * N + i)] = d;
+        tr += d;
+    }
+    for (int i = 0; i < N; ++i)
+        for (int j = i + 1; j < N; ++j) {
+            const double av = width * (2.0 * orc_pcg32_uni(rng) - 1.0);
+            const double bv = width * (2.0 * orc_pcg32_uni(rng) - 1.0);
+            x[2 * (i * N + j)] = av;
+            x[2 * (i * N + j) + 1] = bv;
+            x[2 * (j * N + i)] = av;
+            x[2 * (j * N + i) + 1] = -bv;
+        }
+    for (int i = 0; i < N; ++i) x[2 * (i * N + i)] -= tr / (double)N;
+    double a[MAT], e[MAT];
+    for (int k = 0; k < N * N; ++k) {
+        a[2 * k] = -x[2 * k + 1];
+        a[2 * k + 1] = x[2 * k];
+    }
+    orc_sun_expm(N, a, e, NULL, NULL);
+    orc_sun_mul(N, cur, e, out);
+}
+
+/* dof/sun.hpp:125-137 */
+double orc_sun_unitarity_error(int N, const double* u)
+{
+    double d[MAT], p[MAT];
+    orc_sun_dagger(N, u, d);
+    orc_sun_mul(N, d, u, p);
+    double e = 0.0;
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) {
+            const double v = hypot(p[2 * (i * N + j)] - (i == j ? 1.0 : 0.0), p[2 * (i * N + j) + 1]);
+            if (v > e) e = v;
+        }
+    return e;
+}
+
+/* example/sun_gauge.hpp:37-51 */
+double orc_sun_energy_diff_at(int N, const uint32_t* cells, const uint64_t* inc_off,
+                              const uint32_t* inc_cell, const uint8_t* inc_which,
+                              const double* links, double beta, size_t i, const double* p)
+{
+    const size_t stride = (size_t)(2 * N * N);
+    double d = 0.0;
+    for (uint64_t q = inc_off[i]; q < inc_off[i + 1]; ++q) {
+        const uint32_t* c = cells + 4 * (size_t)inc_cell[q];
+        const double* l[4] = {links + c[0] * stride, links + c[1] * stride, links + c[2] * stride,
+                              links + c[3] * stride};
+        const double before = orc_sun_plaquette(N, l[0], l[1], l[2], l[3]);
+        l[inc_which[q]] = p;
+        const double after = orc_sun_plaquette(N, l[0], l[1], l[2], l[3]);
+        d += beta * (before - after);
+    }
+    return d;
+}
+
+/* example/sun_gauge.hpp:65-70 */
+double orc_sun_mean_plaquette(int N, size_t ncells, const uint32_t* cells, const double* links)
+{
+    const size_t stride = (size_t)(2 * N * N);
+    double sum = 0.0;
+    for (size_t c = 0; c < ncells; ++c)
+        sum += orc_sun_plaquette(N, links + cells[4 * c] * stride, links + cells[4 * c + 1] * stride,
+                                 links + cells[4 * c + 2] * stride, links + cells[4 * c + 3] * stride);
+    return sum / (double)ncells;
+}
+
+void orc_sun_metropolis_seq(int N, size_t L, double* links, double model_beta,
+                            double driver_beta, double width, uint64_t seed, uint64_t stream,
+                            size_t burn, size_t sweeps, double* plaq_out)
+{
+    orc_hc h;
+    hc_build(&h, L);
+    const size_t stride = (size_t)(2 * N * N);
+    orc_pcg32 rng;
+    orc_pcg32_init(&rng, seed, stream);
+    double cand[MAT];
+    for (size_t k = 0; k < burn + sweeps; ++k) {
+        for (size_t i = 0; i < h.sites; ++i) {
+            orc_sun_shift(N, width, links + i * stride, &rng, cand);
+            const double dE = orc_sun_energy_diff_at(N, h.cells, h.inc_off, h.inc_cell,
+                                                     h.inc_which, links, model_beta, i, cand);
+            const double log_p = -driver_beta * dE + 0.0;
+            if (log_p >= 0.0 || log(orc_pcg32_uni(&rng)) < log_p)
+                memcpy(links + i * stride, cand, stride * sizeof(double));
+        }
+        if (k >= burn && plaq_out)
+            plaq_out[k - burn] = orc_sun_mean_plaquette(N, h.ncells, h.cells, links);
+    }
+    hc_free(&h);
+}
+
+/* Keyed replay: per sweep, colours 0..7, and for each link `hits` consecutive
+ * Metropolis steps drawing from the link's own Pcg32(key, link) (proposal
+ * draws, then the accept draw when log_p < 0, per hit).  trace_dE gets the dE
+ * of every hit, index (sweep * sites + link) * hits + hit. */
+void orc_sun_metropolis_keyed(int N, size_t L, double* links, double model_beta,
+                              double driver_beta, double width, int hits, uint64_t run_seed,
+                              uint64_t sweep0, size_t nsweeps, uint64_t* accepts_out,
+                              double* trace_dE)
+{
+    orc_hc h;
+    hc_build(&h, L);
+    const size_t stride = (size_t)(2 * N * N);
+    double cand[MAT];
+    for (size_t k = 0; k < nsweeps; ++k) {
+        const uint64_t key = orc_key(run_seed, sweep0 + k);
+        uint64_t acc = 0;
+        for (int c = 0; c < 8; ++c)
+            for (size_t i = 0; i < h.sites; ++i) {
+                if (link_colour(L, i) != c) continue;
+                orc_pcg32 rng;
+                orc_pcg32_init(&rng, key, i);
+                for (int hit = 0; hit < hits; ++hit) {
+                    orc_sun_shift(N, width, links + i * stride, &rng, cand);
+                    const double dE = orc_sun_energy_diff_at(N, h.cells, h.inc_off, h.inc_cell,
+                                                             h.inc_which, links, model_beta, i,
+                                                             cand);
+                    const double log_p = -driver_beta * dE + 0.0;
+                    if (log_p >= 0.0 || log(orc_pcg32_uni(&rng)) < log_p) {
+                        memcpy(links + i * stride, cand, stride * sizeof(double));
+                        ++acc;
+                    }
+                    if (trace_dE) trace_dE[(k * h.sites + i) * (size_t)hits + hit] = dE;
+                }
+            }
+        if (accepts_out) accepts_out[k] = acc;
+    }
+    hc_free(&h);
+}
+
+/* Reunitarisation: normalise row 0, orthogonalise and normalise row 1, row 2 =
+ * conj(row0 x row1), which makes det = 1. */
+void orc_su3_reunitarize(double* u)
+{
+    double n0 = 0.0;
+    for (int j = 0; j < 3; ++j) n0 += u[2 * j] * u[2 * j] + u[2 * j + 1] * u[2 * j + 1];
+    n0 = 1.0 / sqrt(n0);
+    for (int j = 0; j < 6; ++j) u[j] *= n0;
+    double pr = 0.0, pi = 0.0; /* <row0, row1> = sum conj(r0) r1 */
+    for (int j = 0; j < 3; ++j) {
+        const double ar = u[2 * j], ai = u[2 * j + 1];
+        const double br = u[6 + 2 * j], bi = u[6 + 2 * j + 1];
+        pr += ar * br + ai * bi;
+        pi += ar * bi - ai * br;
+    }
+    for (int j = 0; j < 3; ++j) {
+        const double ar = u[2 * j], ai = u[2 * j + 1];
+        u[6 + 2 * j] -= pr * ar - pi * ai;
+        u[6 + 2 * j + 1] -= pr * ai + pi * ar;
+    }
+    double n1 = 0.0;
+    for (int j = 0; j < 3; ++j)
+        n1 += u[6 + 2 * j] * u[6 + 2 * j] + u[6 + 2 * j + 1] * u[6 + 2 * j + 1];
+    n1 = 1.0 / sqrt(n1);
+    for (int j = 0; j < 6; ++j) u[6 + j] *= n1;
+    /* row2_k = conj(row0_l row1_m - row0_m row1_l), (k, l, m) cyclic */
+    for (int k = 0; k < 3; ++k) {
+        const int l = (k + 1) % 3, m = (k + 2) % 3;
+        const double a0r = u[2 * l], a0i = u[2 * l + 1], b1r = u[6 + 2 * m], b1i = u[6 + 2 * m + 1];
+        const double a1r = u[2 * m], a1i = u[2 * m + 1], b0r = u[6 + 2 * l], b0i = u[6 + 2 * l + 1];
+        const double cr = (a0r * b1r - a0i * b1i) - (a1r * b0r - a1i * b0i);
+        const double ci = (a0r * b1i + a0i * b1r) - (a1r * b0i + a1i * b0r);
+        u[12 + 2 * k] = cr;
+        u[12 + 2 * k + 1] = -ci;
+    }
+}
+
+/* ------------------------------------------------------- XY as an angle -- */
+
+double orc_xy_energy_diff_at(const uint32_t* off, const uint32_t* bonds, const double* t,
+                             double J, size_t i, double p)
+{
+    double d = 0.0;
+    for (uint32_t a = off[i]; a < off[i + 1]; ++a) {
+        const double tj = t[bonds[a]];
+        d += -J * (cos(p - tj) - cos(t[i] - tj));
+    }
+    return d;
+}
+
+void orc_xy_metropolis_keyed(size_t n, const uint32_t* off, const uint32_t* bonds, double* t,
+                             double J, double beta, double width, const uint8_t* colour,
+                             int ncolours, uint64_t run_seed, uint64_t sweep0, size_t nsweeps,
+                             uint64_t* accepts_out, double* trace_dE)
+{
+    for (size_t k = 0; k < nsweeps; ++k) {
+        const uint64_t key = orc_key(run_seed, sweep0 + k);
+        uint64_t acc = 0;
+        for (int c = 0; c < ncolours; ++c)
+            for (size_t i = 0; i < n; ++i) {
+                if (colour[i] != c) continue;
+                orc_pcg32 rng;
+                orc_pcg32_init(&rng, key, i);
+                const double cand = orc_u1_shift(width, t[i], &rng);
+                const double dE = orc_xy_energy_diff_at(off, bonds, t, J, i, cand);
+                const double log_p = -beta * dE + 0.0;
+                if (log_p >= 0.0 || log(orc_pcg32_uni(&rng)) < log_p) {
+                    t[i] = cand;
+                    ++acc;
+                }
+                if (trace_dE) trace_dE[k * n + i] = dE;
+            }
+        if (accepts_out) accepts_out[k] = acc;
+    }
+}
+
+/* ------------------------------------- frustrated-plaquette Ising (SLMC) -- */
+
+static size_t fpi_id(size_t L, size_t x, size_t y) { return (y % L) * L + (x % L); }
+
+double orc_fpi_energy(size_t L, const int8_t* s, double J1, double K)
+{
+    double e = 0.0;
+    for (size_t y = 0; y < L; ++y)
+        for (size_t x = 0; x < L; ++x) {
+            const int si = s[fpi_id(L, x, y)];
+            e += -J1 * (double)(si * s[fpi_id(L, x + 1, y)] + si * s[fpi_id(L, x, y + 1)]);
+            e += -K * (double)(si * s[fpi_id(L, x + 1, y)] * s[fpi_id(L, x, y + 1)] *
+                               s[fpi_id(L, x + 1, y + 1)]);
+        }
+    return e;
+}
+
+/* dE of flipping site i: 2 J1 s_i (sum of the 4 neighbours) + 2 K (sum over
+ * the 4 plaquettes containing i of the product of their four spins). */
+double orc_fpi_energy_diff_flip(size_t L, const int8_t* s, double J1, double K, size_t i)
+{
+    const size_t x = i % L, y = i / L;
+    const int si = s[i];
+    const int nn = s[fpi_id(L, x + 1, y)] + s[fpi_id(L, x + L - 1, y)] + s[fpi_id(L, x, y + 1)] +
+                   s[fpi_id(L, x, y + L - 1)];
+    int pq = 0;
+    for (int dy = -1; dy <= 0; ++dy)
+        for (int dx = -1; dx <= 0; ++dx) {
+            const size_t x0 = x + L + (size_t)dx, y0 = y + L + (size_t)dy;
+            int prod = s[fpi_id(L, x0, y0)] * s[fpi_id(L, x0 + 1, y0)] * s[fpi_id(L, x0, y0 + 1)] *
+                       s[fpi_id(L, x0 + 1, y0 + 1)];
+            pq += prod;
+        }
+    return 2.0 * (double)si * (J1 * (double)nn) + 2.0 * K * (double)pq;
+}
+
+void orc_fpi_metropolis_keyed(size_t L, int8_t* s, double J1, double K, double beta,
+                              uint64_t run_seed, uint64_t sweep0, size_t nsweeps,
+                              uint64_t* accepts_out)
+{
+    const size_t n = L * L;
+    for (size_t k = 0; k < nsweeps; ++k) {
+        const uint64_t key = orc_key(run_seed, sweep0 + k);
+        uint64_t acc = 0;
+        for (int c = 0; c < 4; ++c)
+            for (size_t i = 0; i < n; ++i) {
+                const size_t x = i % L, y = i / L;
+                if ((int)((x & 1) + 2 * (y & 1)) != c) continue;
+                orc_pcg32 rng;
+                orc_pcg32_init(&rng, key, i);
+                const double dE = orc_fpi_energy_diff_flip(L, s, J1, K, i);
+                const double log_p = -beta * dE + 0.0;
+                if (log_p >= 0.0 || log(orc_pcg32_uni(&rng)) < log_p) {
+                    s[i] = (int8_t)(-s[i]);
+                    ++acc;
+                }
+            }
+        if (accepts_out) accepts_out[k] = acc;
+    }
+}

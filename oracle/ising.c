@@ -1,0 +1,244 @@
+/*
+ * Ising part of the oracle (test infrastructure only; see oracle.h).
+ * Restates example/ising.hpp:35-65, dof/ising.hpp:39-62 and the driver loops
+ * of driver.hpp:73-87 (Metropolis) and driver.hpp:148-206 (Wolff).
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* example/ising.hpp:35-43 */
+int32_t orc_ising_energy_at(const uint32_t* off, const uint32_t* bonds, const int8_t* s,
+                            int32_t J, size_t i)
+{
+    int32_t e = 0;
+    for (uint32_t a = off[i]; a < off[i + 1]; ++a)
+        e += -J * ((int32_t)s[i] * (int32_t)s[bonds[a]]);
+    return e;
+}
+
+/* example/ising.hpp:46-56 */
+int32_t orc_ising_energy_diff_at(const uint32_t* off, const uint32_t* bonds, const int8_t* s,
+                                 int32_t J, size_t i, int8_t p)
+{
+    const int32_t old = s[i];
+    int32_t d = 0;
+    for (uint32_t a = off[i]; a < off[i + 1]; ++a) {
+        const int32_t sj = s[bonds[a]];
+        d += -J * ((int32_t)p * sj - old * sj);
+    }
+    return d;
+}
+
+/* example/ising.hpp:59-65 */
+int32_t orc_ising_energy_diff_bond(const int8_t* s, int32_t J, size_t i, uint32_t j, int8_t p)
+{
+    const int32_t fixed = s[i];
+    const int32_t old = s[j];
+    return -J * ((int32_t)p * fixed - old * fixed);
+}
+
+/* Accept rule of driver.hpp:83-84 with log_ratio 0, as a count of accepted
+ * 32-bit draws.  log(k 2^-32) is increasing in k with steps >= 2^-32, far
+ * above libm's error, so the accepted draws form a prefix [0, k*). */
+uint64_t orc_accept_threshold(double beta, double dE)
+{
+    const double log_p = -beta * dE + 0.0;
+    if (log_p >= 0.0) return UINT64_C(1) << 32;
+    if (!(log_p == log_p)) return 0; /* NaN: both comparisons are false */
+    uint64_t lo = 0, hi = UINT64_C(1) << 32; /* predicate true at lo (log 0 = -inf) */
+    while (hi - lo > 1) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        if (log((double)mid * 0x1.0p-32) < log_p) lo = mid;
+        else hi = mid;
+    }
+    return hi;
+}
+
+/* driver.hpp:195: uni() < 1 - exp(-beta dE).  k * 2^-32 is exact, so the
+ * accepted draws are again a prefix. */
+uint64_t orc_wolff_threshold(double beta, double dE)
+{
+    const double p = 1.0 - exp(-beta * dE);
+    if (!(p > 0.0)) return 0;
+    uint64_t lo = 0, hi = UINT64_C(1) << 32;
+    if ((double)(hi - 1) * 0x1.0p-32 < p) return hi;
+    while (hi - lo > 1) { /* invariant: pred(lo) true, pred(hi) false */
+        const uint64_t mid = lo + (hi - lo) / 2;
+        if ((double)mid * 0x1.0p-32 < p) lo = mid;
+        else hi = mid;
+    }
+    return hi;
+}
+
+static double total_energy_ising(size_t n, const uint32_t* off, const uint32_t* bonds,
+                                 const int8_t* s, int32_t J)
+{
+    /* test/model_check.cpp:45-51 */
+    double sum = 0.0;
+    for (size_t i = 0; i < n; ++i) sum += orc_ising_energy_at(off, bonds, s, J, i);
+    return 0.5 * sum;
+}
+
+/* driver.hpp:73-87: one sequential Metropolis sweep with the Flip proposal. */
+static uint64_t ising_sweep_seq(size_t n, const uint32_t* off, const uint32_t* bonds,
+                                int8_t* s, int32_t J, double beta, orc_pcg32* rng)
+{
+    uint64_t acc = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const int8_t cand = (int8_t)(-s[i]); /* dof/ising.hpp:43-47 */
+        const double dE = orc_ising_energy_diff_at(off, bonds, s, J, i, cand);
+        const double log_p = -beta * dE + 0.0;
+        if (log_p >= 0.0 || log(orc_pcg32_uni(rng)) < log_p) {
+            s[i] = cand;
+            ++acc;
+        }
+    }
+    return acc;
+}
+
+void orc_ising_metropolis_seq(size_t n, const uint32_t* off, const uint32_t* bonds,
+                              int8_t* s, int32_t J, double beta, uint64_t seed,
+                              uint64_t stream, size_t burn, size_t sweeps, double* e_out,
+                              double* m_out, uint64_t* accepts_out)
+{
+    orc_pcg32 rng;
+    orc_pcg32_init(&rng, seed, stream);
+    for (size_t k = 0; k < burn; ++k) ising_sweep_seq(n, off, bonds, s, J, beta, &rng);
+    for (size_t k = 0; k < sweeps; ++k) {
+        const uint64_t acc = ising_sweep_seq(n, off, bonds, s, J, beta, &rng);
+        if (accepts_out) accepts_out[k] = acc;
+        if (e_out) e_out[k] = total_energy_ising(n, off, bonds, s, J);
+        if (m_out) {
+            double m = 0.0;
+            for (size_t i = 0; i < n; ++i) m += s[i];
+            m_out[k] = m;
+        }
+    }
+}
+
+/* The GPU schedule: colour classes in increasing colour, sites of a class in
+ * increasing index (the order inside a class does not matter, no two sites of a
+ * class are bonded), each site drawing from its own Pcg32(key, site). */
+void orc_ising_metropolis_keyed(size_t n, const uint32_t* off, const uint32_t* bonds,
+                                int8_t* s, int32_t J, double beta, const uint8_t* colour,
+                                int ncolours, uint64_t run_seed, uint64_t sweep0,
+                                size_t nsweeps, uint64_t* accepts_out)
+{
+    for (size_t k = 0; k < nsweeps; ++k) {
+        const uint64_t key = orc_key(run_seed, sweep0 + k);
+        uint64_t acc = 0;
+        for (int c = 0; c < ncolours; ++c)
+            for (size_t i = 0; i < n; ++i) {
+                if (colour[i] != c) continue;
+                orc_pcg32 rng;
+                orc_pcg32_init(&rng, key, i);
+                const int8_t cand = (int8_t)(-s[i]);
+                const double dE = orc_ising_energy_diff_at(off, bonds, s, J, i, cand);
+                const double log_p = -beta * dE + 0.0;
+                if (log_p >= 0.0 || log(orc_pcg32_uni(&rng)) < log_p) {
+                    s[i] = cand;
+                    ++acc;
+                }
+            }
+        if (accepts_out) accepts_out[k] = acc;
+    }
+}
+
+/* driver.hpp:148-206 with FixedAxis (dof/ising.hpp:53-62): r = 1. */
+static uint64_t ising_wolff_flip_seq(size_t n, const uint32_t* off, const uint32_t* bonds,
+                                     int8_t* s, int32_t J, double beta, orc_pcg32* rng,
+                                     char* in_cluster, size_t* stack)
+{
+    const int8_t r = 1;
+    memset(in_cluster, 0, n);
+    const size_t seed = (size_t)(orc_pcg32_uni(rng) * (double)n);
+    in_cluster[seed] = 1;
+    stack[0] = seed;
+    size_t top = 1;
+    uint64_t cluster = 0;
+    while (top > 0) {
+        const size_t i = stack[--top];
+        ++cluster;
+        const int8_t dot = (int8_t)(s[i] * r);
+        const int8_t prop = (int8_t)(s[i] - 2 * dot * r);
+        for (uint32_t a = off[i]; a < off[i + 1]; ++a) {
+            const uint32_t j = bonds[a];
+            if (in_cluster[j]) continue;
+            const int8_t d = (int8_t)(s[j] * r);
+            const int8_t nb = (int8_t)(s[j] - 2 * d * r);
+            const double dE = orc_ising_energy_diff_bond(s, J, i, j, nb);
+            if (orc_pcg32_uni(rng) < 1.0 - exp(-beta * dE)) {
+                in_cluster[j] = 1;
+                stack[top++] = j;
+            }
+        }
+        s[i] = prop;
+    }
+    return cluster;
+}
+
+void orc_ising_wolff_seq(size_t n, const uint32_t* off, const uint32_t* bonds, int8_t* s,
+                         int32_t J, double beta, uint64_t seed, uint64_t stream, size_t burn,
+                         size_t nflips, double* e_out, uint64_t* size_out)
+{
+    orc_pcg32 rng;
+    orc_pcg32_init(&rng, seed, stream);
+    char* in_cluster = (char*)malloc(n);
+    size_t* stack = (size_t*)malloc(n * sizeof(size_t));
+    for (size_t k = 0; k < burn; ++k)
+        ising_wolff_flip_seq(n, off, bonds, s, J, beta, &rng, in_cluster, stack);
+    for (size_t k = 0; k < nflips; ++k) {
+        const uint64_t c = ising_wolff_flip_seq(n, off, bonds, s, J, beta, &rng, in_cluster, stack);
+        if (size_out) size_out[k] = c;
+        if (e_out) e_out[k] = total_energy_ising(n, off, bonds, s, J);
+    }
+    free(in_cluster);
+    free(stack);
+}
+
+/* Keyed Wolff: the cluster is the connected component of the seed in the graph
+ * of activated edges; edge e is activated iff the first draw of
+ * Pcg32(k_f, 1 + e) is below 1 - exp(-beta dE), dE = energy_diff_bond on the
+ * pre-flip spins (driver.hpp:186-198).  This is the distribution the reference's
+ * depth-first growth samples, made independent of traversal order. */
+void orc_ising_wolff_keyed(size_t n, const uint32_t* off, const uint32_t* bonds,
+                           const uint32_t* edge_id, int8_t* s, int32_t J, double beta,
+                           uint64_t run_seed, uint64_t flip0, size_t nflips,
+                           uint64_t* size_out, uint8_t* last_cluster)
+{
+    uint8_t* in = (uint8_t*)calloc(n, 1);
+    uint32_t* queue = (uint32_t*)malloc(n * sizeof(uint32_t));
+    for (size_t f = 0; f < nflips; ++f) {
+        const uint64_t key = orc_key(run_seed, ORC_WOLFF_COUNTER_BASE + flip0 + f);
+        orc_pcg32 ctl;
+        orc_pcg32_init(&ctl, key, 0);
+        const size_t seed = (size_t)(orc_pcg32_uni(&ctl) * (double)n);
+        memset(in, 0, n);
+        size_t head = 0, tail = 0;
+        in[seed] = 1;
+        queue[tail++] = (uint32_t)seed;
+        while (head < tail) {
+            const uint32_t i = queue[head++];
+            for (uint32_t a = off[i]; a < off[i + 1]; ++a) {
+                const uint32_t j = bonds[a];
+                if (in[j]) continue;
+                const int8_t nb = (int8_t)(-s[j]);
+                const double dE = orc_ising_energy_diff_bond(s, J, i, j, nb);
+                orc_pcg32 er;
+                orc_pcg32_init(&er, key, 1 + (uint64_t)edge_id[a]);
+                if (orc_pcg32_uni(&er) < 1.0 - exp(-beta * dE)) {
+                    in[j] = 1;
+                    queue[tail++] = j;
+                }
+            }
+        }
+        for (size_t q = 0; q < tail; ++q) s[queue[q]] = (int8_t)(-s[queue[q]]);
+        if (size_out) size_out[f] = tail;
+        if (last_cluster && f + 1 == nflips) memcpy(last_cluster, in, n);
+    }
+    free(in);
+    free(queue);
+}

@@ -1,0 +1,225 @@
+/*
+ * oracle.h -- CPU restatement of the reference's Metropolis / Wolff sampling
+ * hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (the HIP library, the C++
+ * host layer, the Python host layer) links, loads or calls this code.  Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it, and
+ * only as the checker.
+ *
+ * Every function states the reference file:line it restates.  The sequential
+ * drivers reproduce the reference chains bit for bit (pinned against the
+ * compiled reference in oracle/_ref and against the committed fixtures under
+ * tests/golden/).  The "keyed" drivers replay the GPU schedule: the reference's
+ * own per-site update (proposal -> energy_diff_at -> accept rule -> update),
+ * applied colour class by colour class, with the uniforms of site i in sweep s
+ * drawn from Pcg32(orc_key(run_seed, s), i).
+ *
+ * Build: gcc -std=c11 -O2 -ffp-contract=off -fPIC -shared (see oracle/Makefile).
+ * -ffp-contract=off keeps a*b+c as two roundings, as in the reference's x86-64
+ * build.
+ */
+#ifndef UCMC_ORACLE_H
+#define UCMC_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ RNG -- */
+/* rng.hpp:8-44 */
+typedef struct { uint64_t state, inc; } orc_pcg32;
+void     orc_pcg32_init(orc_pcg32* r, uint64_t seed, uint64_t stream);
+uint32_t orc_pcg32_next(orc_pcg32* r);
+double   orc_pcg32_uni(orc_pcg32* r);
+/* First n outputs of Pcg32(seed, stream).next(). */
+void     orc_pcg32_block(uint64_t seed, uint64_t stream, size_t n, uint32_t* out);
+/* Counter-based key: the (counter+1)-th output of SplitMix64 seeded with
+ * run_seed (Steele, Lea, Flood 2014).  Metropolis sweep s uses counter s, Wolff
+ * flip f uses counter ORC_WOLFF_COUNTER_BASE + f. */
+uint64_t orc_key(uint64_t run_seed, uint64_t counter);
+#define ORC_WOLFF_COUNTER_BASE (UINT64_C(1) << 62)
+
+/* ------------------------------------------------------------- lattices -- */
+/* Error codes mirror the reference's exception types. */
+#define ORC_OK               0
+#define ORC_INVALID_ARGUMENT -1 /* std::invalid_argument */
+#define ORC_OUT_OF_RANGE     -2 /* std::out_of_range      */
+#define ORC_OVERFLOW         -3 /* std::overflow_error    */
+
+/* PairGraph(n, edges, symmetric) CSR build, lattice/pair_graph.hpp:49-83.
+ * offsets[n+1], bonds[total].  Returns total bonds (>= 0) or an error code.
+ * Pass bonds == NULL to only count. */
+int64_t orc_pair_graph(size_t n, size_t nedges, const uint32_t* ei, const uint32_t* ej,
+                       int symmetric, uint32_t* offsets, uint32_t* bonds);
+/* square_lattice(Lx, Ly), lattice/square.hpp:15-38.  Site id y*Lx + x. */
+int64_t orc_square_lattice(size_t Lx, size_t Ly, uint32_t* offsets, uint32_t* bonds);
+/* Periodic simple-cubic graph (no reference builder exists; built as an edge
+ * list in the same form as square.hpp:30-38): site id (z*L + y)*L + x, edges
+ * (x+1), (y+1), (z+1) per site in that order. */
+int64_t orc_cubic_lattice(size_t L, uint32_t* offsets, uint32_t* bonds);
+/* hypercubic_plaquettes(L) + CellComplex<4>, lattice/hypercubic.hpp:15-64 and
+ * lattice/cell_complex.hpp:36-63.  cells[4*6L^4], inc_off[4L^4+1],
+ * inc_cell[24 L^4], inc_which[24 L^4].  Returns the number of cells. */
+int64_t orc_hypercubic_plaquettes(size_t L, uint32_t* cells, uint64_t* inc_off,
+                                  uint32_t* inc_cell, uint8_t* inc_which);
+/* Canonical undirected edge id of every CSR entry (the m-th copy of j in row i
+ * is paired with the m-th copy of i in row j).  Returns the number of
+ * undirected edges, or ORC_INVALID_ARGUMENT if the graph is not symmetric. */
+int64_t orc_edge_ids(size_t n, const uint32_t* offsets, const uint32_t* bonds,
+                     uint32_t* edge_id);
+/* Greedy colouring in site order (smallest colour not used by a neighbour).
+ * Returns the number of colours. */
+int orc_greedy_colouring(size_t n, const uint32_t* offsets, const uint32_t* bonds,
+                         uint8_t* colour);
+
+/* ---------------------------------------------------------------- Ising -- */
+/* example/ising.hpp:35-65 */
+int32_t orc_ising_energy_at(const uint32_t* off, const uint32_t* bonds, const int8_t* s,
+                            int32_t J, size_t i);
+int32_t orc_ising_energy_diff_at(const uint32_t* off, const uint32_t* bonds, const int8_t* s,
+                                 int32_t J, size_t i, int8_t p);
+int32_t orc_ising_energy_diff_bond(const int8_t* s, int32_t J, size_t i, uint32_t j, int8_t p);
+/* Number of k in [0, 2^32) with log(k * 2^-32) < -beta*dE, or 2^32 when
+ * -beta*dE >= 0 (driver.hpp:83-84 with the Flip log_ratio of 0). */
+uint64_t orc_accept_threshold(double beta, double dE);
+/* Number of k in [0, 2^32) with k * 2^-32 < 1 - exp(-beta*dE)
+ * (driver.hpp:195). */
+uint64_t orc_wolff_threshold(double beta, double dE);
+
+/* Reference chain: Metropolis<Ising, Flip, Pcg32>(model, Flip{}, Pcg32(seed,
+ * stream), beta); run(burn); sample(sweeps, observe) with observe recording
+ * the total energy (model_check.cpp:45-51 convention: half the sum of
+ * energy_at) and sum_i s_i.  driver.hpp:73-105.  e_out/m_out may be NULL.
+ * accepts_out (may be NULL) gets the number of update() calls per sampled
+ * sweep. */
+void orc_ising_metropolis_seq(size_t n, const uint32_t* off, const uint32_t* bonds,
+                              int8_t* s, int32_t J, double beta, uint64_t seed,
+                              uint64_t stream, size_t burn, size_t sweeps, double* e_out,
+                              double* m_out, uint64_t* accepts_out);
+/* Keyed colour-order replay (the GPU schedule), nsweeps sweeps starting at
+ * sweep index sweep0.  accepts_out[k] = number of updates in sweep k. */
+void orc_ising_metropolis_keyed(size_t n, const uint32_t* off, const uint32_t* bonds,
+                                int8_t* s, int32_t J, double beta, const uint8_t* colour,
+                                int ncolours, uint64_t run_seed, uint64_t sweep0,
+                                size_t nsweeps, uint64_t* accepts_out);
+/* Reference chain: Wolff<Ising, FixedAxis, Pcg32>; burn flips then nflips flips
+ * with the total energy recorded after each (driver.hpp:130-240). */
+void orc_ising_wolff_seq(size_t n, const uint32_t* off, const uint32_t* bonds, int8_t* s,
+                         int32_t J, double beta, uint64_t seed, uint64_t stream, size_t burn,
+                         size_t nflips, double* e_out, uint64_t* size_out);
+/* Keyed Wolff replay: flip f uses control stream Pcg32(k_f, 0) for the seed
+ * site and Pcg32(k_f, 1 + e) for undirected edge e, k_f =
+ * orc_key(run_seed, ORC_WOLFF_COUNTER_BASE + f).  size_out[k] = cluster size. */
+void orc_ising_wolff_keyed(size_t n, const uint32_t* off, const uint32_t* bonds,
+                           const uint32_t* edge_id, int8_t* s, int32_t J, double beta,
+                           uint64_t run_seed, uint64_t flip0, size_t nflips,
+                           uint64_t* size_out, uint8_t* last_cluster);
+
+/* ------------------------------------------------------- VectorSpin<N> -- */
+/* dof/vector.hpp:12-122, example/vector_spin.hpp:14-83.  Spins are AoS
+ * double[N*n].  prop_kind 0 = UniformSphereN<N>, 1 = LocalRotation<N>{delta}. */
+double orc_vec_energy_at(int N, const uint32_t* off, const uint32_t* bonds, const double* s,
+                         double J, size_t i);
+double orc_vec_energy_diff_at(int N, const uint32_t* off, const uint32_t* bonds,
+                              const double* s, double J, size_t i, const double* p);
+double orc_vec_energy_diff_bond(int N, const double* s, double J, size_t i, uint32_t j,
+                                const double* p);
+void   orc_sphere_sample(int N, orc_pcg32* rng, double* out);
+void   orc_local_rotation(int N, double delta, const double* cur, orc_pcg32* rng,
+                          double* out);
+void   orc_vec_metropolis_seq(int N, size_t n, const uint32_t* off, const uint32_t* bonds,
+                              double* s, double J, double beta, int prop_kind, double delta,
+                              uint64_t seed, uint64_t stream, size_t burn, size_t sweeps,
+                              double* e_out);
+/* Keyed colour-order replay in double precision.  If trace_dE is not NULL it
+ * receives, per sweep and site, the dE of that site's proposal (site index
+ * order) and trace_acc the 0/1 decision. */
+void   orc_vec_metropolis_keyed(int N, size_t n, const uint32_t* off, const uint32_t* bonds,
+                                double* s, double J, double beta, int prop_kind,
+                                double delta, const uint8_t* colour, int ncolours,
+                                uint64_t run_seed, uint64_t sweep0, size_t nsweeps,
+                                uint64_t* accepts_out, double* trace_dE, uint8_t* trace_acc);
+void   orc_vec_wolff_seq(int N, size_t n, const uint32_t* off, const uint32_t* bonds,
+                         double* s, double J, double beta, uint64_t seed, uint64_t stream,
+                         size_t burn, size_t nflips, double* e_out, uint64_t* size_out);
+void   orc_vec_wolff_keyed(int N, size_t n, const uint32_t* off, const uint32_t* bonds,
+                           const uint32_t* edge_id, double* s, double J, double beta,
+                           uint64_t run_seed, uint64_t flip0, size_t nflips,
+                           uint64_t* size_out, uint8_t* last_cluster);
+
+/* ------------------------------------------------------------------ U(1) -- */
+/* dof/u1.hpp:19-50, example/u1_gauge.hpp:16-85, term/plaquette.hpp:13-48 */
+double orc_u1_shift(double width, double cur, orc_pcg32* rng);
+double orc_u1_energy_diff_at(const uint32_t* cells, const uint64_t* inc_off,
+                             const uint32_t* inc_cell, const uint8_t* inc_which,
+                             const double* theta, double beta, size_t i, double p);
+double orc_u1_mean_plaquette(size_t ncells, const uint32_t* cells, const double* theta);
+void   orc_u1_metropolis_seq(size_t L, double* theta, double model_beta, double driver_beta,
+                             double width, uint64_t seed, uint64_t stream, size_t burn,
+                             size_t sweeps, double* plaq_out);
+/* Keyed replay over the 8 (direction x parity) colours of the 4D lattice. */
+void   orc_u1_metropolis_keyed(size_t L, double* theta, double model_beta,
+                               double driver_beta, double width, uint64_t run_seed,
+                               uint64_t sweep0, size_t nsweeps, uint64_t* accepts_out,
+                               double* trace_dE);
+
+/* ----------------------------------------------------------------- SU(N) -- */
+/* dof/sun.hpp:14-204, example/sun_gauge.hpp:16-86.  A link is 2*N*N doubles,
+ * row-major (re, im) pairs. */
+void   orc_sun_mul(int N, const double* a, const double* b, double* out);
+void   orc_sun_dagger(int N, const double* a, double* out);
+double orc_sun_plaquette(int N, const double* u0, const double* u1, const double* u2,
+                         const double* u3);
+void   orc_sun_expm(int N, const double* a, double* out, int* squarings, int* terms);
+void   orc_sun_shift(int N, double width, const double* cur, orc_pcg32* rng, double* out);
+double orc_sun_unitarity_error(int N, const double* u);
+double orc_sun_energy_diff_at(int N, const uint32_t* cells, const uint64_t* inc_off,
+                              const uint32_t* inc_cell, const uint8_t* inc_which,
+                              const double* links, double beta, size_t i, const double* p);
+double orc_sun_mean_plaquette(int N, size_t ncells, const uint32_t* cells,
+                              const double* links);
+void   orc_sun_metropolis_seq(int N, size_t L, double* links, double model_beta,
+                              double driver_beta, double width, uint64_t seed,
+                              uint64_t stream, size_t burn, size_t sweeps, double* plaq_out);
+/* Keyed replay, 8 colours, `hits` Metropolis hits per link per sweep. */
+void   orc_sun_metropolis_keyed(int N, size_t L, double* links, double model_beta,
+                                double driver_beta, double width, int hits,
+                                uint64_t run_seed, uint64_t sweep0, size_t nsweeps,
+                                uint64_t* accepts_out, double* trace_dE);
+/* Gram-Schmidt reunitarisation of SU(3) (rows 0,1 orthonormalised, row 2 =
+ * conj(row0 x row1)), in double. */
+void   orc_su3_reunitarize(double* link);
+
+/* ------------------------------------------------------- XY as an angle -- */
+/* 2D XY with the spin stored as an angle: E = -J sum cos(t_i - t_j), proposal
+ * U1Shift (dof/u1.hpp:35-50) applied to the angle.  No reference model stores
+ * XY as an angle (example/vector_spin.hpp stores a 2-vector), so dE is
+ * -J sum_j [cos(p - t_j) - cos(t_i - t_j)] summed in CSR row order, the same
+ * form as example/vector_spin.hpp:50-60. */
+double orc_xy_energy_diff_at(const uint32_t* off, const uint32_t* bonds, const double* t,
+                             double J, size_t i, double p);
+void   orc_xy_metropolis_keyed(size_t n, const uint32_t* off, const uint32_t* bonds,
+                               double* t, double J, double beta, double width,
+                               const uint8_t* colour, int ncolours, uint64_t run_seed,
+                               uint64_t sweep0, size_t nsweeps, uint64_t* accepts_out,
+                               double* trace_dE);
+
+/* ------------------------------------- frustrated-plaquette Ising (SLMC) -- */
+/* E = -J1 sum_<ij> s_i s_j - K sum_plaq s_i s_j s_k s_l on a periodic L x L
+ * square lattice; no reference model exists.  Plaquette (x,y) = sites (x,y),
+ * (x+1,y), (x,y+1), (x+1,y+1). */
+double orc_fpi_energy(size_t L, const int8_t* s, double J1, double K);
+double orc_fpi_energy_diff_flip(size_t L, const int8_t* s, double J1, double K, size_t i);
+/* Keyed 4-colour (x mod 2, y mod 2) replay. */
+void   orc_fpi_metropolis_keyed(size_t L, int8_t* s, double J1, double K, double beta,
+                                uint64_t run_seed, uint64_t sweep0, size_t nsweeps,
+                                uint64_t* accepts_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,620 @@
+// ref_shim.cpp -- C entry points over the reference's own headers, compiled
+// from /root/reference by oracle/Makefile into oracle/_ref/libucmc_ref.so.
+//
+// TEST INFRASTRUCTURE ONLY: it pins the C restatement in oracle/*.c against
+// the real reference and serves as the "reference" CPU baseline.  Nothing in
+// the product links it.  The reference headers are included, never copied.
+//
+// Two kinds of entry point:
+//  * the reference's own chains, run through its own drivers
+//    (driver.hpp Metropolis / Wolff) on its own models;
+//  * the keyed colour-order replay, which applies the reference's per-site
+//    update (proposal.sample -> model.energy_diff_at -> accept rule ->
+//    model.update) colour class by colour class with Pcg32(key, site) streams.
+
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <vector>
+
+#include <dof/ising.hpp>
+#include <dof/sun.hpp>
+#include <dof/u1.hpp>
+#include <dof/vector.hpp>
+#include <driver.hpp>
+#include <example/ising.hpp>
+#include <example/sun_gauge.hpp>
+#include <example/u1_gauge.hpp>
+#include <example/vector_spin.hpp>
+#include <lattice/cell_complex.hpp>
+#include <lattice/hypercubic.hpp>
+#include <lattice/pair_graph.hpp>
+#include <lattice/square.hpp>
+#include <rng.hpp>
+
+namespace {
+
+constexpr int kInvalid = -1, kRange = -2, kOverflow = -3, kOther = -4;
+
+template <class F>
+std::int64_t guarded(F&& f)
+{
+    try {
+        return f();
+    } catch (const std::invalid_argument&) {
+        return kInvalid;
+    } catch (const std::out_of_range&) {
+        return kRange;
+    } catch (const std::overflow_error&) {
+        return kOverflow;
+    } catch (...) {
+        return kOther;
+    }
+}
+
+std::uint64_t key_of(std::uint64_t run_seed, std::uint64_t counter)
+{
+    std::uint64_t z = run_seed + (counter + 1u) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+PairGraph<> graph_from(std::size_t n, std::size_t ne, const std::uint32_t* ei,
+                       const std::uint32_t* ej)
+{
+    std::vector<PairGraph<>::Edge> edges(ne);
+    for (std::size_t e = 0; e < ne; ++e) edges[e] = {ei[e], ej[e], {}};
+    return PairGraph<>(n, edges);
+}
+
+/// Forwards to a model and counts update() calls; ModelLike is a concept, so
+/// the reference drivers accept it unchanged.
+template <class M>
+struct Counting
+{
+    using value_type = typename M::value_type;
+    static constexpr std::size_t size = M::size;
+    M* m;
+    std::uint64_t updates = 0;
+    std::size_t sites() const noexcept { return m->sites(); }
+    const value_type* data() const noexcept { return m->data(); }
+    double energy_diff_at(std::size_t i, const value_type* p) const noexcept
+    {
+        return m->energy_diff_at(i, p);
+    }
+    void update(std::size_t i, const value_type* p) noexcept
+    {
+        ++updates;
+        m->update(i, p);
+    }
+};
+
+template <class M>
+double half_energy(const M& m)
+{
+    double sum = 0.0;
+    for (std::size_t i = 0; i < m.sites(); ++i) sum += m.energy_at(i);
+    return 0.5 * sum;
+}
+
+/// One keyed colour-order sweep over any ModelLike with proposal Q.
+template <class M, class Q, class ColourFn>
+std::uint64_t keyed_sweep(M& model, const Q& q, double beta, std::uint64_t key, int ncolours,
+                          ColourFn colour_of, double* trace_dE)
+{
+    using T = typename M::value_type;
+    std::array<T, M::size> cand;
+    std::uint64_t acc = 0;
+    for (int c = 0; c < ncolours; ++c)
+        for (std::size_t i = 0; i < model.sites(); ++i) {
+            if (colour_of(i) != c) continue;
+            Pcg32 rng(key, i);
+            const T* cur = model.data() + i * M::size;
+            q.sample(cur, rng, cand.data());
+            const double dE = model.energy_diff_at(i, cand.data());
+            const double log_p = -beta * dE + q.log_ratio(cur, cand.data());
+            if (log_p >= 0.0 || std::log(rng.uni()) < log_p) {
+                model.update(i, cand.data());
+                ++acc;
+            }
+            if (trace_dE) trace_dE[i] = dE;
+        }
+    return acc;
+}
+
+int link_colour(std::size_t L, std::size_t link)
+{
+    const std::size_t L4 = L * L * L * L;
+    std::size_t r = link % L4, par = 0;
+    for (int k = 0; k < 4; ++k) {
+        par += r % L;
+        r /= L;
+    }
+    return static_cast<int>(2 * (link / L4) + (par & 1));
+}
+
+template <std::size_t N>
+void set_links(SUGauge<N>& g, const double* v)
+{
+    for (std::size_t i = 0; i < g.sites(); ++i) {
+        SULink<N> u;
+        for (std::size_t k = 0; k < N * N; ++k) u.m[k] = {v[(i * N * N + k) * 2], v[(i * N * N + k) * 2 + 1]};
+        g.update(i, &u);
+    }
+}
+
+template <std::size_t N>
+void get_links(const SUGauge<N>& g, double* v)
+{
+    for (std::size_t i = 0; i < g.sites(); ++i)
+        for (std::size_t k = 0; k < N * N; ++k) {
+            v[(i * N * N + k) * 2] = g.data()[i].m[k].real();
+            v[(i * N * N + k) * 2 + 1] = g.data()[i].m[k].imag();
+        }
+}
+
+}  // namespace
+
+extern "C" {
+
+void ref_pcg32_block(std::uint64_t seed, std::uint64_t stream, std::size_t n, double* out)
+{
+    Pcg32 r(seed, stream);
+    for (std::size_t k = 0; k < n; ++k) out[k] = r.uni();
+}
+
+std::int64_t ref_pair_graph(std::size_t n, std::size_t ne, const std::uint32_t* ei,
+                            const std::uint32_t* ej, int symmetric, std::uint32_t* offsets,
+                            std::uint32_t* bonds)
+{
+    return guarded([&]() -> std::int64_t {
+        std::vector<PairGraph<>::Edge> edges(ne);
+        for (std::size_t e = 0; e < ne; ++e) edges[e] = {ei[e], ej[e], {}};
+        PairGraph<> g(n, edges, symmetric != 0);
+        std::int64_t total = 0;
+        for (std::size_t i = 0; i < g.sites(); ++i) {
+            offsets[i] = static_cast<std::uint32_t>(total);
+            for (std::size_t a = 0; a < g.degree(i); ++a) bonds[total + a] = g.bonds(i)[a].j;
+            total += static_cast<std::int64_t>(g.degree(i));
+        }
+        offsets[g.sites()] = static_cast<std::uint32_t>(total);
+        return total;
+    });
+}
+
+std::int64_t ref_square_lattice(std::size_t Lx, std::size_t Ly, std::uint32_t* offsets,
+                                std::uint32_t* bonds)
+{
+    return guarded([&]() -> std::int64_t {
+        const auto g = square_lattice(Lx, Ly);
+        std::int64_t total = 0;
+        for (std::size_t i = 0; i < g.sites(); ++i) {
+            if (offsets) offsets[i] = static_cast<std::uint32_t>(total);
+            for (std::size_t a = 0; a < g.degree(i); ++a)
+                if (bonds) bonds[total + a] = g.bonds(i)[a].j;
+            total += static_cast<std::int64_t>(g.degree(i));
+        }
+        if (offsets) offsets[g.sites()] = static_cast<std::uint32_t>(total);
+        return total;
+    });
+}
+
+std::int64_t ref_hypercubic(std::size_t L, std::uint32_t* cells, std::uint64_t* inc_off,
+                            std::uint32_t* inc_cell, std::uint8_t* inc_which)
+{
+    return guarded([&]() -> std::int64_t {
+        const auto cc = hypercubic_plaquettes(L);
+        for (std::size_t c = 0; c < cc.cells(); ++c)
+            for (std::size_t k = 0; k < 4; ++k) cells[4 * c + k] = cc.cell(c)[k];
+        std::uint64_t pos = 0;
+        for (std::size_t i = 0; i < cc.sites(); ++i) {
+            inc_off[i] = pos;
+            for (const auto inc : cc.incident(i)) {
+                inc_cell[pos] = inc.cell;
+                inc_which[pos] = inc.which;
+                ++pos;
+            }
+        }
+        inc_off[cc.sites()] = pos;
+        return static_cast<std::int64_t>(cc.cells());
+    });
+}
+
+// ------------------------------------------------------------------ Ising --
+
+void ref_ising_eval(std::size_t n, std::size_t ne, const std::uint32_t* ei,
+                    const std::uint32_t* ej, const std::int8_t* spins, std::int32_t J,
+                    std::int32_t* dE_flip, std::int32_t* e_at)
+{
+    Ising m(graph_from(n, ne, ei, ej), J);
+    for (std::size_t i = 0; i < n; ++i) m.update(i, spins + i);
+    for (std::size_t i = 0; i < n; ++i) {
+        const IsingSpin p = static_cast<IsingSpin>(-spins[i]);
+        dE_flip[i] = m.energy_diff_at(i, &p);
+        e_at[i] = m.energy_at(i);
+    }
+}
+
+void ref_ising_metropolis_seq(std::size_t n, std::size_t ne, const std::uint32_t* ei,
+                              const std::uint32_t* ej, std::int8_t* spins, std::int32_t J,
+                              double beta, std::uint64_t seed, std::uint64_t stream,
+                              std::size_t burn, std::size_t sweeps, double* e_out, double* m_out,
+                              std::uint64_t* acc_out)
+{
+    Ising m(graph_from(n, ne, ei, ej), J);
+    for (std::size_t i = 0; i < n; ++i) m.update(i, spins + i);
+    Counting<Ising> cm{&m};
+    Metropolis<Counting<Ising>, Flip, Pcg32> metro(cm, Flip{}, Pcg32(seed, stream), beta);
+    metro.run(burn);
+    std::size_t k = 0;
+    metro.sample(sweeps, [&](const Counting<Ising>& c) {
+        if (acc_out) acc_out[k] = c.updates;
+        cm.updates = 0;
+        if (e_out) e_out[k] = half_energy(*c.m);
+        if (m_out) {
+            double s = 0.0;
+            for (std::size_t i = 0; i < n; ++i) s += c.m->order_param_at(i);
+            m_out[k] = s;
+        }
+        ++k;
+    });
+    std::memcpy(spins, m.data(), n);
+}
+
+void ref_ising_keyed(std::size_t n, std::size_t ne, const std::uint32_t* ei,
+                     const std::uint32_t* ej, std::int8_t* spins, std::int32_t J, double beta,
+                     const std::uint8_t* colour, int ncolours, std::uint64_t run_seed,
+                     std::uint64_t sweep0, std::size_t nsweeps, std::uint64_t* acc_out)
+{
+    Ising m(graph_from(n, ne, ei, ej), J);
+    for (std::size_t i = 0; i < n; ++i) m.update(i, spins + i);
+    for (std::size_t k = 0; k < nsweeps; ++k) {
+        const auto acc = keyed_sweep(m, Flip{}, beta, key_of(run_seed, sweep0 + k), ncolours,
+                                     [&](std::size_t i) { return int(colour[i]); }, nullptr);
+        if (acc_out) acc_out[k] = acc;
+    }
+    std::memcpy(spins, m.data(), n);
+}
+
+void ref_ising_wolff_seq(std::size_t n, std::size_t ne, const std::uint32_t* ei,
+                         const std::uint32_t* ej, std::int8_t* spins, std::int32_t J, double beta,
+                         std::uint64_t seed, std::uint64_t stream, std::size_t burn,
+                         std::size_t nflips, double* e_out, std::uint64_t* size_out,
+                         double* mean_cluster)
+{
+    Ising m(graph_from(n, ne, ei, ej), J);
+    for (std::size_t i = 0; i < n; ++i) m.update(i, spins + i);
+    Wolff<Ising, FixedAxis, Pcg32> w(m, FixedAxis{}, Pcg32(seed, stream), beta);
+    for (std::size_t k = 0; k < burn; ++k) w.flip();
+    for (std::size_t k = 0; k < nflips; ++k) {
+        const auto c = w.flip();
+        if (size_out) size_out[k] = c;
+        if (e_out) e_out[k] = half_energy(m);
+    }
+    if (mean_cluster) *mean_cluster = w.mean_cluster_size();
+    std::memcpy(spins, m.data(), n);
+}
+
+// ------------------------------------------------------------ VectorSpin --
+
+extern "C++" {
+template <std::size_t N>
+static void vec_seq(std::size_t n, std::size_t ne, const std::uint32_t* ei, const std::uint32_t* ej,
+                    double* spins, double J, double beta, int kind, double delta,
+                    std::uint64_t seed, std::uint64_t stream, std::size_t burn, std::size_t sweeps,
+                    double* e_out)
+{
+    VectorSpin<N> m(graph_from(n, ne, ei, ej), J);
+    for (std::size_t i = 0; i < n; ++i) m.update(i, spins + i * N);
+    std::size_t k = 0;
+    auto obs = [&](const VectorSpin<N>& mm) {
+        if (e_out) e_out[k] = half_energy(mm);
+        ++k;
+    };
+    if (kind == 0) {
+        Metropolis metro(m, UniformSphereN<N>{}, Pcg32(seed, stream), beta);
+        metro.run(burn);
+        metro.sample(sweeps, obs);
+    } else {
+        Metropolis metro(m, LocalRotation<N>{delta}, Pcg32(seed, stream), beta);
+        metro.run(burn);
+        metro.sample(sweeps, obs);
+    }
+    std::memcpy(spins, m.data(), n * N * sizeof(double));
+}
+
+template <std::size_t N>
+static void vec_keyed(std::size_t n, std::size_t ne, const std::uint32_t* ei, const std::uint32_t* ej,
+                      double* spins, double J, double beta, int kind, double delta,
+                      const std::uint8_t* colour, int ncolours, std::uint64_t run_seed,
+                      std::uint64_t sweep0, std::size_t nsweeps, std::uint64_t* acc_out,
+                      double* trace_dE)
+{
+    VectorSpin<N> m(graph_from(n, ne, ei, ej), J);
+    for (std::size_t i = 0; i < n; ++i) m.update(i, spins + i * N);
+    for (std::size_t k = 0; k < nsweeps; ++k) {
+        const auto key = key_of(run_seed, sweep0 + k);
+        auto col = [&](std::size_t i) { return int(colour[i]); };
+        double* tr = trace_dE ? trace_dE + k * n : nullptr;
+        const auto acc = kind == 0
+            ? keyed_sweep(m, UniformSphereN<N>{}, beta, key, ncolours, col, tr)
+            : keyed_sweep(m, LocalRotation<N>{delta}, beta, key, ncolours, col, tr);
+        if (acc_out) acc_out[k] = acc;
+    }
+    std::memcpy(spins, m.data(), n * N * sizeof(double));
+}
+
+template <std::size_t N>
+static void vec_wolff(std::size_t n, std::size_t ne, const std::uint32_t* ei, const std::uint32_t* ej,
+                      double* spins, double J, double beta, std::uint64_t seed,
+                      std::uint64_t stream, std::size_t burn, std::size_t nflips, double* e_out,
+                      std::uint64_t* size_out)
+{
+    VectorSpin<N> m(graph_from(n, ne, ei, ej), J);
+    for (std::size_t i = 0; i < n; ++i) m.update(i, spins + i * N);
+    Wolff<VectorSpin<N>, UniformSphereN<N>, Pcg32> w(m, UniformSphereN<N>{}, Pcg32(seed, stream), beta);
+    for (std::size_t k = 0; k < burn; ++k) w.flip();
+    for (std::size_t k = 0; k < nflips; ++k) {
+        const auto c = w.flip();
+        if (size_out) size_out[k] = c;
+        if (e_out) e_out[k] = half_energy(m);
+    }
+    std::memcpy(spins, m.data(), n * N * sizeof(double));
+}
+}  // extern "C++"
+
+void ref_vec_metropolis_seq(int N, std::size_t n, std::size_t ne, const std::uint32_t* ei,
+                            const std::uint32_t* ej, double* spins, double J, double beta,
+                            int kind, double delta, std::uint64_t seed, std::uint64_t stream,
+                            std::size_t burn, std::size_t sweeps, double* e_out)
+{
+    if (N == 2) vec_seq<2>(n, ne, ei, ej, spins, J, beta, kind, delta, seed, stream, burn, sweeps, e_out);
+    if (N == 3) vec_seq<3>(n, ne, ei, ej, spins, J, beta, kind, delta, seed, stream, burn, sweeps, e_out);
+    if (N == 4) vec_seq<4>(n, ne, ei, ej, spins, J, beta, kind, delta, seed, stream, burn, sweeps, e_out);
+}
+
+void ref_vec_keyed(int N, std::size_t n, std::size_t ne, const std::uint32_t* ei,
+                   const std::uint32_t* ej, double* spins, double J, double beta, int kind,
+                   double delta, const std::uint8_t* colour, int ncolours, std::uint64_t run_seed,
+                   std::uint64_t sweep0, std::size_t nsweeps, std::uint64_t* acc_out,
+                   double* trace_dE)
+{
+    if (N == 2) vec_keyed<2>(n, ne, ei, ej, spins, J, beta, kind, delta, colour, ncolours, run_seed, sweep0, nsweeps, acc_out, trace_dE);
+    if (N == 3) vec_keyed<3>(n, ne, ei, ej, spins, J, beta, kind, delta, colour, ncolours, run_seed, sweep0, nsweeps, acc_out, trace_dE);
+    if (N == 4) vec_keyed<4>(n, ne, ei, ej, spins, J, beta, kind, delta, colour, ncolours, run_seed, sweep0, nsweeps, acc_out, trace_dE);
+}
+
+void ref_vec_wolff_seq(int N, std::size_t n, std::size_t ne, const std::uint32_t* ei,
+                       const std::uint32_t* ej, double* spins, double J, double beta,
+                       std::uint64_t seed, std::uint64_t stream, std::size_t burn,
+                       std::size_t nflips, double* e_out, std::uint64_t* size_out)
+{
+    if (N == 2) vec_wolff<2>(n, ne, ei, ej, spins, J, beta, seed, stream, burn, nflips, e_out, size_out);
+    if (N == 3) vec_wolff<3>(n, ne, ei, ej, spins, J, beta, seed, stream, burn, nflips, e_out, size_out);
+}
+
+void ref_vec_eval(int N, std::size_t n, std::size_t ne, const std::uint32_t* ei,
+                  const std::uint32_t* ej, const double* spins, double J, const double* props,
+                  double* dE, double* e_at)
+{
+    auto run = [&](auto tag) {
+        constexpr std::size_t NN = decltype(tag)::value;
+        VectorSpin<NN> m(graph_from(n, ne, ei, ej), J);
+        for (std::size_t i = 0; i < n; ++i) m.update(i, spins + i * NN);
+        for (std::size_t i = 0; i < n; ++i) {
+            dE[i] = m.energy_diff_at(i, props + i * NN);
+            e_at[i] = m.energy_at(i);
+        }
+    };
+    if (N == 2) run(std::integral_constant<std::size_t, 2>{});
+    if (N == 3) run(std::integral_constant<std::size_t, 3>{});
+}
+
+void ref_sphere(int N, std::uint64_t seed, std::uint64_t stream, std::size_t count, double* out)
+{
+    Pcg32 rng(seed, stream);
+    for (std::size_t k = 0; k < count; ++k) {
+        if (N == 2) UniformSphereN<2>{}.sample(nullptr, rng, out + k * 2);
+        if (N == 3) UniformSphereN<3>{}.sample(nullptr, rng, out + k * 3);
+        if (N == 4) UniformSphereN<4>{}.sample(nullptr, rng, out + k * 4);
+    }
+}
+
+// Chained LocalRotation steps: out[k] = rotate(out[k-1]), out[-1] = cur.
+void ref_local_rotation(int N, double delta, const double* cur, std::uint64_t seed,
+                        std::uint64_t stream, std::size_t count, double* out)
+{
+    Pcg32 rng(seed, stream);
+    for (std::size_t k = 0; k < count; ++k) {
+        const double* in = k == 0 ? cur : out + (k - 1) * N;
+        if (N == 2) LocalRotation<2>{delta}.sample(in, rng, out + k * 2);
+        if (N == 3) LocalRotation<3>{delta}.sample(in, rng, out + k * 3);
+        if (N == 4) LocalRotation<4>{delta}.sample(in, rng, out + k * 4);
+    }
+}
+
+// ------------------------------------------------------------------- U(1) --
+
+void ref_u1_shift(double width, double cur, std::uint64_t seed, std::uint64_t stream,
+                  std::size_t count, double* out)
+{
+    Pcg32 rng(seed, stream);
+    U1Link c{cur};
+    for (std::size_t k = 0; k < count; ++k) {
+        U1Link o;
+        U1Shift{width}.sample(&c, rng, &o);
+        out[k] = o.theta;
+        c = o;
+    }
+}
+
+void ref_u1_eval(std::size_t L, double beta, const double* theta, const double* cand, double* dE,
+                 double* mean_plaq)
+{
+    U1Gauge g(L, beta);
+    for (std::size_t i = 0; i < g.sites(); ++i) {
+        U1Link p{theta[i]};
+        g.update(i, &p);
+    }
+    for (std::size_t i = 0; i < g.sites(); ++i) {
+        U1Link p{cand[i]};
+        dE[i] = g.energy_diff_at(i, &p);
+    }
+    *mean_plaq = g.mean_plaquette();
+}
+
+void ref_u1_metropolis_seq(std::size_t L, double* theta, double model_beta, double driver_beta,
+                           double width, std::uint64_t seed, std::uint64_t stream,
+                           std::size_t burn, std::size_t sweeps, double* plaq_out)
+{
+    U1Gauge g(L, model_beta);
+    for (std::size_t i = 0; i < g.sites(); ++i) {
+        U1Link p{theta[i]};
+        g.update(i, &p);
+    }
+    Metropolis metro(g, U1Shift{width}, Pcg32(seed, stream), driver_beta);
+    metro.run(burn);
+    std::size_t k = 0;
+    metro.sample(sweeps, [&](const U1Gauge& m) { plaq_out[k++] = m.mean_plaquette(); });
+    for (std::size_t i = 0; i < g.sites(); ++i) theta[i] = g.data()[i].theta;
+}
+
+void ref_u1_keyed(std::size_t L, double* theta, double model_beta, double driver_beta,
+                  double width, std::uint64_t run_seed, std::uint64_t sweep0, std::size_t nsweeps,
+                  std::uint64_t* acc_out, double* trace_dE)
+{
+    U1Gauge g(L, model_beta);
+    for (std::size_t i = 0; i < g.sites(); ++i) {
+        U1Link p{theta[i]};
+        g.update(i, &p);
+    }
+    for (std::size_t k = 0; k < nsweeps; ++k) {
+        const auto acc = keyed_sweep(g, U1Shift{width}, driver_beta, key_of(run_seed, sweep0 + k), 8,
+                                     [&](std::size_t i) { return link_colour(L, i); },
+                                     trace_dE ? trace_dE + k * g.sites() : nullptr);
+        if (acc_out) acc_out[k] = acc;
+    }
+    for (std::size_t i = 0; i < g.sites(); ++i) theta[i] = g.data()[i].theta;
+}
+
+// ------------------------------------------------------------------ SU(N) --
+
+void ref_sun_shift(int N, double width, const double* cur, std::uint64_t seed,
+                   std::uint64_t stream, std::size_t count, double* out)
+{
+    auto run = [&](auto tag) {
+        constexpr std::size_t NN = decltype(tag)::value;
+        Pcg32 rng(seed, stream);
+        SULink<NN> c;
+        for (std::size_t k = 0; k < NN * NN; ++k) c.m[k] = {cur[2 * k], cur[2 * k + 1]};
+        for (std::size_t s = 0; s < count; ++s) {
+            SULink<NN> o;
+            SUShift<NN>{width}.sample(&c, rng, &o);
+            for (std::size_t k = 0; k < NN * NN; ++k) {
+                out[(s * NN * NN + k) * 2] = o.m[k].real();
+                out[(s * NN * NN + k) * 2 + 1] = o.m[k].imag();
+            }
+            c = o;
+        }
+    };
+    if (N == 2) run(std::integral_constant<std::size_t, 2>{});
+    if (N == 3) run(std::integral_constant<std::size_t, 3>{});
+}
+
+void ref_sun_expm(int N, const double* a, double* out)
+{
+    auto run = [&](auto tag) {
+        constexpr std::size_t NN = decltype(tag)::value;
+        sun_detail::Mat<NN> m;
+        for (std::size_t k = 0; k < NN * NN; ++k) m[k] = {a[2 * k], a[2 * k + 1]};
+        const auto e = sun_detail::expm<NN>(m);
+        for (std::size_t k = 0; k < NN * NN; ++k) {
+            out[2 * k] = e.m[k].real();
+            out[2 * k + 1] = e.m[k].imag();
+        }
+    };
+    if (N == 2) run(std::integral_constant<std::size_t, 2>{});
+    if (N == 3) run(std::integral_constant<std::size_t, 3>{});
+}
+
+void ref_sun_eval(int N, std::size_t L, double beta, const double* links, const double* cands,
+                  double* dE, double* mean_plaq, double* total_energy)
+{
+    auto run = [&](auto tag) {
+        constexpr std::size_t NN = decltype(tag)::value;
+        SUGauge<NN> g(L, beta);
+        set_links<NN>(g, links);
+        for (std::size_t i = 0; i < g.sites(); ++i) {
+            SULink<NN> p;
+            for (std::size_t k = 0; k < NN * NN; ++k)
+                p.m[k] = {cands[(i * NN * NN + k) * 2], cands[(i * NN * NN + k) * 2 + 1]};
+            dE[i] = g.energy_diff_at(i, &p);
+        }
+        *mean_plaq = g.mean_plaquette();
+        *total_energy = g.total_energy();
+    };
+    if (N == 2) run(std::integral_constant<std::size_t, 2>{});
+    if (N == 3) run(std::integral_constant<std::size_t, 3>{});
+}
+
+void ref_sun_metropolis_seq(int N, std::size_t L, double* links, double model_beta,
+                            double driver_beta, double width, std::uint64_t seed,
+                            std::uint64_t stream, std::size_t burn, std::size_t sweeps,
+                            double* plaq_out)
+{
+    auto run = [&](auto tag) {
+        constexpr std::size_t NN = decltype(tag)::value;
+        SUGauge<NN> g(L, model_beta);
+        set_links<NN>(g, links);
+        Metropolis<SUGauge<NN>, SUShift<NN>, Pcg32> m(g, SUShift<NN>{width}, Pcg32(seed, stream),
+                                                      driver_beta);
+        m.run(burn);
+        std::size_t k = 0;
+        m.sample(sweeps, [&](const SUGauge<NN>& model) { plaq_out[k++] = model.mean_plaquette(); });
+        get_links<NN>(g, links);
+    };
+    if (N == 2) run(std::integral_constant<std::size_t, 2>{});
+    if (N == 3) run(std::integral_constant<std::size_t, 3>{});
+}
+
+// Keyed replay with `hits` hits per link: the same per-link Pcg32 stream feeds
+// every hit of the link.
+void ref_sun_keyed(int N, std::size_t L, double* links, double model_beta, double driver_beta,
+                   double width, int hits, std::uint64_t run_seed, std::uint64_t sweep0,
+                   std::size_t nsweeps, std::uint64_t* acc_out)
+{
+    auto run = [&](auto tag) {
+        constexpr std::size_t NN = decltype(tag)::value;
+        SUGauge<NN> g(L, model_beta);
+        set_links<NN>(g, links);
+        const SUShift<NN> q{width};
+        for (std::size_t k = 0; k < nsweeps; ++k) {
+            const auto key = key_of(run_seed, sweep0 + k);
+            std::uint64_t acc = 0;
+            for (int c = 0; c < 8; ++c)
+                for (std::size_t i = 0; i < g.sites(); ++i) {
+                    if (link_colour(L, i) != c) continue;
+                    Pcg32 rng(key, i);
+                    for (int h = 0; h < hits; ++h) {
+                        SULink<NN> cand;
+                        q.sample(g.data() + i, rng, &cand);
+                        const double dE = g.energy_diff_at(i, &cand);
+                        const double log_p = -driver_beta * dE + q.log_ratio(g.data() + i, &cand);
+                        if (log_p >= 0.0 || std::log(rng.uni()) < log_p) {
+                            g.update(i, &cand);
+                            ++acc;
+                        }
+                    }
+                }
+            if (acc_out) acc_out[k] = acc;
+        }
+        get_links<NN>(g, links);
+    };
+    if (N == 2) run(std::integral_constant<std::size_t, 2>{});
+    if (N == 3) run(std::integral_constant<std::size_t, 3>{});
+}
+
+}  // extern "C"

@@ -1,0 +1,226 @@
+/*
+ * RNG and topology builders of the oracle (test infrastructure only; see
+ * oracle.h).
+ */
+#include "oracle.h"
+
+#include <stdlib.h>
+#include <string.h>
+
+/* rng.hpp:14-20: state = 0, inc = (stream << 1) | 1, step, state += seed, step. */
+void orc_pcg32_init(orc_pcg32* r, uint64_t seed, uint64_t stream)
+{
+    r->state = 0;
+    r->inc   = (stream << 1u) | 1u;
+    (void)orc_pcg32_next(r);
+    r->state += seed;
+    (void)orc_pcg32_next(r);
+}
+
+/* rng.hpp:31-40: LCG step, XSH-RR output of the old state. */
+uint32_t orc_pcg32_next(orc_pcg32* r)
+{
+    const uint64_t old = r->state;
+    r->state = old * UINT64_C(6364136223846793005) + r->inc;
+    const uint32_t xs  = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    const uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((32u - rot) & 31u));
+}
+
+/* rng.hpp:24-27: exactly k * 2^-32. */
+double orc_pcg32_uni(orc_pcg32* r)
+{
+    return (double)orc_pcg32_next(r) * 0x1.0p-32;
+}
+
+void orc_pcg32_block(uint64_t seed, uint64_t stream, size_t n, uint32_t* out)
+{
+    orc_pcg32 r;
+    orc_pcg32_init(&r, seed, stream);
+    for (size_t k = 0; k < n; ++k) out[k] = orc_pcg32_next(&r);
+}
+
+uint64_t orc_key(uint64_t run_seed, uint64_t counter)
+{
+    uint64_t z = run_seed + (counter + 1u) * UINT64_C(0x9E3779B97F4A7C15);
+    z = (z ^ (z >> 30)) * UINT64_C(0xBF58476D1CE4E5B9);
+    z = (z ^ (z >> 27)) * UINT64_C(0x94D049BB133111EB);
+    return z ^ (z >> 31);
+}
+
+/* lattice/pair_graph.hpp:49-83: validate, count degrees, prefix sum with a
+ * uint32 overflow check, then place both directions in edge order. */
+int64_t orc_pair_graph(size_t n, size_t nedges, const uint32_t* ei, const uint32_t* ej,
+                       int symmetric, uint32_t* offsets, uint32_t* bonds)
+{
+    if (n > UINT32_MAX) return ORC_OVERFLOW;
+    uint64_t* degree = (uint64_t*)calloc(n ? n : 1, sizeof(uint64_t));
+    if (!degree) return ORC_OVERFLOW;
+    for (size_t e = 0; e < nedges; ++e) {
+        if (ei[e] >= n || ej[e] >= n) { free(degree); return ORC_OUT_OF_RANGE; }
+        if (ei[e] == ej[e]) { free(degree); return ORC_INVALID_ARGUMENT; }
+        ++degree[ei[e]];
+        if (symmetric) ++degree[ej[e]];
+    }
+    uint64_t total = 0;
+    for (size_t i = 0; i < n; ++i) {
+        if (offsets) offsets[i] = (uint32_t)total;
+        total += degree[i];
+        if (total > UINT32_MAX) { free(degree); return ORC_OVERFLOW; }
+    }
+    if (offsets) offsets[n] = (uint32_t)total;
+    if (bonds && offsets) {
+        uint32_t* pos = (uint32_t*)malloc((n ? n : 1) * sizeof(uint32_t));
+        memcpy(pos, offsets, n * sizeof(uint32_t));
+        for (size_t e = 0; e < nedges; ++e) {
+            bonds[pos[ei[e]]++] = ej[e];
+            if (symmetric) bonds[pos[ej[e]]++] = ei[e];
+        }
+        free(pos);
+    }
+    free(degree);
+    return (int64_t)total;
+}
+
+/* lattice/square.hpp:15-38 */
+int64_t orc_square_lattice(size_t Lx, size_t Ly, uint32_t* offsets, uint32_t* bonds)
+{
+    if (Lx == 0 || Ly == 0) return ORC_INVALID_ARGUMENT;
+    if (Lx > SIZE_MAX / Ly) return ORC_OVERFLOW;
+    const size_t n = Lx * Ly;
+    if (n > UINT32_MAX) return ORC_OVERFLOW;
+    uint32_t* ei = (uint32_t*)malloc(2 * n * sizeof(uint32_t) + 4);
+    uint32_t* ej = (uint32_t*)malloc(2 * n * sizeof(uint32_t) + 4);
+    size_t ne = 0;
+    for (size_t y = 0; y < Ly; ++y)
+        for (size_t x = 0; x < Lx; ++x) {
+            const uint32_t i = (uint32_t)(y * Lx + x);
+            if (Lx > 1) { ei[ne] = i; ej[ne] = (uint32_t)(y * Lx + (x + 1) % Lx); ++ne; }
+            if (Ly > 1) { ei[ne] = i; ej[ne] = (uint32_t)(((y + 1) % Ly) * Lx + x); ++ne; }
+        }
+    const int64_t r = orc_pair_graph(n, ne, ei, ej, 1, offsets, bonds);
+    free(ei);
+    free(ej);
+    return r;
+}
+
+int64_t orc_cubic_lattice(size_t L, uint32_t* offsets, uint32_t* bonds)
+{
+    if (L == 0) return ORC_INVALID_ARGUMENT;
+    const size_t n = L * L * L;
+    if (n > UINT32_MAX) return ORC_OVERFLOW;
+    uint32_t* ei = (uint32_t*)malloc(3 * n * sizeof(uint32_t) + 4);
+    uint32_t* ej = (uint32_t*)malloc(3 * n * sizeof(uint32_t) + 4);
+    size_t ne = 0;
+    for (size_t z = 0; z < L; ++z)
+        for (size_t y = 0; y < L; ++y)
+            for (size_t x = 0; x < L; ++x) {
+                const uint32_t i = (uint32_t)((z * L + y) * L + x);
+                if (L > 1) {
+                    ei[ne] = i; ej[ne] = (uint32_t)((z * L + y) * L + (x + 1) % L); ++ne;
+                    ei[ne] = i; ej[ne] = (uint32_t)((z * L + (y + 1) % L) * L + x); ++ne;
+                    ei[ne] = i; ej[ne] = (uint32_t)((((z + 1) % L) * L + y) * L + x); ++ne;
+                }
+            }
+    const int64_t r = orc_pair_graph(n, ne, ei, ej, 1, offsets, bonds);
+    free(ei);
+    free(ej);
+    return r;
+}
+
+/* lattice/hypercubic.hpp:15-64 (cell order mu < nu, then t, z, y, x; links
+ * (mu,c), (nu,c+mu), (mu,c+nu), (nu,c)) and cell_complex.hpp:36-63 (incidence
+ * rows in cell order, position `which` within the cell). */
+int64_t orc_hypercubic_plaquettes(size_t L, uint32_t* cells, uint64_t* inc_off,
+                                  uint32_t* inc_cell, uint8_t* inc_which)
+{
+    if (L == 0) return ORC_INVALID_ARGUMENT;
+    const size_t L4 = L * L * L * L;
+    const size_t sites = 4 * L4;
+    if (sites > UINT32_MAX) return ORC_OVERFLOW;
+    const size_t ncells = 6 * L4;
+    if (!cells) return (int64_t)ncells;
+    size_t c = 0;
+    for (size_t mu = 0; mu < 4; ++mu)
+        for (size_t nu = mu + 1; nu < 4; ++nu)
+            for (size_t t = 0; t < L; ++t)
+                for (size_t z = 0; z < L; ++z)
+                    for (size_t y = 0; y < L; ++y)
+                        for (size_t x = 0; x < L; ++x) {
+                            size_t co[4] = {t, z, y, x};
+                            size_t cm[4] = {t, z, y, x};
+                            size_t cn[4] = {t, z, y, x};
+                            cm[mu] = (cm[mu] + 1) % L;
+                            cn[nu] = (cn[nu] + 1) % L;
+#define ORC_IDX(m, v) ((m) * L4 + (((v)[0] * L + (v)[1]) * L + (v)[2]) * L + (v)[3])
+                            cells[4 * c + 0] = (uint32_t)ORC_IDX(mu, co);
+                            cells[4 * c + 1] = (uint32_t)ORC_IDX(nu, cm);
+                            cells[4 * c + 2] = (uint32_t)ORC_IDX(mu, cn);
+                            cells[4 * c + 3] = (uint32_t)ORC_IDX(nu, co);
+#undef ORC_IDX
+                            ++c;
+                        }
+    if (inc_off) {
+        memset(inc_off, 0, (sites + 1) * sizeof(uint64_t));
+        for (size_t k = 0; k < 4 * ncells; ++k) ++inc_off[cells[k] + 1];
+        for (size_t i = 0; i < sites; ++i) inc_off[i + 1] += inc_off[i];
+        uint64_t* cursor = (uint64_t*)malloc(sites * sizeof(uint64_t));
+        memcpy(cursor, inc_off, sites * sizeof(uint64_t));
+        for (size_t cc = 0; cc < ncells; ++cc)
+            for (size_t k = 0; k < 4; ++k) {
+                const size_t s = cells[4 * cc + k];
+                inc_cell[cursor[s]] = (uint32_t)cc;
+                inc_which[cursor[s]] = (uint8_t)k;
+                ++cursor[s];
+            }
+        free(cursor);
+    }
+    return (int64_t)ncells;
+}
+
+int64_t orc_edge_ids(size_t n, const uint32_t* off, const uint32_t* bonds, uint32_t* edge_id)
+{
+    const size_t total = off[n];
+    for (size_t k = 0; k < total; ++k) edge_id[k] = UINT32_MAX;
+    uint32_t next = 0;
+    for (size_t i = 0; i < n; ++i)
+        for (size_t a = off[i]; a < off[i + 1]; ++a) {
+            const uint32_t j = bonds[a];
+            if (j < i) continue;
+            if (edge_id[a] != UINT32_MAX) continue;
+            /* m-th copy of j in row i pairs with m-th copy of i in row j */
+            size_t m = 0;
+            for (size_t b = off[i]; b < a; ++b) m += (bonds[b] == j);
+            size_t seen = 0, twin = SIZE_MAX;
+            for (size_t b = off[j]; b < off[j + 1]; ++b)
+                if (bonds[b] == i) {
+                    if (seen == m) { twin = b; break; }
+                    ++seen;
+                }
+            if (twin == SIZE_MAX) return ORC_INVALID_ARGUMENT;
+            edge_id[a] = next;
+            edge_id[twin] = next;
+            ++next;
+        }
+    for (size_t k = 0; k < total; ++k)
+        if (edge_id[k] == UINT32_MAX) return ORC_INVALID_ARGUMENT;
+    return next;
+}
+
+int orc_greedy_colouring(size_t n, const uint32_t* off, const uint32_t* bonds, uint8_t* colour)
+{
+    int ncol = 0;
+    for (size_t i = 0; i < n; ++i) colour[i] = 255;
+    for (size_t i = 0; i < n; ++i) {
+        uint64_t used[4] = {0, 0, 0, 0};
+        for (size_t a = off[i]; a < off[i + 1]; ++a) {
+            const uint8_t c = colour[bonds[a]];
+            if (c != 255) used[c >> 6] |= UINT64_C(1) << (c & 63);
+        }
+        int c = 0;
+        while (used[c >> 6] & (UINT64_C(1) << (c & 63))) ++c;
+        colour[i] = (uint8_t)c;
+        if (c + 1 > ncol) ncol = c + 1;
+    }
+    return ncol;
+}

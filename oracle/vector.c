@@ -1,0 +1,310 @@
+/*
+ * O(N) vector-spin part of the oracle (test infrastructure only; see oracle.h).
+ * Restates dof/vector.hpp:12-122 (dot, UniformSphereN, LocalRotation),
+ * example/vector_spin.hpp:40-83 and driver.hpp:73-87 / 148-206.
+ */
+#define _GNU_SOURCE
+#include "oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_PI 3.14159265358979323846
+#define ORC_NMAX 8
+
+/* dof/vector.hpp:19-26 */
+static double vdot(int N, const double* a, const double* b)
+{
+    double s = 0.0;
+    for (int k = 0; k < N; ++k) s += a[k] * b[k];
+    return s;
+}
+
+/* example/vector_spin.hpp:40-47 */
+double orc_vec_energy_at(int N, const uint32_t* off, const uint32_t* bonds, const double* s,
+                         double J, size_t i)
+{
+    const double* si = s + i * (size_t)N;
+    double e = 0.0;
+    for (uint32_t a = off[i]; a < off[i + 1]; ++a)
+        e += -J * vdot(N, si, s + (size_t)bonds[a] * N);
+    return e;
+}
+
+/* example/vector_spin.hpp:50-60 */
+double orc_vec_energy_diff_at(int N, const uint32_t* off, const uint32_t* bonds,
+                              const double* s, double J, size_t i, const double* p)
+{
+    const double* old = s + i * (size_t)N;
+    double d = 0.0;
+    for (uint32_t a = off[i]; a < off[i + 1]; ++a) {
+        const double* sj = s + (size_t)bonds[a] * N;
+        d += -J * (vdot(N, p, sj) - vdot(N, old, sj));
+    }
+    return d;
+}
+
+/* example/vector_spin.hpp:63-69 */
+double orc_vec_energy_diff_bond(int N, const double* s, double J, size_t i, uint32_t j,
+                                const double* p)
+{
+    const double* fixed = s + i * (size_t)N;
+    const double* old = s + (size_t)j * N;
+    return -J * (vdot(N, p, fixed) - vdot(N, old, fixed));
+}
+
+/* dof/vector.hpp:35-61: N == 1 sign, N == 2 angle, N >= 3 height then recurse. */
+void orc_sphere_sample(int N, orc_pcg32* rng, double* out)
+{
+    if (N == 1) {
+        out[0] = orc_pcg32_uni(rng) < 0.5 ? -1.0 : 1.0;
+    } else if (N == 2) {
+        const double phi = 2.0 * ORC_PI * orc_pcg32_uni(rng);
+        out[0] = cos(phi);
+        out[1] = sin(phi);
+    } else {
+        const double z = 2.0 * orc_pcg32_uni(rng) - 1.0;
+        const double r = sqrt(1.0 - z * z);
+        orc_sphere_sample(N - 1, rng, out);
+        for (int k = 0; k < N - 1; ++k) out[k] *= r;
+        out[N - 1] = z;
+    }
+}
+
+/* dof/vector.hpp:79-118: uniform oriented 2-frame (u, v) by projection and
+ * rejection (eps = 64 DBL_EPSILON), angle delta (2 uni - 1), rotation in the
+ * (u, v) plane. */
+void orc_local_rotation(int N, double delta, const double* cur, orc_pcg32* rng, double* out)
+{
+    double u[ORC_NMAX], v[ORC_NMAX];
+    orc_sphere_sample(N, rng, u);
+    const double eps = 64.0 * DBL_EPSILON;
+    for (;;) {
+        orc_sphere_sample(N, rng, v);
+        double proj = 0.0;
+        for (int k = 0; k < N; ++k) proj += v[k] * u[k];
+        double r2 = 0.0;
+        for (int k = 0; k < N; ++k) {
+            v[k] -= proj * u[k];
+            r2 += v[k] * v[k];
+        }
+        if (r2 > eps) {
+            const double inv = 1.0 / sqrt(r2);
+            for (int k = 0; k < N; ++k) v[k] *= inv;
+            break;
+        }
+    }
+    const double th = delta * (2.0 * orc_pcg32_uni(rng) - 1.0);
+    const double c = cos(th);
+    const double sn = sin(th);
+    double cu = 0.0, cv = 0.0;
+    for (int k = 0; k < N; ++k) {
+        cu += u[k] * cur[k];
+        cv += v[k] * cur[k];
+    }
+    double tmp[ORC_NMAX];
+    for (int k = 0; k < N; ++k)
+        tmp[k] = cur[k] + (c - 1.0) * (cu * u[k] + cv * v[k]) + sn * (cu * v[k] - cv * u[k]);
+    memcpy(out, tmp, (size_t)N * sizeof(double));
+}
+
+static void propose(int N, int kind, double delta, const double* cur, orc_pcg32* rng,
+                    double* out)
+{
+    if (kind == 0) orc_sphere_sample(N, rng, out);
+    else orc_local_rotation(N, delta, cur, rng, out);
+}
+
+static double total_energy_vec(int N, size_t n, const uint32_t* off, const uint32_t* bonds,
+                               const double* s, double J)
+{
+    double sum = 0.0;
+    for (size_t i = 0; i < n; ++i) sum += orc_vec_energy_at(N, off, bonds, s, J, i);
+    return 0.5 * sum;
+}
+
+void orc_vec_metropolis_seq(int N, size_t n, const uint32_t* off, const uint32_t* bonds,
+                            double* s, double J, double beta, int prop_kind, double delta,
+                            uint64_t seed, uint64_t stream, size_t burn, size_t sweeps,
+                            double* e_out)
+{
+    orc_pcg32 rng;
+    orc_pcg32_init(&rng, seed, stream);
+    double cand[ORC_NMAX];
+    for (size_t k = 0; k < burn + sweeps; ++k) {
+        for (size_t i = 0; i < n; ++i) {
+            double* cur = s + i * (size_t)N;
+            propose(N, prop_kind, delta, cur, &rng, cand);
+            const double dE = orc_vec_energy_diff_at(N, off, bonds, s, J, i, cand);
+            const double log_p = -beta * dE + 0.0;
+            if (log_p >= 0.0 || log(orc_pcg32_uni(&rng)) < log_p)
+                memcpy(cur, cand, (size_t)N * sizeof(double));
+        }
+        if (k >= burn && e_out) e_out[k - burn] = total_energy_vec(N, n, off, bonds, s, J);
+    }
+}
+
+void orc_vec_metropolis_keyed(int N, size_t n, const uint32_t* off, const uint32_t* bonds,
+                              double* s, double J, double beta, int prop_kind, double delta,
+                              const uint8_t* colour, int ncolours, uint64_t run_seed,
+                              uint64_t sweep0, size_t nsweeps, uint64_t* accepts_out,
+                              double* trace_dE, uint8_t* trace_acc)
+{
+    double cand[ORC_NMAX];
+    for (size_t k = 0; k < nsweeps; ++k) {
+        const uint64_t key = orc_key(run_seed, sweep0 + k);
+        uint64_t acc = 0;
+        for (int c = 0; c < ncolours; ++c)
+            for (size_t i = 0; i < n; ++i) {
+                if (colour[i] != c) continue;
+                orc_pcg32 rng;
+                orc_pcg32_init(&rng, key, i);
+                double* cur = s + i * (size_t)N;
+                propose(N, prop_kind, delta, cur, &rng, cand);
+                const double dE = orc_vec_energy_diff_at(N, off, bonds, s, J, i, cand);
+                const double log_p = -beta * dE + 0.0;
+                const int a = (log_p >= 0.0 || log(orc_pcg32_uni(&rng)) < log_p);
+                if (a) {
+                    memcpy(cur, cand, (size_t)N * sizeof(double));
+                    ++acc;
+                }
+                if (trace_dE) trace_dE[k * n + i] = dE;
+                if (trace_acc) trace_acc[k * n + i] = (uint8_t)a;
+            }
+        if (accepts_out) accepts_out[k] = acc;
+    }
+}
+
+/* driver.hpp:148-206 with a UniformSphereN<N> axis. */
+static uint64_t vec_wolff_flip_seq(int N, size_t n, const uint32_t* off, const uint32_t* bonds,
+                                   double* s, double J, double beta, orc_pcg32* rng,
+                                   char* in_cluster, size_t* stack)
+{
+    double r[ORC_NMAX], prop[ORC_NMAX], nb[ORC_NMAX];
+    orc_sphere_sample(N, rng, r);
+    memset(in_cluster, 0, n);
+    const size_t seed = (size_t)(orc_pcg32_uni(rng) * (double)n);
+    in_cluster[seed] = 1;
+    stack[0] = seed;
+    size_t top = 1;
+    uint64_t cluster = 0;
+    while (top > 0) {
+        const size_t i = stack[--top];
+        ++cluster;
+        const double* si = s + i * (size_t)N;
+        double dot = 0.0;
+        for (int k = 0; k < N; ++k) dot += si[k] * r[k];
+        for (int k = 0; k < N; ++k) prop[k] = si[k] - 2.0 * dot * r[k];
+        for (uint32_t a = off[i]; a < off[i + 1]; ++a) {
+            const uint32_t j = bonds[a];
+            if (in_cluster[j]) continue;
+            const double* t = s + (size_t)j * N;
+            double d = 0.0;
+            for (int k = 0; k < N; ++k) d += t[k] * r[k];
+            for (int k = 0; k < N; ++k) nb[k] = t[k] - 2.0 * d * r[k];
+            const double dE = orc_vec_energy_diff_bond(N, s, J, i, j, nb);
+            if (orc_pcg32_uni(rng) < 1.0 - exp(-beta * dE)) {
+                in_cluster[j] = 1;
+                stack[top++] = j;
+            }
+        }
+        memcpy(s + i * (size_t)N, prop, (size_t)N * sizeof(double));
+    }
+    return cluster;
+}
+
+void orc_vec_wolff_seq(int N, size_t n, const uint32_t* off, const uint32_t* bonds, double* s,
+                       double J, double beta, uint64_t seed, uint64_t stream, size_t burn,
+                       size_t nflips, double* e_out, uint64_t* size_out)
+{
+    orc_pcg32 rng;
+    orc_pcg32_init(&rng, seed, stream);
+    char* in_cluster = (char*)malloc(n);
+    size_t* stack = (size_t*)malloc(n * sizeof(size_t));
+    for (size_t k = 0; k < burn; ++k)
+        vec_wolff_flip_seq(N, n, off, bonds, s, J, beta, &rng, in_cluster, stack);
+    for (size_t k = 0; k < nflips; ++k) {
+        const uint64_t c = vec_wolff_flip_seq(N, n, off, bonds, s, J, beta, &rng, in_cluster, stack);
+        if (size_out) size_out[k] = c;
+        if (e_out) e_out[k] = total_energy_vec(N, n, off, bonds, s, J);
+    }
+    free(in_cluster);
+    free(stack);
+}
+
+/* Keyed Wolff for O(N): axis from the control stream Pcg32(k_f, 0) (drawn
+ * before the seed site, as in driver.hpp:154-159); edge e is activated iff the
+ * first draw of Pcg32(k_f, 1 + e) is below 1 - exp(-beta dE) with
+ * dE = (2J) * ((s_i.r) (s_j.r)), which equals energy_diff_bond of the reflected
+ * neighbour (example/vector_spin.hpp:63-69) and is symmetric in i and j.  The
+ * reflection s - 2 (s.r) r is applied to the whole cluster at the end.  When
+ * store_f32 is set every stored component is rounded to float, as on the
+ * device. */
+static void vec_wolff_keyed_impl(int N, size_t n, const uint32_t* off, const uint32_t* bonds,
+                                 const uint32_t* edge_id, double* s, double J, double beta,
+                                 uint64_t run_seed, uint64_t flip0, size_t nflips,
+                                 uint64_t* size_out, uint8_t* last_cluster, int store_f32)
+{
+    uint8_t* in = (uint8_t*)calloc(n, 1);
+    uint32_t* queue = (uint32_t*)malloc(n * sizeof(uint32_t));
+    double r[ORC_NMAX];
+    for (size_t f = 0; f < nflips; ++f) {
+        const uint64_t key = orc_key(run_seed, ORC_WOLFF_COUNTER_BASE + flip0 + f);
+        orc_pcg32 ctl;
+        orc_pcg32_init(&ctl, key, 0);
+        orc_sphere_sample(N, &ctl, r);
+        const size_t seed = (size_t)(orc_pcg32_uni(&ctl) * (double)n);
+        memset(in, 0, n);
+        size_t head = 0, tail = 0;
+        in[seed] = 1;
+        queue[tail++] = (uint32_t)seed;
+        while (head < tail) {
+            const uint32_t i = queue[head++];
+            const double ai = vdot(N, s + (size_t)i * N, r);
+            for (uint32_t a = off[i]; a < off[i + 1]; ++a) {
+                const uint32_t j = bonds[a];
+                if (in[j]) continue;
+                const double aj = vdot(N, s + (size_t)j * N, r);
+                const double dE = (2.0 * J) * (ai * aj);
+                orc_pcg32 er;
+                orc_pcg32_init(&er, key, 1 + (uint64_t)edge_id[a]);
+                if (orc_pcg32_uni(&er) < 1.0 - exp(-beta * dE)) {
+                    in[j] = 1;
+                    queue[tail++] = j;
+                }
+            }
+        }
+        for (size_t q = 0; q < tail; ++q) {
+            double* si = s + (size_t)queue[q] * N;
+            const double d = vdot(N, si, r);
+            for (int k = 0; k < N; ++k) {
+                const double v = si[k] - 2.0 * d * r[k];
+                si[k] = store_f32 ? (double)(float)v : v;
+            }
+        }
+        if (size_out) size_out[f] = tail;
+        if (last_cluster && f + 1 == nflips) memcpy(last_cluster, in, n);
+    }
+    free(in);
+    free(queue);
+}
+
+void orc_vec_wolff_keyed(int N, size_t n, const uint32_t* off, const uint32_t* bonds,
+                         const uint32_t* edge_id, double* s, double J, double beta,
+                         uint64_t run_seed, uint64_t flip0, size_t nflips,
+                         uint64_t* size_out, uint8_t* last_cluster)
+{
+    vec_wolff_keyed_impl(N, n, off, bonds, edge_id, s, J, beta, run_seed, flip0, nflips,
+                         size_out, last_cluster, 0);
+}
+
+void orc_vec_wolff_keyed_f32(int N, size_t n, const uint32_t* off, const uint32_t* bonds,
+                             const uint32_t* edge_id, double* s, double J, double beta,
+                             uint64_t run_seed, uint64_t flip0, size_t nflips,
+                             uint64_t* size_out, uint8_t* last_cluster)
+{
+    vec_wolff_keyed_impl(N, n, off, bonds, edge_id, s, J, beta, run_seed, flip0, nflips,
+                         size_out, last_cluster, 1);
+}

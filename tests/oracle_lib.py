@@ -1,0 +1,264 @@
+"""ctypes bindings of the oracle (oracle/_build/liboracle.so) and, when it was
+built, of the compiled reference (oracle/_ref/libucmc_ref.so).
+
+Test infrastructure: only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg import this module, and only as the checker.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libucmc_ref.so")
+
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+i8p = np.ctypeslib.ndpointer(np.int8, flags="C_CONTIGUOUS")
+u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+class _Opt:
+    """Nullable ndpointer: accepts None or an array of the given dtype."""
+
+    def __init__(self, dtype):
+        self.base = np.ctypeslib.ndpointer(dtype, flags="C_CONTIGUOUS")
+
+    def from_param(self, obj):
+        if obj is None:
+            return None
+        return self.base.from_param(obj)
+
+
+ou8, ou32, ou64, of64 = _Opt(np.uint8), _Opt(np.uint32), _Opt(np.uint64), _Opt(np.float64)
+sz, u64, i64, i32, f64, cint = C.c_size_t, C.c_uint64, C.c_int64, C.c_int32, C.c_double, C.c_int
+
+
+def _sig(lib, name, res, *args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = list(args)
+    return f
+
+
+_oracle = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is not None:
+        return _oracle
+    if not os.path.exists(ORACLE_SO):
+        raise RuntimeError(f"oracle not built: {ORACLE_SO} (run make -C oracle)")
+    L = C.CDLL(ORACLE_SO)
+    _sig(L, "orc_pcg32_block", None, u64, u64, sz, u32p)
+    _sig(L, "orc_key", u64, u64, u64)
+    _sig(L, "orc_pair_graph", i64, sz, sz, u32p, u32p, cint, ou32, ou32)
+    _sig(L, "orc_square_lattice", i64, sz, sz, ou32, ou32)
+    _sig(L, "orc_cubic_lattice", i64, sz, ou32, ou32)
+    _sig(L, "orc_hypercubic_plaquettes", i64, sz, ou32, ou64, ou32, ou8)
+    _sig(L, "orc_edge_ids", i64, sz, u32p, u32p, u32p)
+    _sig(L, "orc_greedy_colouring", cint, sz, u32p, u32p, u8p)
+    _sig(L, "orc_ising_energy_at", i32, u32p, u32p, i8p, i32, sz)
+    _sig(L, "orc_ising_energy_diff_at", i32, u32p, u32p, i8p, i32, sz, C.c_int8)
+    _sig(L, "orc_ising_energy_diff_bond", i32, i8p, i32, sz, C.c_uint32, C.c_int8)
+    _sig(L, "orc_accept_threshold", u64, f64, f64)
+    _sig(L, "orc_wolff_threshold", u64, f64, f64)
+    _sig(L, "orc_ising_metropolis_seq", None, sz, u32p, u32p, i8p, i32, f64, u64, u64, sz, sz,
+         of64, of64, ou64)
+    _sig(L, "orc_ising_metropolis_keyed", None, sz, u32p, u32p, i8p, i32, f64, u8p, cint, u64,
+         u64, sz, ou64)
+    _sig(L, "orc_ising_wolff_seq", None, sz, u32p, u32p, i8p, i32, f64, u64, u64, sz, sz, of64,
+         ou64)
+    _sig(L, "orc_ising_wolff_keyed", None, sz, u32p, u32p, u32p, i8p, i32, f64, u64, u64, sz,
+         ou64, ou8)
+    _sig(L, "orc_vec_energy_at", f64, cint, u32p, u32p, f64p, f64, sz)
+    _sig(L, "orc_vec_energy_diff_at", f64, cint, u32p, u32p, f64p, f64, sz, f64p)
+    _sig(L, "orc_vec_metropolis_seq", None, cint, sz, u32p, u32p, f64p, f64, f64, cint, f64, u64,
+         u64, sz, sz, of64)
+    _sig(L, "orc_vec_metropolis_keyed", None, cint, sz, u32p, u32p, f64p, f64, f64, cint, f64,
+         u8p, cint, u64, u64, sz, ou64, of64, ou8)
+    _sig(L, "orc_vec_wolff_seq", None, cint, sz, u32p, u32p, f64p, f64, f64, u64, u64, sz, sz,
+         of64, ou64)
+    _sig(L, "orc_vec_wolff_keyed", None, cint, sz, u32p, u32p, u32p, f64p, f64, f64, u64, u64, sz,
+         ou64, ou8)
+    _sig(L, "orc_vec_wolff_keyed_f32", None, cint, sz, u32p, u32p, u32p, f64p, f64, f64, u64, u64,
+         sz, ou64, ou8)
+    _sig(L, "orc_u1_energy_diff_at", f64, u32p, u64p, u32p, u8p, f64p, f64, sz, f64)
+    _sig(L, "orc_u1_mean_plaquette", f64, sz, u32p, f64p)
+    _sig(L, "orc_u1_metropolis_seq", None, sz, f64p, f64, f64, f64, u64, u64, sz, sz, of64)
+    _sig(L, "orc_u1_metropolis_keyed", None, sz, f64p, f64, f64, f64, u64, u64, sz, ou64, of64)
+    _sig(L, "orc_sun_mul", None, cint, f64p, f64p, f64p)
+    _sig(L, "orc_sun_plaquette", f64, cint, f64p, f64p, f64p, f64p)
+    _sig(L, "orc_sun_expm", None, cint, f64p, f64p, C.POINTER(cint), C.POINTER(cint))
+    _sig(L, "orc_sun_unitarity_error", f64, cint, f64p)
+    _sig(L, "orc_sun_energy_diff_at", f64, cint, u32p, u64p, u32p, u8p, f64p, f64, sz, f64p)
+    _sig(L, "orc_sun_mean_plaquette", f64, cint, sz, u32p, f64p)
+    _sig(L, "orc_sun_metropolis_seq", None, cint, sz, f64p, f64, f64, f64, u64, u64, sz, sz, of64)
+    _sig(L, "orc_sun_metropolis_keyed", None, cint, sz, f64p, f64, f64, f64, cint, u64, u64, sz,
+         ou64, of64)
+    _sig(L, "orc_su3_reunitarize", None, f64p)
+    _sig(L, "orc_xy_energy_diff_at", f64, u32p, u32p, f64p, f64, sz, f64)
+    _sig(L, "orc_xy_metropolis_keyed", None, sz, u32p, u32p, f64p, f64, f64, f64, u8p, cint, u64,
+         u64, sz, ou64, of64)
+    _sig(L, "orc_fpi_energy", f64, sz, i8p, f64, f64)
+    _sig(L, "orc_fpi_energy_diff_flip", f64, sz, i8p, f64, f64, sz)
+    _sig(L, "orc_fpi_metropolis_keyed", None, sz, i8p, f64, f64, f64, u64, u64, sz, ou64)
+    # helpers that take an orc_pcg32* are wrapped through a small struct
+    L.orc_pcg32_init.restype = None
+    L.orc_pcg32_init.argtypes = [C.c_void_p, u64, u64]
+    L.orc_sphere_sample.restype = None
+    L.orc_sphere_sample.argtypes = [cint, C.c_void_p, f64p]
+    L.orc_local_rotation.restype = None
+    L.orc_local_rotation.argtypes = [cint, f64, f64p, C.c_void_p, f64p]
+    L.orc_u1_shift.restype = f64
+    L.orc_u1_shift.argtypes = [f64, f64, C.c_void_p]
+    L.orc_sun_shift.restype = None
+    L.orc_sun_shift.argtypes = [cint, f64, f64p, C.c_void_p, f64p]
+    L.orc_pcg32_uni.restype = f64
+    L.orc_pcg32_uni.argtypes = [C.c_void_p]
+    _oracle = L
+    return L
+
+
+class Pcg32:
+    """Oracle Pcg32 state usable with the oracle's proposal functions."""
+
+    def __init__(self, seed, stream=0):
+        self.buf = (C.c_uint64 * 2)()
+        oracle().orc_pcg32_init(self.buf, seed, stream)
+
+    @property
+    def ptr(self):
+        return C.cast(self.buf, C.c_void_p)
+
+    def uni(self):
+        return oracle().orc_pcg32_uni(self.ptr)
+
+
+_ref = None
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is not None:
+        return _ref
+    L = C.CDLL(REF_SO)
+    _sig(L, "ref_pcg32_block", None, u64, u64, sz, f64p)
+    _sig(L, "ref_pair_graph", i64, sz, sz, u32p, u32p, cint, u32p, u32p)
+    _sig(L, "ref_square_lattice", i64, sz, sz, ou32, ou32)
+    _sig(L, "ref_hypercubic", i64, sz, u32p, u64p, u32p, u8p)
+    _sig(L, "ref_ising_eval", None, sz, sz, u32p, u32p, i8p, i32, i32p, i32p)
+    _sig(L, "ref_ising_metropolis_seq", None, sz, sz, u32p, u32p, i8p, i32, f64, u64, u64, sz, sz,
+         of64, of64, ou64)
+    _sig(L, "ref_ising_keyed", None, sz, sz, u32p, u32p, i8p, i32, f64, u8p, cint, u64, u64, sz,
+         ou64)
+    _sig(L, "ref_ising_wolff_seq", None, sz, sz, u32p, u32p, i8p, i32, f64, u64, u64, sz, sz,
+         of64, ou64, of64)
+    _sig(L, "ref_vec_metropolis_seq", None, cint, sz, sz, u32p, u32p, f64p, f64, f64, cint, f64,
+         u64, u64, sz, sz, of64)
+    _sig(L, "ref_vec_keyed", None, cint, sz, sz, u32p, u32p, f64p, f64, f64, cint, f64, u8p, cint,
+         u64, u64, sz, ou64, of64)
+    _sig(L, "ref_vec_wolff_seq", None, cint, sz, sz, u32p, u32p, f64p, f64, f64, u64, u64, sz, sz,
+         of64, ou64)
+    _sig(L, "ref_vec_eval", None, cint, sz, sz, u32p, u32p, f64p, f64, f64p, f64p, f64p)
+    _sig(L, "ref_sphere", None, cint, u64, u64, sz, f64p)
+    _sig(L, "ref_local_rotation", None, cint, f64, f64p, u64, u64, sz, f64p)
+    _sig(L, "ref_u1_shift", None, f64, f64, u64, u64, sz, f64p)
+    _sig(L, "ref_u1_eval", None, sz, f64, f64p, f64p, f64p, f64p)
+    _sig(L, "ref_u1_metropolis_seq", None, sz, f64p, f64, f64, f64, u64, u64, sz, sz, f64p)
+    _sig(L, "ref_u1_keyed", None, sz, f64p, f64, f64, f64, u64, u64, sz, ou64, of64)
+    _sig(L, "ref_sun_shift", None, cint, f64, f64p, u64, u64, sz, f64p)
+    _sig(L, "ref_sun_expm", None, cint, f64p, f64p)
+    _sig(L, "ref_sun_eval", None, cint, sz, f64, f64p, f64p, f64p, f64p, f64p)
+    _sig(L, "ref_sun_metropolis_seq", None, cint, sz, f64p, f64, f64, f64, u64, u64, sz, sz, f64p)
+    _sig(L, "ref_sun_keyed", None, cint, sz, f64p, f64, f64, f64, cint, u64, u64, sz, ou64)
+    _ref = L
+    return L
+
+
+# ----------------------------------------------------------------- helpers --
+
+def square_edges(Lx, Ly):
+    """Edge list of square_lattice(Lx, Ly) (lattice/square.hpp:30-38), in emission
+    order: per site (y-major), the (x+1) edge then the (y+1) edge."""
+    idx = np.arange(Lx * Ly, dtype=np.int64)
+    x, y = idx % Lx, idx // Lx
+    cols = []
+    if Lx > 1:
+        cols.append((idx, y * Lx + (x + 1) % Lx))
+    if Ly > 1:
+        cols.append((idx, ((y + 1) % Ly) * Lx + x))
+    if not cols:
+        return np.zeros(0, np.uint32), np.zeros(0, np.uint32)
+    ei = np.stack([c[0] for c in cols], axis=1).reshape(-1)
+    ej = np.stack([c[1] for c in cols], axis=1).reshape(-1)
+    return ei.astype(np.uint32), ej.astype(np.uint32)
+
+
+def cubic_edges(L):
+    ei, ej = [], []
+    for z in range(L):
+        for y in range(L):
+            for x in range(L):
+                i = (z * L + y) * L + x
+                if L > 1:
+                    ei += [i, i, i]
+                    ej += [(z * L + y) * L + (x + 1) % L, (z * L + (y + 1) % L) * L + x,
+                           (((z + 1) % L) * L + y) * L + x]
+    return np.array(ei, np.uint32), np.array(ej, np.uint32)
+
+
+def csr_square(Lx, Ly):
+    o = oracle()
+    n = Lx * Ly
+    total = o.orc_square_lattice(Lx, Ly, None, None)
+    off = np.zeros(n + 1, np.uint32)
+    bonds = np.zeros(max(total, 1), np.uint32)
+    o.orc_square_lattice(Lx, Ly, off, bonds)
+    return off, bonds
+
+
+def csr_cubic(L):
+    o = oracle()
+    n = L ** 3
+    total = o.orc_cubic_lattice(L, None, None)
+    off = np.zeros(n + 1, np.uint32)
+    bonds = np.zeros(max(total, 1), np.uint32)
+    o.orc_cubic_lattice(L, off, bonds)
+    return off, bonds
+
+
+def hypercubic(L):
+    o = oracle()
+    nc = o.orc_hypercubic_plaquettes(L, None, None, None, None)
+    sites = 4 * L ** 4
+    cells = np.zeros(4 * nc, np.uint32)
+    inc_off = np.zeros(sites + 1, np.uint64)
+    inc_cell = np.zeros(4 * nc, np.uint32)
+    inc_which = np.zeros(4 * nc, np.uint8)
+    o.orc_hypercubic_plaquettes(L, cells, inc_off, inc_cell, inc_which)
+    return cells, inc_off, inc_cell, inc_which
+
+
+def checkerboard(shape):
+    """Colour (sum of coordinates) & 1 for a lattice with site id
+    ((z*Ly + y)*Lx + x); shape = (Lx, Ly[, Lz])."""
+    idx = np.indices(tuple(reversed(shape))).sum(axis=0) & 1
+    return np.ascontiguousarray(idx.reshape(-1).astype(np.uint8))
+
+
+def random_spins(n, seed, stream=0):
+    """s_i = (uni < 0.5 ? -1 : 1) from one Pcg32(seed, stream), in site order."""
+    out = np.empty(n, np.uint32)
+    oracle().orc_pcg32_block(seed, stream, n, out)
+    return np.where(out.astype(np.float64) * 2.0 ** -32 < 0.5, -1, 1).astype(np.int8)

@@ -1,0 +1,152 @@
+"""GPU parity of VectorSpin<N> Metropolis (example/vector_spin.hpp +
+UniformSphereN / LocalRotation, driver.hpp:73-87) against the oracle's keyed
+colour-order replay.
+
+The device computes in fp32 (field, proposal, dE) and decides in double; the
+oracle replays the reference's double arithmetic on the same fp32 field.
+Tolerance (north_star): float dE within 1e-5 relative, with an absolute floor
+of 1e-5 for |dE| < 1; spins within 2e-6 per component."""
+import numpy as np
+import pytest
+
+from oracle_lib import (Pcg32, checkerboard, csr_cubic, csr_square, oracle)
+from universal_classical_montecarlo_amd import abi
+
+pytestmark = pytest.mark.gpu
+
+DE_RTOL = 1e-5
+
+
+def random_unit(N, n, seed):
+    """UniformSphereN<N> samples from one Pcg32(seed, 7), rounded to fp32."""
+    o = oracle()
+    rng = Pcg32(seed, 7)
+    out = np.zeros(n * N)
+    v = np.zeros(N)
+    for i in range(n):
+        o.orc_sphere_sample(N, rng.ptr, v)
+        out[i * N:(i + 1) * N] = v
+    return out.astype(np.float32).astype(np.float64)
+
+
+def replay(N, off, bonds, spins, J, beta, kind, delta, colour, run_seed, sweep0, nsweeps):
+    s = spins.copy()
+    n = len(s) // N
+    acc = np.zeros(nsweeps, np.uint64)
+    tr = np.zeros(nsweeps * n)
+    tacc = np.zeros(nsweeps * n, np.uint8)
+    oracle().orc_vec_metropolis_keyed(N, n, off, bonds, s, J, beta, kind, delta, colour,
+                                      int(colour.max()) + 1, run_seed, sweep0, nsweeps, acc, tr,
+                                      tacc)
+    return s, acc, tr, tacc
+
+
+def assert_dE_close(got, want):
+    err = np.abs(got.astype(np.float64) - want)
+    tol = DE_RTOL * np.maximum(np.abs(want), 1.0)
+    assert np.all(err <= tol), f"max dE error {err.max()} (tolerance {DE_RTOL} relative)"
+
+
+CASES = [
+    # N, geometry, proposal, width, beta
+    (3, ("cubic", 8), abi.PROP_ROTATION, 0.5, 0.693),
+    (3, ("cubic", 6), abi.PROP_SPHERE, 0.0, 0.3),
+    (2, ("square", 16), abi.PROP_ROTATION, 1.0, 1.1),
+    (2, ("square", 12), abi.PROP_SPHERE, 0.0, 0.5),
+    (4, ("square", 8), abi.PROP_ROTATION, 0.3, 0.7),
+]
+
+
+def graph(geom):
+    kind, L = geom
+    return csr_cubic(L) if kind == "cubic" else csr_square(L, L)
+
+
+@pytest.mark.parametrize("N,geom,prop,width,beta", CASES)
+def test_vector_trace_one_sweep(gpu, N, geom, prop, width, beta):
+    print(f"vector N={N} {geom} prop={prop}", flush=True)
+    off, bonds = graph(geom)
+    n = len(off) - 1
+    sysm = abi.System.vector(N, off, bonds, 1.0)
+    colour = sysm.colouring()
+    spins = random_unit(N, n, 5 + N)
+    sysm.upload(spins)
+    dE, acc = sysm.trace(beta, 31, sweep0=2, proposal=prop, width=width)
+    kind = 0 if prop == abi.PROP_SPHERE else 1
+    want, wacc, tr, tacc = replay(N, off, bonds, spins, 1.0, beta, kind, width, colour, 31, 2, 1)
+    assert_dE_close(dE, tr)
+    assert np.array_equal(acc, tacc), f"{int((acc != tacc).sum())} decisions differ"
+    got = sysm.download()
+    assert np.max(np.abs(got - want)) < 2e-6
+    sysm.close()
+
+
+@pytest.mark.parametrize("N,geom,prop,width,beta", CASES[:3])
+def test_vector_multi_sweep(gpu, N, geom, prop, width, beta):
+    off, bonds = graph(geom)
+    n = len(off) - 1
+    sysm = abi.System.vector(N, off, bonds, 1.0)
+    colour = sysm.colouring()
+    spins = random_unit(N, n, 17)
+    sysm.upload(spins)
+    ser = sysm.metropolis(beta, 3, 8, sweep0=0, proposal=prop, width=width,
+                          flags=abi.RECORD_ACCEPTS | abi.RECORD_ENERGY | abi.RECORD_ORDER,
+                          record=True)
+    kind = 0 if prop == abi.PROP_SPHERE else 1
+    want, wacc, _, _ = replay(N, off, bonds, spins, 1.0, beta, kind, width, colour, 8, 0, 3)
+    got = sysm.download()
+    assert np.array_equal(np.array([o.accepted for o in ser], np.uint64), wacc)
+    assert np.max(np.abs(got - want)) < 1e-5
+    # device energy of the final field vs the reference's total energy
+    o = oracle()
+    e = 0.5 * sum(o.orc_vec_energy_at(N, off, bonds, got, 1.0, i) for i in range(n))
+    assert abs(ser[-1].energy - e) <= 1e-9 * max(1.0, abs(e))
+    m = got.reshape(n, N).sum(axis=0)
+    assert np.allclose([ser[-1].order[k] for k in range(N)], m, atol=1e-9)
+    sysm.close()
+
+
+def test_vector_general_graph(gpu):
+    rng = np.random.default_rng(2)
+    n = 200
+    ei = rng.integers(0, n, 500).astype(np.uint32)
+    ej = rng.integers(0, n, 500).astype(np.uint32)
+    keep = ei != ej
+    ei, ej = ei[keep], ej[keep]
+    o = oracle()
+    off = np.zeros(n + 1, np.uint32)
+    total = o.orc_pair_graph(n, len(ei), ei, ej, 1, off, None)
+    bonds = np.zeros(total, np.uint32)
+    o.orc_pair_graph(n, len(ei), ei, ej, 1, off, bonds)
+    sysm = abi.System.vector(3, off, bonds, 0.8)
+    assert sysm.info.schedule == abi.SCHED_CSR
+    colour = sysm.colouring()
+    spins = random_unit(3, n, 3)
+    sysm.upload(spins)
+    dE, acc = sysm.trace(0.9, 4, proposal=abi.PROP_ROTATION, width=0.7)
+    want, _, tr, tacc = replay(3, off, bonds, spins, 0.8, 0.9, 1, 0.7, colour, 4, 0, 1)
+    assert_dE_close(dE, tr)
+    assert np.array_equal(acc, tacc)
+    sysm.close()
+
+
+def test_heisenberg_128_full_size(gpu):
+    """BASELINE config size: norm preservation and device vs host reductions."""
+    L = 128
+    off, bonds = csr_cubic(L)
+    sysm = abi.System.vector(3, off, bonds, 1.0)
+    assert sysm.info.schedule == abi.SCHED_CUBIC
+    rng = np.random.default_rng(5)
+    v = rng.normal(size=(L ** 3, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    sysm.upload(v.reshape(-1))
+    ser = sysm.metropolis(0.693, 4, 1, proposal=abi.PROP_ROTATION, width=0.5,
+                          flags=abi.RECORD_ENERGY | abi.RECORD_ACCEPTS, record=True)
+    got = sysm.download().reshape(L, L, L, 3)
+    norms = np.linalg.norm(got, axis=3)
+    assert np.max(np.abs(norms - 1.0)) < 1e-6
+    e = -sum((got * np.roll(got, -1, ax)).sum() for ax in (0, 1, 2))
+    assert abs(ser[-1].energy - e) < 1e-9 * abs(e)
+    acc = ser[-1].accepted / L ** 3
+    assert 0.2 < acc < 0.99
+    sysm.close()

@@ -1,0 +1,81 @@
+"""GPU parity of Wolff cluster flips (driver.hpp:136-254) against the oracle's
+keyed replay: the cluster is the connected component of the seed in the graph
+of bonds activated by their own Pcg32(k_f, 1 + e) draw.  Ising: bit-exact
+spins and cluster sizes.  O(N): identical cluster sizes, spins within 2e-6."""
+import numpy as np
+import pytest
+
+from oracle_lib import csr_cubic, csr_square, oracle, random_spins
+from test_gpu_vector import random_unit
+from universal_classical_montecarlo_amd import abi
+
+pytestmark = pytest.mark.gpu
+
+
+def edge_ids(off, bonds):
+    n = len(off) - 1
+    e = np.zeros(len(bonds), np.uint32)
+    assert oracle().orc_edge_ids(n, off, bonds, e) > 0
+    return e
+
+
+@pytest.mark.parametrize("L,beta,J", [(16, 0.44, 1), (8, 0.3, 1), (32, 0.2, -1), (64, 0.45, 2)])
+def test_ising_wolff_bit_exact(gpu, L, beta, J):
+    print(f"ising wolff L={L} beta={beta}", flush=True)
+    off, bonds = csr_square(L, L)
+    sysm = abi.System.ising(off, bonds, J)
+    spins = random_spins(L * L, 19, 2)
+    sysm.upload(spins)
+    nflips = 300
+    sizes, ser = sysm.wolff(beta, nflips, 55, flip0=10, record=True)
+    got = sysm.download()
+    want = spins.copy()
+    wsz = np.zeros(nflips, np.uint64)
+    oracle().orc_ising_wolff_keyed(L * L, off, bonds, edge_ids(off, bonds), want, J, beta, 55, 10,
+                                   nflips, wsz, None)
+    assert np.array_equal(sizes, wsz)
+    assert np.array_equal(got, want)
+    o = sysm.observe()
+    assert ser[-1].energy == o.energy
+    assert ser[-1].order[0] == o.order[0]
+    sysm.close()
+
+
+def test_ising_wolff_dims_created_4096(gpu):
+    L = 4096
+    off, bonds = csr_square(L, L)
+    sysm = abi.System.ising_square(L, L, 1)
+    spins = random_spins(L * L, 3, 0)
+    sysm.upload(spins)
+    sizes, _ = sysm.wolff(0.44, 4, 9)
+    got = sysm.download()
+    want = spins.copy()
+    wsz = np.zeros(4, np.uint64)
+    oracle().orc_ising_wolff_keyed(L * L, off, bonds, edge_ids(off, bonds), want, 1, 0.44, 9, 0, 4,
+                                   wsz, None)
+    assert np.array_equal(sizes, wsz)
+    assert np.array_equal(got, want)
+    sysm.close()
+
+
+@pytest.mark.parametrize("N,geom,beta", [(3, ("cubic", 8), 0.693), (2, ("square", 16), 1.0),
+                                         (3, ("square", 8), 0.3)])
+def test_vector_wolff(gpu, N, geom, beta):
+    kind, L = geom
+    off, bonds = csr_cubic(L) if kind == "cubic" else csr_square(L, L)
+    n = len(off) - 1
+    sysm = abi.System.vector(N, off, bonds, 1.0)
+    spins = random_unit(N, n, 23)
+    sysm.upload(spins)
+    nflips = 200
+    sizes, ser = sysm.wolff(beta, nflips, 77, record=True)
+    got = sysm.download()
+    want = spins.copy()
+    wsz = np.zeros(nflips, np.uint64)
+    oracle().orc_vec_wolff_keyed_f32(N, n, off, bonds, edge_ids(off, bonds), want, 1.0, beta, 77,
+                                     0, nflips, wsz, None)
+    assert np.array_equal(sizes, wsz)
+    assert np.max(np.abs(got - want)) < 2e-6
+    o = sysm.observe()
+    assert abs(ser[-1].energy - o.energy) < 1e-5 * max(1.0, abs(o.energy))
+    sysm.close()

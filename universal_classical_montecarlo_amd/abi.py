@@ -1,0 +1,265 @@
+"""ctypes binding of the C ABI in include/ucmc.h (libucmc.so, built in-tree).
+
+This is plumbing for Python callers (the tests and bench.py); the C++23 host
+layer in include/ucmc/gpu.hpp is the drop-in for the reference's drivers.
+Loading fails loudly when the HIP library has not been built: there is no CPU
+fallback anywhere in the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libucmc.so")
+
+UCMC_OK = 0
+UCMC_E_INVALID_ARGUMENT = -1
+UCMC_E_OUT_OF_RANGE = -2
+UCMC_E_OVERFLOW = -3
+UCMC_E_UNSUPPORTED = -4
+UCMC_E_DEVICE = -5
+UCMC_E_NO_DEVICE = -6
+
+ISING, VECTOR, U1_GAUGE, SUN_GAUGE, XY_ANGLE, PLAQ_ISING = 1, 2, 3, 4, 5, 6
+PROP_FLIP, PROP_SPHERE, PROP_ROTATION, PROP_U1_SHIFT, PROP_SU_SHIFT = 1, 2, 3, 4, 5
+SCHED_CSR, SCHED_CHECKERBOARD, SCHED_FUSED, SCHED_CUBIC, SCHED_LINK8, SCHED_FOUR = 1, 2, 3, 4, 5, 6
+RECORD_ENERGY, RECORD_ORDER, RECORD_ACCEPTS, FORCE_SIMPLE = 1, 2, 4, 8
+
+# Every function the header declares (checked by the CPU test suite).
+EXPORTED = [
+    "ucmc_abi_version", "ucmc_last_error", "ucmc_device_count", "ucmc_key",
+    "ucmc_ising_create", "ucmc_ising_square_create", "ucmc_vector_create", "ucmc_u1_gauge_create",
+    "ucmc_sun_gauge_create", "ucmc_xy_angle_create", "ucmc_plaquette_ising_create",
+    "ucmc_system_destroy", "ucmc_system_info", "ucmc_colouring", "ucmc_field_upload",
+    "ucmc_field_download", "ucmc_metropolis_run", "ucmc_metropolis_trace", "ucmc_wolff_run",
+    "ucmc_observe", "ucmc_metropolis_bench",
+]
+
+
+class PairGraph(C.Structure):
+    _fields_ = [("sites", C.c_size_t), ("offsets", C.POINTER(C.c_uint32)),
+                ("bonds", C.POINTER(C.c_uint32))]
+
+
+class Info(C.Structure):
+    _fields_ = [("kind", C.c_int), ("components", C.c_int), ("sites", C.c_size_t),
+                ("schedule", C.c_int), ("ncolours", C.c_int), ("dims", C.c_size_t * 4),
+                ("device_bytes", C.c_size_t)]
+
+
+class Observables(C.Structure):
+    _fields_ = [("energy", C.c_double), ("order", C.c_double * 4), ("plaquette", C.c_double),
+                ("accepted", C.c_uint64), ("cluster", C.c_uint64)]
+
+
+class MetropolisParams(C.Structure):
+    _fields_ = [("beta", C.c_double), ("proposal", C.c_int), ("width", C.c_double),
+                ("hits", C.c_int), ("run_seed", C.c_uint64), ("sweep0", C.c_uint64),
+                ("nsweeps", C.c_uint64), ("flags", C.c_uint32)]
+
+
+class WolffParams(C.Structure):
+    _fields_ = [("beta", C.c_double), ("run_seed", C.c_uint64), ("flip0", C.c_uint64),
+                ("nflips", C.c_uint64), ("flags", C.c_uint32)]
+
+
+class UcmcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"ucmc error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load libucmc.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"HIP library not built: {LIB_PATH}; run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.ucmc_abi_version.restype = C.c_int
+    L.ucmc_last_error.restype = C.c_char_p
+    L.ucmc_device_count.argtypes = [C.POINTER(C.c_int)]
+    L.ucmc_key.restype = C.c_uint64
+    L.ucmc_key.argtypes = [C.c_uint64, C.c_uint64]
+    L.ucmc_ising_create.argtypes = [C.POINTER(PairGraph), C.c_int32, C.c_int, C.POINTER(vp)]
+    L.ucmc_ising_square_create.argtypes = [C.c_size_t, C.c_size_t, C.c_int32, C.c_int,
+                                           C.POINTER(vp)]
+    L.ucmc_vector_create.argtypes = [C.c_int, C.POINTER(PairGraph), C.c_double, C.c_int,
+                                     C.POINTER(vp)]
+    L.ucmc_u1_gauge_create.argtypes = [C.c_size_t, C.c_double, C.c_int, C.POINTER(vp)]
+    L.ucmc_sun_gauge_create.argtypes = [C.c_int, C.c_size_t, C.c_double, C.c_int, C.POINTER(vp)]
+    L.ucmc_xy_angle_create.argtypes = [C.POINTER(PairGraph), C.c_double, C.c_int, C.POINTER(vp)]
+    L.ucmc_plaquette_ising_create.argtypes = [C.c_size_t, C.c_double, C.c_double, C.c_int,
+                                              C.POINTER(vp)]
+    L.ucmc_system_destroy.argtypes = [vp]
+    L.ucmc_system_info.argtypes = [vp, C.POINTER(Info)]
+    L.ucmc_colouring.argtypes = [vp, C.c_void_p]
+    L.ucmc_field_upload.argtypes = [vp, C.c_void_p, C.c_size_t]
+    L.ucmc_field_download.argtypes = [vp, C.c_void_p, C.c_size_t]
+    L.ucmc_metropolis_run.argtypes = [vp, C.POINTER(MetropolisParams), C.c_void_p]
+    L.ucmc_metropolis_trace.argtypes = [vp, C.POINTER(MetropolisParams), C.c_void_p, C.c_void_p]
+    L.ucmc_wolff_run.argtypes = [vp, C.POINTER(WolffParams), C.c_void_p, C.c_void_p]
+    L.ucmc_observe.argtypes = [vp, C.POINTER(Observables)]
+    L.ucmc_metropolis_bench.argtypes = [vp, C.POINTER(MetropolisParams), C.c_int,
+                                        C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
+    _lib = L
+    return L
+
+
+def check(code):
+    if code != UCMC_OK:
+        raise UcmcError(code, lib().ucmc_last_error().decode())
+
+
+def device_count():
+    n = C.c_int(0)
+    check(lib().ucmc_device_count(C.byref(n)))
+    return n.value
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _graph(offsets, bonds):
+    off = np.ascontiguousarray(offsets, dtype=np.uint32)
+    b = np.ascontiguousarray(bonds, dtype=np.uint32)
+    g = PairGraph(len(off) - 1, off.ctypes.data_as(C.POINTER(C.c_uint32)),
+                  b.ctypes.data_as(C.POINTER(C.c_uint32)))
+    return g, (off, b)
+
+
+class System:
+    """Owns one ucmc_system handle."""
+
+    def __init__(self, handle, field_dtype, per_site):
+        self.h = handle
+        self.field_dtype = field_dtype
+        self.per_site = per_site
+        info = Info()
+        check(lib().ucmc_system_info(self.h, C.byref(info)))
+        self.info = info
+        self.sites = info.sites
+
+    # -- constructors --
+    @classmethod
+    def ising(cls, offsets, bonds, J=1, device=0):
+        g, keep = _graph(offsets, bonds)
+        h = C.c_void_p()
+        check(lib().ucmc_ising_create(C.byref(g), int(J), device, C.byref(h)))
+        return cls(h, np.int8, 1)
+
+    @classmethod
+    def ising_square(cls, Lx, Ly, J=1, device=0):
+        h = C.c_void_p()
+        check(lib().ucmc_ising_square_create(Lx, Ly, int(J), device, C.byref(h)))
+        return cls(h, np.int8, 1)
+
+    @classmethod
+    def vector(cls, N, offsets, bonds, J=1.0, device=0):
+        g, keep = _graph(offsets, bonds)
+        h = C.c_void_p()
+        check(lib().ucmc_vector_create(N, C.byref(g), float(J), device, C.byref(h)))
+        return cls(h, np.float64, N)
+
+    @classmethod
+    def u1_gauge(cls, L, beta, device=0):
+        h = C.c_void_p()
+        check(lib().ucmc_u1_gauge_create(L, float(beta), device, C.byref(h)))
+        return cls(h, np.float64, 1)
+
+    @classmethod
+    def sun_gauge(cls, N, L, beta, device=0):
+        h = C.c_void_p()
+        check(lib().ucmc_sun_gauge_create(N, L, float(beta), device, C.byref(h)))
+        return cls(h, np.float64, 2 * N * N)
+
+    @classmethod
+    def xy_angle(cls, offsets, bonds, J=1.0, device=0):
+        g, keep = _graph(offsets, bonds)
+        h = C.c_void_p()
+        check(lib().ucmc_xy_angle_create(C.byref(g), float(J), device, C.byref(h)))
+        return cls(h, np.float64, 1)
+
+    @classmethod
+    def plaquette_ising(cls, L, J1, K, device=0):
+        h = C.c_void_p()
+        check(lib().ucmc_plaquette_ising_create(L, float(J1), float(K), device, C.byref(h)))
+        return cls(h, np.int8, 1)
+
+    def close(self):
+        if self.h:
+            check(lib().ucmc_system_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- field --
+    def upload(self, field):
+        a = np.ascontiguousarray(field, dtype=self.field_dtype).reshape(-1)
+        check(lib().ucmc_field_upload(self.h, _ptr(a), a.nbytes))
+
+    def download(self):
+        a = np.empty(self.sites * self.per_site, dtype=self.field_dtype)
+        check(lib().ucmc_field_download(self.h, _ptr(a), a.nbytes))
+        return a
+
+    def colouring(self):
+        c = np.empty(self.sites, np.uint8)
+        check(lib().ucmc_colouring(self.h, _ptr(c)))
+        return c
+
+    # -- drivers --
+    @staticmethod
+    def _mparams(beta, proposal, width, hits, run_seed, sweep0, nsweeps, flags):
+        return MetropolisParams(float(beta), int(proposal), float(width), int(hits),
+                                int(run_seed), int(sweep0), int(nsweeps), int(flags))
+
+    def metropolis(self, beta, nsweeps, run_seed, sweep0=0, proposal=PROP_FLIP, width=0.0,
+                   hits=1, flags=0, record=False):
+        p = self._mparams(beta, proposal, width, hits, run_seed, sweep0, nsweeps, flags)
+        if record:
+            series = (Observables * nsweeps)()
+            check(lib().ucmc_metropolis_run(self.h, C.byref(p), series))
+            return series
+        check(lib().ucmc_metropolis_run(self.h, C.byref(p), None))
+        return None
+
+    def trace(self, beta, run_seed, sweep0=0, proposal=PROP_FLIP, width=0.0, hits=1):
+        p = self._mparams(beta, proposal, width, hits, run_seed, sweep0, 1, 0)
+        dE = np.empty(self.sites, np.float32)
+        acc = np.empty(self.sites, np.uint8)
+        check(lib().ucmc_metropolis_trace(self.h, C.byref(p), _ptr(dE), _ptr(acc)))
+        return dE, acc
+
+    def wolff(self, beta, nflips, run_seed, flip0=0, flags=0, record=False):
+        p = WolffParams(float(beta), int(run_seed), int(flip0), int(nflips), int(flags))
+        sizes = np.zeros(nflips, np.uint64)
+        series = (Observables * nflips)() if record else None
+        check(lib().ucmc_wolff_run(self.h, C.byref(p), _ptr(sizes), series))
+        return sizes, series
+
+    def observe(self):
+        o = Observables()
+        check(lib().ucmc_observe(self.h, C.byref(o)))
+        return o
+
+    def bench(self, beta, nsweeps, run_seed, repeats=1, proposal=PROP_FLIP, width=0.0, hits=1,
+              flags=0):
+        p = self._mparams(beta, proposal, width, hits, run_seed, 0, nsweeps, flags)
+        ms = C.c_double()
+        n = C.c_uint64()
+        check(lib().ucmc_metropolis_bench(self.h, C.byref(p), repeats, C.byref(ms), C.byref(n)))
+        return ms.value, n.value

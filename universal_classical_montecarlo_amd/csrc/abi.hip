@@ -1,0 +1,573 @@
+// abi.hip -- the extern "C" boundary declared in include/ucmc.h: argument
+// checks (mirroring the reference's exception conditions), system lifetime,
+// field transfer and dispatch to the per-model launchers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "ucmc_core.hpp"
+#include "ucmc_device.hpp"
+
+struct ucmc_system : ucmc::System {};
+
+namespace ucmc {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+System::~System()
+{
+    if (stream) {
+        (void)hipSetDevice(device);
+        (void)hipStreamSynchronize(stream);
+        (void)hipStreamDestroy(stream);
+    }
+}
+
+std::size_t System::device_bytes() const
+{
+    return offsets.bytes + bonds.bytes + edge_id.bytes + colour_sites.bytes + field[0].bytes +
+           field[1].bytes + scratch.bytes + scratch2.bytes + stamp.bytes + queue.bytes +
+           flip_out.bytes;
+}
+
+// square_lattice(Lx, Ly) (lattice/square.hpp:15-38) gives row 0 =
+// [1, Lx, Lx - 1, (Ly - 1) Lx] for Lx, Ly > 2.  Accept any table whose every row
+// holds exactly the four periodic neighbours (as a multiset), so the device
+// stencil reproduces the CSR energies.
+bool detect_square(const std::uint32_t* off, const std::uint32_t* bonds, std::size_t n,
+                   std::size_t& Lx, std::size_t& Ly)
+{
+    if (n < 9 || off[1] - off[0] != 4) return false;
+    const std::size_t lx = bonds[off[0] + 1];
+    if (lx < 3 || n % lx != 0) return false;
+    const std::size_t ly = n / lx;
+    if (ly < 3) return false;
+    for (std::size_t i = 0; i < n; ++i) {
+        if (off[i + 1] - off[i] != 4) return false;
+        const std::size_t x = i % lx, y = i / lx;
+        std::uint32_t want[4] = {
+            static_cast<std::uint32_t>(y * lx + (x + 1) % lx),
+            static_cast<std::uint32_t>(y * lx + (x + lx - 1) % lx),
+            static_cast<std::uint32_t>(((y + 1) % ly) * lx + x),
+            static_cast<std::uint32_t>(((y + ly - 1) % ly) * lx + x)};
+        std::uint32_t have[4] = {bonds[off[i]], bonds[off[i] + 1], bonds[off[i] + 2], bonds[off[i] + 3]};
+        std::sort(want, want + 4);
+        std::sort(have, have + 4);
+        if (!std::equal(want, want + 4, have)) return false;
+    }
+    Lx = lx;
+    Ly = ly;
+    return true;
+}
+
+bool detect_cubic(const std::uint32_t* off, const std::uint32_t* bonds, std::size_t n,
+                  std::size_t& L)
+{
+    if (n < 27 || off[1] - off[0] != 6) return false;
+    const std::size_t l = bonds[off[0] + 1];
+    if (l < 3 || l * l * l != n) return false;
+    for (std::size_t i = 0; i < n; ++i) {
+        if (off[i + 1] - off[i] != 6) return false;
+        const std::size_t x = i % l, y = (i / l) % l, z = i / (l * l);
+        auto id = [l](std::size_t xx, std::size_t yy, std::size_t zz) {
+            return static_cast<std::uint32_t>((zz * l + yy) * l + xx);
+        };
+        std::uint32_t want[6] = {id((x + 1) % l, y, z), id((x + l - 1) % l, y, z),
+                                 id(x, (y + 1) % l, z), id(x, (y + l - 1) % l, z),
+                                 id(x, y, (z + 1) % l), id(x, y, (z + l - 1) % l)};
+        std::uint32_t have[6];
+        for (int k = 0; k < 6; ++k) have[k] = bonds[off[i] + k];
+        std::sort(want, want + 6);
+        std::sort(have, have + 6);
+        if (!std::equal(want, want + 6, have)) return false;
+    }
+    L = l;
+    return true;
+}
+
+int greedy_colouring(const std::uint32_t* off, const std::uint32_t* bonds, std::size_t n,
+                     std::vector<std::uint8_t>& colour)
+{
+    colour.assign(n, 255);
+    int ncol = 0;
+    for (std::size_t i = 0; i < n; ++i) {
+        std::uint64_t used[4] = {0, 0, 0, 0};
+        for (std::uint32_t a = off[i]; a < off[i + 1]; ++a) {
+            const std::uint8_t c = colour[bonds[a]];
+            if (c != 255) used[c >> 6] |= 1ull << (c & 63);
+        }
+        int c = 0;
+        while (c < 255 && (used[c >> 6] & (1ull << (c & 63)))) ++c;
+        if (c == 255) fail(UCMC_E_UNSUPPORTED, "graph needs more than 255 colours");
+        colour[i] = static_cast<std::uint8_t>(c);
+        ncol = std::max(ncol, c + 1);
+    }
+    return ncol;
+}
+
+void upload_colour_lists(System& s)
+{
+    s.colour_start.assign(s.ncolours + 1, 0);
+    for (auto c : s.colour) ++s.colour_start[c + 1];
+    for (int c = 0; c < s.ncolours; ++c) s.colour_start[c + 1] += s.colour_start[c];
+    std::vector<std::uint32_t> list(s.sites);
+    std::vector<std::size_t> cursor(s.colour_start.begin(), s.colour_start.end() - 1);
+    for (std::size_t i = 0; i < s.sites; ++i) list[cursor[s.colour[i]]++] = static_cast<std::uint32_t>(i);
+    s.colour_sites.alloc(list.size() * 4);
+    UCMC_HIP(hipMemcpy(s.colour_sites.p, list.data(), list.size() * 4, hipMemcpyHostToDevice));
+}
+
+std::size_t build_edge_ids(System& s)
+{
+    const std::size_t n = s.sites;
+    const auto& off = s.h_offsets;
+    const auto& b = s.h_bonds;
+    std::vector<std::uint32_t> id(off[n], UINT32_MAX);
+    std::uint32_t next = 0;
+    for (std::size_t i = 0; i < n; ++i)
+        for (std::uint32_t a = off[i]; a < off[i + 1]; ++a) {
+            const std::uint32_t j = b[a];
+            if (j < i || id[a] != UINT32_MAX) continue;
+            std::size_t m = 0;
+            for (std::uint32_t q = off[i]; q < a; ++q) m += (b[q] == j);
+            std::size_t seen = 0, twin = SIZE_MAX;
+            for (std::uint32_t q = off[j]; q < off[j + 1]; ++q)
+                if (b[q] == i) {
+                    if (seen == m) { twin = q; break; }
+                    ++seen;
+                }
+            if (twin == SIZE_MAX) fail(UCMC_E_INVALID_ARGUMENT, "Wolff needs a symmetric graph");
+            id[a] = next;
+            id[twin] = next;
+            ++next;
+        }
+    for (auto v : id)
+        if (v == UINT32_MAX) fail(UCMC_E_INVALID_ARGUMENT, "Wolff needs a symmetric graph");
+    s.edge_id.alloc(id.size() * 4);
+    UCMC_HIP(hipMemcpy(s.edge_id.p, id.data(), id.size() * 4, hipMemcpyHostToDevice));
+    s.n_edges = next;
+    return next;
+}
+
+// Accepted draws of driver.hpp:83-84 form a prefix [0, k*): log(k 2^-32) rises
+// in k by at least 2^-32 per step, far above libm's error.
+std::uint64_t accept_threshold(double beta, double dE)
+{
+    const double log_p = -beta * dE + 0.0;
+    if (log_p >= 0.0) return 1ull << 32;
+    if (!(log_p == log_p)) return 0;
+    std::uint64_t lo = 0, hi = 1ull << 32;
+    while (hi - lo > 1) {
+        const std::uint64_t mid = lo + (hi - lo) / 2;
+        if (std::log(static_cast<double>(mid) * 0x1.0p-32) < log_p) lo = mid;
+        else hi = mid;
+    }
+    return hi;
+}
+
+std::uint64_t wolff_threshold(double beta, double dE)
+{
+    const double p = 1.0 - std::exp(-beta * dE);
+    if (!(p > 0.0)) return 0;
+    std::uint64_t lo = 0, hi = 1ull << 32;
+    if (static_cast<double>(hi - 1) * 0x1.0p-32 < p) return hi;
+    while (hi - lo > 1) {
+        const std::uint64_t mid = lo + (hi - lo) / 2;
+        if (static_cast<double>(mid) * 0x1.0p-32 < p) lo = mid;
+        else hi = mid;
+    }
+    return hi;
+}
+
+namespace {
+
+template <class F>
+int guard(F&& f)
+{
+    try {
+        f();
+        return UCMC_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return UCMC_E_OVERFLOW;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return UCMC_E_DEVICE;
+    }
+}
+
+void open_device(System& s, int device)
+{
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+        fail(UCMC_E_NO_DEVICE, "no HIP device");
+    if (device < 0 || device >= count) fail(UCMC_E_OUT_OF_RANGE, "device index out of range");
+    s.device = device;
+    UCMC_HIP(hipSetDevice(device));
+    UCMC_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+}
+
+// PairGraph invariants (lattice/pair_graph.hpp:49-83): indices fit uint32,
+// offsets non-decreasing, endpoints < sites, no self-loops.
+void check_graph(const ucmc_pair_graph* g)
+{
+    if (!g || (!g->offsets && g->sites)) fail(UCMC_E_INVALID_ARGUMENT, "null graph");
+    if (g->sites > UINT32_MAX) fail(UCMC_E_OVERFLOW, "PairGraph: site count exceeds uint32_t");
+    if (g->sites == 0) fail(UCMC_E_INVALID_ARGUMENT, "empty graph");
+    if (g->offsets[0] != 0) fail(UCMC_E_INVALID_ARGUMENT, "offsets[0] must be 0");
+    for (std::size_t i = 0; i < g->sites; ++i) {
+        if (g->offsets[i + 1] < g->offsets[i]) fail(UCMC_E_INVALID_ARGUMENT, "offsets decrease");
+        for (std::uint32_t a = g->offsets[i]; a < g->offsets[i + 1]; ++a) {
+            if (g->bonds[a] >= g->sites) fail(UCMC_E_OUT_OF_RANGE, "PairGraph: edge endpoint out of range");
+            if (g->bonds[a] == i) fail(UCMC_E_INVALID_ARGUMENT, "PairGraph: self-loop");
+        }
+    }
+}
+
+void take_graph(System& s, const ucmc_pair_graph* g)
+{
+    check_graph(g);
+    s.sites = g->sites;
+    s.h_offsets.assign(g->offsets, g->offsets + g->sites + 1);
+    s.h_bonds.assign(g->bonds, g->bonds + g->offsets[g->sites]);
+    s.max_degree = 0;
+    for (std::size_t i = 0; i < s.sites; ++i)
+        s.max_degree = std::max(s.max_degree, s.h_offsets[i + 1] - s.h_offsets[i]);
+    s.offsets.alloc(s.h_offsets.size() * 4);
+    UCMC_HIP(hipMemcpy(s.offsets.p, s.h_offsets.data(), s.h_offsets.size() * 4, hipMemcpyHostToDevice));
+    s.bonds.alloc(std::max<std::size_t>(s.h_bonds.size(), 1) * 4);
+    if (!s.h_bonds.empty())
+        UCMC_HIP(hipMemcpy(s.bonds.p, s.h_bonds.data(), s.h_bonds.size() * 4, hipMemcpyHostToDevice));
+}
+
+// Doubles per site in the reference's data() layout.
+std::size_t field_doubles_per_site(const System& s)
+{
+    switch (s.kind) {
+    case UCMC_VECTOR: return static_cast<std::size_t>(s.N);
+    case UCMC_SUN_GAUGE: return static_cast<std::size_t>(2 * s.N * s.N);
+    default: return 1;
+    }
+}
+
+System* checked(ucmc_system* s)
+{
+    if (!s) fail(UCMC_E_INVALID_ARGUMENT, "null system");
+    UCMC_HIP(hipSetDevice(s->device));
+    return s;
+}
+
+}  // namespace
+}  // namespace ucmc
+
+using namespace ucmc;
+
+extern "C" {
+
+int ucmc_abi_version(void) { return UCMC_ABI_VERSION; }
+
+const char* ucmc_last_error(void) { return g_last_error.c_str(); }
+
+int ucmc_device_count(int* count)
+{
+    return guard([&] {
+        if (!count) fail(UCMC_E_INVALID_ARGUMENT, "null count");
+        int c = 0;
+        if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+        *count = c;
+    });
+}
+
+uint64_t ucmc_key(uint64_t run_seed, uint64_t counter) { return key_of(run_seed, counter); }
+
+int ucmc_ising_create(const ucmc_pair_graph* graph, int32_t J, int device, ucmc_system** out)
+{
+    return guard([&] {
+        if (!out) fail(UCMC_E_INVALID_ARGUMENT, "null out");
+        *out = nullptr;
+        auto* s = new ucmc_system();
+        try {
+            s->kind = UCMC_ISING;
+            s->Ji = J;
+            open_device(*s, device);
+            take_graph(*s, graph);
+            ising_setup(*s);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+int ucmc_ising_square_create(size_t Lx, size_t Ly, int32_t J, int device, ucmc_system** out)
+{
+    return guard([&] {
+        if (!out) fail(UCMC_E_INVALID_ARGUMENT, "null out");
+        *out = nullptr;
+        // square.hpp:17-24 argument checks
+        if (Lx == 0 || Ly == 0) fail(UCMC_E_INVALID_ARGUMENT, "square_lattice: Lx and Ly must be positive");
+        if (Lx > SIZE_MAX / Ly) fail(UCMC_E_OVERFLOW, "square_lattice: site count overflows size_t");
+        if (Lx * Ly > UINT32_MAX) fail(UCMC_E_OVERFLOW, "square_lattice: site count exceeds uint32_t");
+        auto* s = new ucmc_system();
+        try {
+            s->kind = UCMC_ISING;
+            s->Ji = J;
+            open_device(*s, device);
+            s->sites = Lx * Ly;
+            s->geom = Geom::Square;
+            s->Lx = Lx;
+            s->Ly = Ly;
+            ising_setup(*s);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+int ucmc_vector_create(int N, const ucmc_pair_graph* graph, double J, int device,
+                       ucmc_system** out)
+{
+    return guard([&] {
+        if (!out) fail(UCMC_E_INVALID_ARGUMENT, "null out");
+        *out = nullptr;
+        if (N < 2 || N > 4) fail(UCMC_E_UNSUPPORTED, "VectorSpin<N> on device needs N in {2,3,4}");
+        auto* s = new ucmc_system();
+        try {
+            s->kind = UCMC_VECTOR;
+            s->N = N;
+            s->J = J;
+            open_device(*s, device);
+            take_graph(*s, graph);
+            vector_setup(*s);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+int ucmc_u1_gauge_create(size_t L, double beta, int device, ucmc_system** out)
+{
+    return guard([&] {
+        (void)L; (void)beta; (void)device;
+        if (out) *out = nullptr;
+        fail(UCMC_E_UNSUPPORTED, "U1Gauge not built yet");
+    });
+}
+
+int ucmc_sun_gauge_create(int N, size_t L, double beta, int device, ucmc_system** out)
+{
+    return guard([&] {
+        (void)N; (void)L; (void)beta; (void)device;
+        if (out) *out = nullptr;
+        fail(UCMC_E_UNSUPPORTED, "SUGauge not built yet");
+    });
+}
+
+int ucmc_xy_angle_create(const ucmc_pair_graph* graph, double J, int device, ucmc_system** out)
+{
+    return guard([&] {
+        (void)graph; (void)J; (void)device;
+        if (out) *out = nullptr;
+        fail(UCMC_E_UNSUPPORTED, "XY angle model not built yet");
+    });
+}
+
+int ucmc_plaquette_ising_create(size_t L, double J1, double K, int device, ucmc_system** out)
+{
+    return guard([&] {
+        (void)L; (void)J1; (void)K; (void)device;
+        if (out) *out = nullptr;
+        fail(UCMC_E_UNSUPPORTED, "plaquette Ising not built yet");
+    });
+}
+
+int ucmc_system_destroy(ucmc_system* sys)
+{
+    return guard([&] {
+        if (!sys) return;
+        (void)hipSetDevice(sys->device);
+        delete sys;
+    });
+}
+
+int ucmc_system_info(const ucmc_system* sys, ucmc_info* out)
+{
+    return guard([&] {
+        if (!sys || !out) fail(UCMC_E_INVALID_ARGUMENT, "null argument");
+        std::memset(out, 0, sizeof(*out));
+        out->kind = sys->kind;
+        out->components = sys->N;
+        out->sites = sys->sites;
+        out->schedule = sys->schedule;
+        out->ncolours = sys->ncolours;
+        out->dims[0] = sys->geom == Geom::Square ? sys->Lx : sys->L;
+        out->dims[1] = sys->geom == Geom::Square ? sys->Ly : (sys->L ? sys->L : 0);
+        out->device_bytes = sys->device_bytes();
+    });
+}
+
+int ucmc_colouring(const ucmc_system* sys, uint8_t* colour)
+{
+    return guard([&] {
+        if (!sys || !colour) fail(UCMC_E_INVALID_ARGUMENT, "null argument");
+        std::memcpy(colour, sys->colour.data(), sys->sites);
+    });
+}
+
+int ucmc_field_upload(ucmc_system* sys, const void* host, size_t bytes)
+{
+    return guard([&] {
+        System* s = checked(sys);
+        if (!host) fail(UCMC_E_INVALID_ARGUMENT, "null host pointer");
+        switch (s->kind) {
+        case UCMC_ISING:
+        case UCMC_PLAQ_ISING: {
+            if (bytes != s->sites) fail(UCMC_E_INVALID_ARGUMENT, "Ising field is sites int8 values");
+            const auto* v = static_cast<const std::int8_t*>(host);
+            for (std::size_t i = 0; i < s->sites; ++i)
+                if (v[i] != 1 && v[i] != -1) fail(UCMC_E_INVALID_ARGUMENT, "Ising spins must be +-1");
+            UCMC_HIP(hipMemcpyAsync(s->field[s->cur].p, host, bytes, hipMemcpyHostToDevice, s->stream));
+            break;
+        }
+        default: {
+            const std::size_t per = field_doubles_per_site(*s);
+            if (bytes != s->sites * per * sizeof(double))
+                fail(UCMC_E_INVALID_ARGUMENT, "field size does not match the model");
+            // host AoS double -> device SoA fp32: component k of site i at [k * sites + i]
+            std::vector<float> f(s->sites * per);
+            const auto* v = static_cast<const double*>(host);
+            for (std::size_t i = 0; i < s->sites; ++i)
+                for (std::size_t k = 0; k < per; ++k) f[k * s->sites + i] = static_cast<float>(v[i * per + k]);
+            UCMC_HIP(hipMemcpyAsync(s->field[s->cur].p, f.data(), f.size() * 4, hipMemcpyHostToDevice, s->stream));
+            UCMC_HIP(hipStreamSynchronize(s->stream));
+            break;
+        }
+        }
+        UCMC_HIP(hipStreamSynchronize(s->stream));
+    });
+}
+
+int ucmc_field_download(ucmc_system* sys, void* host, size_t bytes)
+{
+    return guard([&] {
+        System* s = checked(sys);
+        if (!host) fail(UCMC_E_INVALID_ARGUMENT, "null host pointer");
+        switch (s->kind) {
+        case UCMC_ISING:
+        case UCMC_PLAQ_ISING:
+            if (bytes != s->sites) fail(UCMC_E_INVALID_ARGUMENT, "Ising field is sites int8 values");
+            UCMC_HIP(hipMemcpyAsync(host, s->field[s->cur].p, bytes, hipMemcpyDeviceToHost, s->stream));
+            UCMC_HIP(hipStreamSynchronize(s->stream));
+            break;
+        default: {
+            const std::size_t per = field_doubles_per_site(*s);
+            if (bytes != s->sites * per * sizeof(double))
+                fail(UCMC_E_INVALID_ARGUMENT, "field size does not match the model");
+            std::vector<float> f(s->sites * per);
+            UCMC_HIP(hipMemcpyAsync(f.data(), s->field[s->cur].p, f.size() * 4, hipMemcpyDeviceToHost, s->stream));
+            UCMC_HIP(hipStreamSynchronize(s->stream));
+            auto* v = static_cast<double*>(host);
+            for (std::size_t i = 0; i < s->sites; ++i)
+                for (std::size_t k = 0; k < per; ++k) v[i * per + k] = static_cast<double>(f[k * s->sites + i]);
+            break;
+        }
+        }
+    });
+}
+
+int ucmc_metropolis_run(ucmc_system* sys, const ucmc_metropolis_params* p,
+                        ucmc_observables* series)
+{
+    return guard([&] {
+        System* s = checked(sys);
+        if (!p) fail(UCMC_E_INVALID_ARGUMENT, "null params");
+        if (!(p->beta == p->beta)) fail(UCMC_E_INVALID_ARGUMENT, "beta is NaN");
+        if (p->hits < 1) fail(UCMC_E_INVALID_ARGUMENT, "hits must be >= 1");
+        switch (s->kind) {
+        case UCMC_ISING: ising_metropolis(*s, *p, series, nullptr, nullptr); break;
+        case UCMC_VECTOR: vector_metropolis(*s, *p, series, nullptr, nullptr); break;
+        default: fail(UCMC_E_UNSUPPORTED, "model kind");
+        }
+    });
+}
+
+int ucmc_metropolis_trace(ucmc_system* sys, const ucmc_metropolis_params* p, float* dE,
+                          uint8_t* accepted)
+{
+    return guard([&] {
+        System* s = checked(sys);
+        if (!p || !dE || !accepted) fail(UCMC_E_INVALID_ARGUMENT, "null argument");
+        if (p->nsweeps != 1) fail(UCMC_E_INVALID_ARGUMENT, "trace runs exactly one sweep");
+        if (p->hits < 1) fail(UCMC_E_INVALID_ARGUMENT, "hits must be >= 1");
+        switch (s->kind) {
+        case UCMC_ISING: ising_metropolis(*s, *p, nullptr, accepted, dE); break;
+        case UCMC_VECTOR: vector_metropolis(*s, *p, nullptr, accepted, dE); break;
+        default: fail(UCMC_E_UNSUPPORTED, "model kind");
+        }
+    });
+}
+
+int ucmc_wolff_run(ucmc_system* sys, const ucmc_wolff_params* p, uint64_t* cluster_sizes,
+                   ucmc_observables* series)
+{
+    return guard([&] {
+        System* s = checked(sys);
+        if (!p) fail(UCMC_E_INVALID_ARGUMENT, "null params");
+        switch (s->kind) {
+        case UCMC_ISING:
+        case UCMC_VECTOR: wolff_run(*s, *p, cluster_sizes, series); break;
+        default: fail(UCMC_E_UNSUPPORTED, "Wolff needs a pair model (driver.hpp:117-128)");
+        }
+    });
+}
+
+int ucmc_observe(ucmc_system* sys, ucmc_observables* out)
+{
+    return guard([&] {
+        System* s = checked(sys);
+        if (!out) fail(UCMC_E_INVALID_ARGUMENT, "null out");
+        std::memset(out, 0, sizeof(*out));
+        switch (s->kind) {
+        case UCMC_ISING: ising_observe(*s, *out); break;
+        case UCMC_VECTOR: vector_observe(*s, *out); break;
+        default: fail(UCMC_E_UNSUPPORTED, "model kind");
+        }
+    });
+}
+
+int ucmc_metropolis_bench(ucmc_system* sys, const ucmc_metropolis_params* p, int repeats,
+                          double* elapsed_ms, uint64_t* launches)
+{
+    return guard([&] {
+        System* s = checked(sys);
+        if (!p || !elapsed_ms || !launches || repeats < 1) fail(UCMC_E_INVALID_ARGUMENT, "bad argument");
+        float ms = 0.f;
+        std::uint64_t n = 0;
+        switch (s->kind) {
+        case UCMC_ISING: n = ising_bench(*s, *p, repeats, ms); break;
+        case UCMC_VECTOR: n = vector_bench(*s, *p, repeats, ms); break;
+        default: fail(UCMC_E_UNSUPPORTED, "model kind");
+        }
+        *elapsed_ms = ms;
+        *launches = n;
+    });
+}
+
+}  // extern "C"

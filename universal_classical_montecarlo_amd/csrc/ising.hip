@@ -1,0 +1,586 @@
+// ising.hip -- Ising Metropolis on the GPU (example/ising.hpp:14-76 with the
+// Flip proposal of dof/ising.hpp:39-51, driven as in driver.hpp:73-87).
+//
+// Two kernel families:
+//  * ising_csr_colour: one launch per colour class of any PairGraph, one thread
+//    per site, dE summed over the CSR row.  Used for general graphs, odd
+//    lattices, and as the simple reference-shaped path.
+//  * ising_fused: periodic square lattice with Lx % 512 == 0 and Ly % H == 0.
+//    Both checkerboard colours of a sweep in one pass: each wave owns a 512-byte
+//    column band and an H-row block, slides down the rows with a 4-row register
+//    window (colour 0 of row r-1, then colour 1 of row r-2), and reads every
+//    spin once and writes it once (2 B/site).  Rows are ping-ponged between two
+//    buffers so halo rows are always read unmodified.
+//
+// The accept rule log(u) < -beta dE (driver.hpp:83-84) is evaluated exactly as
+// u_32 <= k*(dE) - 1, with k* computed on the host with the same double
+// expression (accept_threshold), so decisions are bit-identical to the CPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "ucmc_core.hpp"
+#include "ucmc_device.hpp"
+
+namespace ucmc {
+
+void square_csr(System& s);
+
+namespace {
+
+constexpr int kMaxDegTable = 255;
+
+// ------------------------------------------------------------ CSR kernel --
+struct CsrArgs {
+    std::int8_t* field;
+    const std::uint32_t* offsets;
+    const std::uint32_t* bonds;
+    const std::uint32_t* sites;   // colour class
+    std::uint32_t count;
+    std::uint32_t max_degree;
+    std::uint64_t key;
+    const std::uint32_t* tm1;     // [2 max_degree + 1], index h + max_degree
+    std::int32_t twoJ;
+    unsigned long long* accepts;  // nullable
+    float* trace_dE;              // nullable
+    std::uint8_t* trace_acc;      // nullable
+};
+
+__global__ __launch_bounds__(256) void ising_csr_colour(CsrArgs a)
+{
+    const std::uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    bool acc = false;
+    if (t < a.count) {
+        const std::uint32_t i = a.sites[t];
+        const int s = a.field[i];
+        int sum = 0;
+        for (std::uint32_t q = a.offsets[i]; q < a.offsets[i + 1]; ++q) sum += a.field[a.bonds[q]];
+        const int h = s * sum;
+        const std::uint32_t u = pcg_out(first_state(a.key, i));
+        acc = u <= a.tm1[h + static_cast<int>(a.max_degree)];
+        if (acc) a.field[i] = static_cast<std::int8_t>(-s);
+        if (a.trace_dE) {
+            a.trace_dE[i] = static_cast<float>(a.twoJ * h);
+            a.trace_acc[i] = acc ? 1 : 0;
+        }
+    }
+    if (a.accepts) {
+        const unsigned long long n = __popcll(__ballot(acc));
+        if ((threadIdx.x & 63) == 0 && n) atomicAdd(a.accepts, n);
+    }
+}
+
+// ---------------------------------------------------------- fused kernel --
+struct FusedArgs {
+    const std::uint32_t* in;   // field as 32-bit words, Lx / 4 per row
+    std::uint32_t* out;
+    std::uint32_t Lx, Ly, H;
+    std::uint32_t bands;       // Lx / 512
+    std::uint64_t A;           // first_state(key, 0)
+    std::uint64_t LxD;         // Lx * stride (mod 2^64)
+    std::uint64_t D;           // stream stride
+    std::uint32_t tm1[16];     // index C * 8 + S (C = site is -1, S = # of -1 neighbours)
+    unsigned long long* accepts;
+};
+
+struct Row {
+    std::uint32_t E, O; // 4b-form (0 for +1, 4 for -1 per byte) of even / odd x
+};
+
+__device__ __forceinline__ Row load_row(const std::uint32_t* in, std::uint32_t row_words,
+                                        std::uint32_t Y, std::uint32_t col)
+{
+    const uint2 w = *reinterpret_cast<const uint2*>(in + static_cast<std::size_t>(Y) * row_words + 2 * col);
+    const std::uint32_t e = __builtin_amdgcn_perm(w.y, w.x, 0x06040200u);
+    const std::uint32_t o = __builtin_amdgcn_perm(w.y, w.x, 0x07050301u);
+    return Row{(e << 1) & 0x04040404u, (o << 1) & 0x04040404u};
+}
+
+__device__ __forceinline__ void store_row(std::uint32_t* out, std::uint32_t row_words,
+                                          std::uint32_t Y, std::uint32_t col, const Row& r)
+{
+    const std::uint32_t lo = __builtin_amdgcn_perm(r.O, r.E, 0x05010400u);
+    const std::uint32_t hi = __builtin_amdgcn_perm(r.O, r.E, 0x07030602u);
+    uint2 w;
+    w.x = __builtin_amdgcn_perm(0xFFFFFFFFu, 0x01010101u, lo);
+    w.y = __builtin_amdgcn_perm(0xFFFFFFFFu, 0x01010101u, hi);
+    *reinterpret_cast<uint2*>(out + static_cast<std::size_t>(Y) * row_words + 2 * col) = w;
+}
+
+// Decide the four sites of one 4b-form word C with neighbour counts S.
+// xb: per-lane stream offset of byte 0 relative to the row; rowk[k]: the
+// wave-uniform part of site k's state.  Returns the updated word.
+template <bool COUNT>
+__device__ __forceinline__ std::uint32_t decide(std::uint32_t C, std::uint32_t S,
+                                                std::uint64_t lane_base,
+                                                const std::uint64_t (&rowk)[4],
+                                                const std::uint32_t* tbl, std::uint32_t& acc)
+{
+    const std::uint32_t key = (C << 3) | S; // byte k = 4 * (8 C_k + S_k)
+    std::uint32_t F = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const std::uint32_t addr = (key >> (8 * k)) & 0xFFu;
+        const std::uint32_t t = *reinterpret_cast<const std::uint32_t*>(
+            reinterpret_cast<const char*>(tbl) + addr);
+        const std::uint32_t u = pcg_out(lane_base + rowk[k]);
+        F |= (u <= t) ? (4u << (8 * k)) : 0u;
+    }
+    if (COUNT) acc += __builtin_popcount(F);
+    return C ^ F;
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(256) void ising_fused(FusedArgs a)
+{
+    __shared__ std::uint32_t tbl[16];
+    __shared__ std::uint32_t haloL[4][66];
+    __shared__ std::uint32_t haloR[4][66];
+
+    const std::uint32_t lane = threadIdx.x & 63u;
+    const std::uint32_t wid = threadIdx.x >> 6;
+    const std::uint32_t gw = blockIdx.x * 4u + wid;          // global wave index
+    const std::uint32_t nblocks = a.Ly / a.H;
+    if (threadIdx.x < 16) tbl[threadIdx.x] = a.tm1[threadIdx.x];
+    __syncthreads();
+
+    const bool active = gw < a.bands * nblocks;
+    const std::uint32_t band = active ? gw % a.bands : 0;
+    const std::uint32_t rb = active ? gw / a.bands : 0;
+    const std::uint32_t y0 = rb * a.H;
+    const std::uint32_t row_words = a.Lx >> 2;
+    const std::uint32_t col = band * 64u + lane;              // 8-byte column
+    const std::uint32_t x0 = col * 8u;
+    const std::uint8_t* inb = reinterpret_cast<const std::uint8_t*>(a.in);
+
+    // ---- halo pre-pass: one lane per row y0-1 .. y0+H (H <= 64) ----
+    if (active && lane < a.H + 2) {
+        const std::uint32_t Y = (y0 + a.Ly - 1u + lane) % a.Ly;
+        const std::uint32_t Yu = (Y + a.Ly - 1u) % a.Ly, Yd = (Y + 1u) % a.Ly;
+        const std::uint32_t xl = (band * 512u + a.Lx - 1u) % a.Lx;   // odd
+        const std::uint32_t xr = (band * 512u + 512u) % a.Lx;        // even
+        auto b4 = [&](std::uint32_t x, std::uint32_t y) -> std::uint32_t {
+            return (static_cast<std::uint32_t>(inb[static_cast<std::size_t>(y) * a.Lx + x]) << 1) & 4u;
+        };
+        auto update0 = [&](std::uint32_t x) -> std::uint32_t {
+            // colour-0 update of site (x, Y) from unmodified neighbours
+            const std::uint32_t c = b4(x, Y);
+            const std::uint32_t s = b4((x + 1u) % a.Lx, Y) + b4((x + a.Lx - 1u) % a.Lx, Y) +
+                                    b4(x, Yu) + b4(x, Yd);
+            const std::uint32_t t = tbl[(c << 1) + (s >> 2)];
+            const std::uint64_t st = a.A + (static_cast<std::uint64_t>(Y) * a.Lx + x) * a.D;
+            return (pcg_out(st) <= t) ? (c ^ 4u) : c;
+        };
+        const bool even = (Y & 1u) == 0;
+        // left: odd x; even rows need the old value, odd rows the colour-0 value
+        haloL[wid][lane] = (even ? b4(xl, Y) : update0(xl)) << 24;
+        // right: even x; even rows need the colour-0 value, odd rows the old one
+        haloR[wid][lane] = even ? update0(xr) : b4(xr, Y);
+    }
+    __syncthreads();
+    if (!active) return;
+
+    std::uint32_t acc = 0;
+    const std::uint64_t lane_base = static_cast<std::uint64_t>(x0) * a.D;
+
+    auto rowk = [&](std::uint32_t Y, std::uint32_t odd, std::uint64_t (&rk)[4]) {
+        const std::uint64_t base = a.A + static_cast<std::uint64_t>(Y) * a.LxD;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) rk[k] = base + static_cast<std::uint64_t>(2 * k + odd) * a.D;
+    };
+    auto wrap = [&](std::uint32_t y) { return y >= a.Ly ? y - a.Ly : y; };
+    // Row index relative to the block start, as used by the halo arrays.
+    auto hidx = [&](std::uint32_t r) { return r + 1u; };
+
+    // E update of row Y (site colour == Y & 1 ... handled by caller).
+    auto updE = [&](Row& cur, const Row& up, const Row& dn, std::uint32_t Y, std::uint32_t hr) {
+        const std::uint32_t Oprev = from_prev_lane(cur.O, haloL[wid][hr]);
+        const std::uint32_t Lft = __builtin_amdgcn_alignbit(cur.O, Oprev, 24);
+        const std::uint32_t S = Lft + cur.O + up.E + dn.E;
+        std::uint64_t rk[4];
+        rowk(Y, 0u, rk);
+        cur.E = decide<COUNT>(cur.E, S, lane_base, rk, tbl, acc);
+    };
+    auto updO = [&](Row& cur, const Row& up, const Row& dn, std::uint32_t Y, std::uint32_t hr) {
+        const std::uint32_t Enext = from_next_lane(cur.E, haloR[wid][hr]);
+        const std::uint32_t Rgt = __builtin_amdgcn_alignbit(Enext, cur.E, 8);
+        const std::uint32_t S = cur.E + Rgt + up.O + dn.O;
+        std::uint64_t rk[4];
+        rowk(Y, 1u, rk);
+        cur.O = decide<COUNT>(cur.O, S, lane_base, rk, tbl, acc);
+    };
+
+    // rows y0-2 .. y0+1
+    Row A_, B_, C_, D_;
+    D_ = load_row(a.in, row_words, wrap(y0 + a.Ly - 2u), col);
+    A_ = load_row(a.in, row_words, wrap(y0 + a.Ly - 1u), col);
+    B_ = load_row(a.in, row_words, y0, col);
+    updO(A_, D_, B_, wrap(y0 + a.Ly - 1u), hidx(0u) - 1u);  // colour 0 of row y0-1 (odd)
+    C_ = load_row(a.in, row_words, wrap(y0 + 1u), col);
+    updE(B_, A_, C_, y0, hidx(0u));                      // colour 0 of row y0 (even)
+
+    // main loop: step for absolute row r = y0 + 2 + k
+    for (std::uint32_t k = 0; k < a.H; k += 4) {
+        const std::uint32_t r = y0 + 2u + k; // even
+        // k % 4 == 0: window (A,B,C,D) = rows r-3 .. r
+        D_ = load_row(a.in, row_words, wrap(r), col);
+        updO(C_, B_, D_, wrap(r - 1u), hidx(k + 1u));      // colour 0, row r-1 (odd)
+        updO(B_, A_, C_, r - 2u, hidx(k));                // colour 1, row r-2 (even)
+        store_row(a.out, row_words, r - 2u, col, B_);
+        // k % 4 == 1: window (B,C,D,A), r+1 odd
+        A_ = load_row(a.in, row_words, wrap(r + 1u), col);
+        updE(D_, C_, A_, wrap(r), hidx(k + 2u));           // colour 0, row r (even)
+        updE(C_, B_, D_, wrap(r - 1u), hidx(k + 1u));      // colour 1, row r-1 (odd)
+        store_row(a.out, row_words, wrap(r - 1u), col, C_);
+        // k % 4 == 2: window (C,D,A,B), r+2 even
+        B_ = load_row(a.in, row_words, wrap(r + 2u), col);
+        updO(A_, D_, B_, wrap(r + 1u), hidx(k + 3u));      // colour 0, row r+1 (odd)
+        updO(D_, C_, A_, wrap(r), hidx(k + 2u));           // colour 1, row r (even)
+        store_row(a.out, row_words, wrap(r), col, D_);
+        // k % 4 == 3: window (D,A,B,C), r+3 odd
+        C_ = load_row(a.in, row_words, wrap(r + 3u), col);
+        updE(B_, A_, C_, wrap(r + 2u), hidx(k + 4u));      // colour 0, row r+2 (even)
+        updE(A_, D_, B_, wrap(r + 1u), hidx(k + 3u));      // colour 1, row r+1 (odd)
+        store_row(a.out, row_words, wrap(r + 1u), col, A_);
+    }
+
+    if (COUNT && a.accepts) {
+        const unsigned long long tot = wave_sum_u64(acc);
+        if (lane == 0 && tot) atomicAdd(a.accepts, tot);
+    }
+}
+
+// ------------------------------------------------------------ reductions --
+// Square lattice: sum over sites of s_i (s_right + s_down) and of s_i, one
+// thread per 4-byte word.  out[0] = anti-aligned bonds, out[1] = # of -1.
+__global__ __launch_bounds__(256) void ising_reduce_square(const std::uint32_t* f, std::uint32_t Lx,
+                                                           std::uint32_t Ly,
+                                                           unsigned long long* out)
+{
+    const std::uint32_t words = Lx / 4u;
+    const std::size_t nw = static_cast<std::size_t>(words) * Ly;
+    unsigned long long anti = 0, neg = 0;
+    for (std::size_t w = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; w < nw;
+         w += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+        const std::uint32_t y = static_cast<std::uint32_t>(w / words), c = static_cast<std::uint32_t>(w % words);
+        const std::uint32_t v = f[w] & 0x02020202u;
+        const std::uint32_t dn = f[static_cast<std::size_t>((y + 1u) % Ly) * words + c] & 0x02020202u;
+        const std::uint32_t nx = f[static_cast<std::size_t>(y) * words + (c + 1u) % words] & 0x02020202u;
+        const std::uint32_t right = __builtin_amdgcn_alignbit(nx, v, 8);
+        anti += __builtin_popcount(v ^ dn) + __builtin_popcount(v ^ right);
+        neg += __builtin_popcount(v);
+    }
+    anti = wave_sum_u64(anti);
+    neg = wave_sum_u64(neg);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(out, anti);
+        atomicAdd(out + 1, neg);
+    }
+}
+
+// General graph: sum_i s_i sum_row s_j and sum_i s_i (as signed 64-bit).
+__global__ __launch_bounds__(256) void ising_reduce_csr(const std::int8_t* f, const std::uint32_t* off,
+                                                        const std::uint32_t* bonds, std::uint32_t n,
+                                                        long long* out)
+{
+    long long e = 0, m = 0;
+    for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int sum = 0;
+        for (std::uint32_t q = off[i]; q < off[i + 1]; ++q) sum += f[bonds[q]];
+        e += static_cast<long long>(f[i]) * sum;
+        m += f[i];
+    }
+    e = static_cast<long long>(wave_sum_u64(static_cast<unsigned long long>(e)));
+    m = static_cast<long long>(wave_sum_u64(static_cast<unsigned long long>(m)));
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(out), static_cast<unsigned long long>(e));
+        atomicAdd(reinterpret_cast<unsigned long long*>(out + 1), static_cast<unsigned long long>(m));
+    }
+}
+
+constexpr std::uint32_t kFusedH = 16;
+
+bool fused_ok(const System& s)
+{
+    return s.geom == Geom::Square && s.Lx % 512 == 0 && s.Ly % kFusedH == 0 && s.Ly >= kFusedH;
+}
+
+struct IsingTables {
+    std::vector<std::uint32_t> csr;   // 2 maxdeg + 1
+    std::uint32_t sq[16];
+};
+
+IsingTables make_tables(const System& s, double beta)
+{
+    IsingTables t{};
+    const int md = static_cast<int>(s.max_degree);
+    t.csr.resize(2 * md + 1);
+    for (int h = -md; h <= md; ++h) {
+        const std::uint64_t k = accept_threshold(beta, static_cast<double>(2 * s.Ji * h));
+        t.csr[h + md] = static_cast<std::uint32_t>(k - 1); // k >= 1 for finite beta
+    }
+    for (int c = 0; c < 2; ++c)
+        for (int n = 0; n < 8; ++n) {
+            const int sgn = c ? -1 : 1;
+            const int h = sgn * (4 - 2 * std::min(n, 4));
+            const std::uint64_t k = accept_threshold(beta, static_cast<double>(2 * s.Ji * h));
+            t.sq[c * 8 + n] = static_cast<std::uint32_t>(k - 1);
+        }
+    return t;
+}
+
+void observe_into(System& s, unsigned long long* dev_out, ucmc_observables& o);
+
+void launch_sweep(System& s, const IsingTables& t, std::uint64_t key, bool simple,
+                  unsigned long long* dacc, float* trace_dE, std::uint8_t* trace_acc)
+{
+    if (!simple && fused_ok(s)) {
+        FusedArgs a{};
+        a.in = s.field[s.cur].as<std::uint32_t>();
+        a.out = s.field[s.cur ^ 1].as<std::uint32_t>();
+        a.Lx = static_cast<std::uint32_t>(s.Lx);
+        a.Ly = static_cast<std::uint32_t>(s.Ly);
+        a.H = kFusedH;
+        a.bands = static_cast<std::uint32_t>(s.Lx / 512);
+        a.D = kStreamStride;
+        a.A = first_state(key, 0);
+        a.LxD = static_cast<std::uint64_t>(s.Lx) * kStreamStride;
+        std::memcpy(a.tm1, t.sq, sizeof(a.tm1));
+        a.accepts = dacc;
+        const std::uint32_t waves = a.bands * (a.Ly / a.H);
+        const dim3 grid((waves + 3) / 4);
+        if (dacc) hipLaunchKernelGGL(ising_fused<true>, grid, dim3(256), 0, s.stream, a);
+        else hipLaunchKernelGGL(ising_fused<false>, grid, dim3(256), 0, s.stream, a);
+        UCMC_HIP(hipGetLastError());
+        s.cur ^= 1;
+        return;
+    }
+    for (int c = 0; c < s.ncolours; ++c) {
+        CsrArgs a{};
+        a.field = s.field[s.cur].as<std::int8_t>();
+        a.offsets = s.offsets.as<std::uint32_t>();
+        a.bonds = s.bonds.as<std::uint32_t>();
+        a.sites = s.colour_sites.as<std::uint32_t>() + s.colour_start[c];
+        a.count = static_cast<std::uint32_t>(s.colour_start[c + 1] - s.colour_start[c]);
+        a.max_degree = s.max_degree;
+        a.key = key;
+        a.tm1 = s.scratch2.as<std::uint32_t>();
+        a.twoJ = 2 * s.Ji;
+        a.accepts = dacc;
+        a.trace_dE = trace_dE;
+        a.trace_acc = trace_acc;
+        if (a.count == 0) continue;
+        hipLaunchKernelGGL(ising_csr_colour, dim3((a.count + 255) / 256), dim3(256), 0, s.stream, a);
+        UCMC_HIP(hipGetLastError());
+    }
+}
+
+void upload_csr_table(System& s, const IsingTables& t)
+{
+    UCMC_HIP(hipMemcpyAsync(s.scratch2.p, t.csr.data(), t.csr.size() * 4, hipMemcpyHostToDevice, s.stream));
+}
+
+void reduce_launch(System& s, unsigned long long* dev_out)
+{
+    if (s.geom == Geom::Square && s.Lx % 4 == 0) {
+        const std::size_t nw = s.sites / 4;
+        const unsigned blocks = static_cast<unsigned>(std::min<std::size_t>((nw + 255) / 256, 2048));
+        hipLaunchKernelGGL(ising_reduce_square, dim3(blocks), dim3(256), 0, s.stream,
+                           s.field[s.cur].as<std::uint32_t>(), static_cast<std::uint32_t>(s.Lx),
+                           static_cast<std::uint32_t>(s.Ly), dev_out);
+    } else {
+        const unsigned blocks = static_cast<unsigned>(std::min<std::size_t>((s.sites + 255) / 256, 2048));
+        hipLaunchKernelGGL(ising_reduce_csr, dim3(blocks), dim3(256), 0, s.stream,
+                           s.field[s.cur].as<std::int8_t>(), s.offsets.as<std::uint32_t>(),
+                           s.bonds.as<std::uint32_t>(), static_cast<std::uint32_t>(s.sites),
+                           reinterpret_cast<long long*>(dev_out));
+    }
+    UCMC_HIP(hipGetLastError());
+}
+
+// Convert the two reduction words into the reference's observables.
+void finish_obs(const System& s, const unsigned long long* w, ucmc_observables& o)
+{
+    if (s.geom == Geom::Square && s.Lx % 4 == 0) {
+        const long long bonds = 2LL * static_cast<long long>(s.sites);
+        const long long anti = static_cast<long long>(w[0]);
+        const long long neg = static_cast<long long>(w[1]);
+        o.energy = -static_cast<double>(s.Ji) * static_cast<double>(bonds - 2 * anti);
+        o.order[0] = static_cast<double>(static_cast<long long>(s.sites) - 2 * neg);
+    } else {
+        const long long e = static_cast<long long>(w[0]);
+        const long long m = static_cast<long long>(w[1]);
+        // model_check.cpp:45-51: 0.5 * sum_i energy_at(i), energy_at = -J s_i sum_row s_j
+        o.energy = 0.5 * static_cast<double>(-static_cast<long long>(s.Ji) * e);
+        o.order[0] = static_cast<double>(m);
+    }
+}
+
+}  // namespace
+
+// The CSR of square_lattice(Lx, Ly) (lattice/square.hpp:15-38 +
+// pair_graph.hpp:49-83), for systems created from the dimensions that still
+// need a table (non-fused schedules).
+void square_csr(System& s)
+{
+    const std::size_t Lx = s.Lx, Ly = s.Ly, n = Lx * Ly;
+    std::vector<std::uint32_t> deg(n, 0);
+    std::vector<std::pair<std::uint32_t, std::uint32_t>> edges;
+    edges.reserve(2 * n);
+    for (std::size_t y = 0; y < Ly; ++y)
+        for (std::size_t x = 0; x < Lx; ++x) {
+            const auto i = static_cast<std::uint32_t>(y * Lx + x);
+            if (Lx > 1) edges.push_back({i, static_cast<std::uint32_t>(y * Lx + (x + 1) % Lx)});
+            if (Ly > 1) edges.push_back({i, static_cast<std::uint32_t>(((y + 1) % Ly) * Lx + x)});
+        }
+    for (auto& e : edges) { ++deg[e.first]; ++deg[e.second]; }
+    s.h_offsets.assign(n + 1, 0);
+    for (std::size_t i = 0; i < n; ++i) s.h_offsets[i + 1] = s.h_offsets[i] + deg[i];
+    s.h_bonds.assign(s.h_offsets[n], 0);
+    std::vector<std::uint32_t> pos(s.h_offsets.begin(), s.h_offsets.end() - 1);
+    for (auto& e : edges) {
+        s.h_bonds[pos[e.first]++] = e.second;
+        s.h_bonds[pos[e.second]++] = e.first;
+    }
+    s.max_degree = 0;
+    for (std::size_t i = 0; i < n; ++i) s.max_degree = std::max(s.max_degree, deg[i]);
+    s.offsets.alloc(s.h_offsets.size() * 4);
+    UCMC_HIP(hipMemcpy(s.offsets.p, s.h_offsets.data(), s.h_offsets.size() * 4, hipMemcpyHostToDevice));
+    s.bonds.alloc(std::max<std::size_t>(s.h_bonds.size(), 1) * 4);
+    if (!s.h_bonds.empty())
+        UCMC_HIP(hipMemcpy(s.bonds.p, s.h_bonds.data(), s.h_bonds.size() * 4, hipMemcpyHostToDevice));
+}
+
+void ising_setup(System& s)
+{
+    const bool from_dims = s.h_offsets.empty();
+    std::size_t Lx = 0, Ly = 0;
+    bool square = false;
+    if (from_dims) {
+        Lx = s.Lx;
+        Ly = s.Ly;
+        square = true;
+    } else {
+        square = detect_square(s.h_offsets.data(), s.h_bonds.data(), s.sites, Lx, Ly);
+    }
+    if (square && Lx % 2 == 0 && Ly % 2 == 0 && Lx >= 4 && Ly >= 4) {
+        s.geom = Geom::Square;
+        s.Lx = Lx;
+        s.Ly = Ly;
+        s.ncolours = 2;
+        s.colour.resize(s.sites);
+        for (std::size_t i = 0; i < s.sites; ++i)
+            s.colour[i] = static_cast<std::uint8_t>(((i % Lx) + (i / Lx)) & 1u);
+        s.schedule = fused_ok(s) ? UCMC_SCHED_FUSED : UCMC_SCHED_CHECKERBOARD;
+        if (from_dims && !fused_ok(s)) square_csr(s);
+        if (from_dims && fused_ok(s)) s.max_degree = 4;
+    } else {
+        if (from_dims) {
+            s.geom = Geom::General;
+            square_csr(s);
+        }
+        s.geom = Geom::General;
+        s.ncolours = greedy_colouring(s.h_offsets.data(), s.h_bonds.data(), s.sites, s.colour);
+        s.schedule = UCMC_SCHED_CSR;
+    }
+    if (s.max_degree > kMaxDegTable) fail(UCMC_E_UNSUPPORTED, "Ising degree above 255");
+    upload_colour_lists(s);
+    const std::size_t bytes = (s.sites + 7) / 8 * 8;
+    s.field[0].alloc(bytes);
+    s.field[1].alloc(bytes);
+    UCMC_HIP(hipMemsetAsync(s.field[0].p, 1, s.sites, s.stream)); // all +1 (example/ising.hpp:24)
+    s.scratch2.alloc((2 * s.max_degree + 1) * 4);
+    s.scratch.alloc(4096);
+    UCMC_HIP(hipStreamSynchronize(s.stream));
+}
+
+void ising_metropolis(System& s, const ucmc_metropolis_params& p, ucmc_observables* series,
+                      std::uint8_t* trace_acc, float* trace_dE)
+{
+    if (p.proposal != UCMC_PROP_FLIP) fail(UCMC_E_UNSUPPORTED, "Ising takes the Flip proposal");
+    const bool simple = (p.flags & UCMC_FORCE_SIMPLE) || trace_acc;
+    if ((simple || !fused_ok(s)) && !s.offsets.p) square_csr(s);
+    const IsingTables t = make_tables(s, p.beta);
+    upload_csr_table(s, t);
+    const bool rec = series && (p.flags & (UCMC_RECORD_ENERGY | UCMC_RECORD_ORDER | UCMC_RECORD_ACCEPTS));
+    // scratch layout per sweep: [accepts, anti/e, neg/m]
+    DevBuf rec_buf;
+    if (rec) {
+        rec_buf.alloc(p.nsweeps * 3 * sizeof(unsigned long long));
+        UCMC_HIP(hipMemsetAsync(rec_buf.p, 0, rec_buf.bytes, s.stream));
+    }
+    DevBuf trace_buf;
+    float* d_dE = nullptr;
+    std::uint8_t* d_acc = nullptr;
+    if (trace_acc) {
+        trace_buf.alloc(s.sites * 5);
+        d_dE = trace_buf.as<float>();
+        d_acc = reinterpret_cast<std::uint8_t*>(d_dE + s.sites);
+    }
+    for (std::uint64_t k = 0; k < p.nsweeps; ++k) {
+        const std::uint64_t key = key_of(p.run_seed, p.sweep0 + k);
+        unsigned long long* slot = rec ? rec_buf.as<unsigned long long>() + 3 * k : nullptr;
+        launch_sweep(s, t, key, simple, (rec && (p.flags & UCMC_RECORD_ACCEPTS)) ? slot : nullptr,
+                     d_dE, d_acc);
+        if (rec && (p.flags & (UCMC_RECORD_ENERGY | UCMC_RECORD_ORDER))) reduce_launch(s, slot + 1);
+    }
+    if (rec) {
+        std::vector<unsigned long long> h(p.nsweeps * 3);
+        UCMC_HIP(hipMemcpyAsync(h.data(), rec_buf.p, h.size() * 8, hipMemcpyDeviceToHost, s.stream));
+        UCMC_HIP(hipStreamSynchronize(s.stream));
+        for (std::uint64_t k = 0; k < p.nsweeps; ++k) {
+            ucmc_observables o{};
+            finish_obs(s, &h[3 * k + 1], o);
+            o.accepted = h[3 * k];
+            series[k] = o;
+        }
+    }
+    if (trace_acc) {
+        UCMC_HIP(hipMemcpyAsync(trace_dE, d_dE, s.sites * 4, hipMemcpyDeviceToHost, s.stream));
+        UCMC_HIP(hipMemcpyAsync(trace_acc, d_acc, s.sites, hipMemcpyDeviceToHost, s.stream));
+    }
+    UCMC_HIP(hipStreamSynchronize(s.stream));
+}
+
+void ising_observe(System& s, ucmc_observables& o)
+{
+    DevBuf b;
+    b.alloc(16);
+    UCMC_HIP(hipMemsetAsync(b.p, 0, 16, s.stream));
+    reduce_launch(s, b.as<unsigned long long>());
+    unsigned long long h[2];
+    UCMC_HIP(hipMemcpyAsync(h, b.p, 16, hipMemcpyDeviceToHost, s.stream));
+    UCMC_HIP(hipStreamSynchronize(s.stream));
+    finish_obs(s, h, o);
+}
+
+std::uint64_t ising_bench(System& s, const ucmc_metropolis_params& p, int repeats, float& ms)
+{
+    if (p.proposal != UCMC_PROP_FLIP) fail(UCMC_E_UNSUPPORTED, "Ising takes the Flip proposal");
+    const bool simple = (p.flags & UCMC_FORCE_SIMPLE) != 0;
+    if ((simple || !fused_ok(s)) && !s.offsets.p) square_csr(s);
+    const IsingTables t = make_tables(s, p.beta);
+    upload_csr_table(s, t);
+    hipEvent_t e0, e1;
+    UCMC_HIP(hipEventCreate(&e0));
+    UCMC_HIP(hipEventCreate(&e1));
+    UCMC_HIP(hipStreamSynchronize(s.stream));
+    UCMC_HIP(hipEventRecord(e0, s.stream));
+    std::uint64_t launches = 0;
+    for (int r = 0; r < repeats; ++r)
+        for (std::uint64_t k = 0; k < p.nsweeps; ++k) {
+            launch_sweep(s, t, key_of(p.run_seed, p.sweep0 + r * p.nsweeps + k), simple, nullptr,
+                         nullptr, nullptr);
+            launches += (!simple && fused_ok(s)) ? 1 : s.ncolours;
+        }
+    UCMC_HIP(hipEventRecord(e1, s.stream));
+    UCMC_HIP(hipEventSynchronize(e1));
+    UCMC_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return launches;
+}
+
+}  // namespace ucmc

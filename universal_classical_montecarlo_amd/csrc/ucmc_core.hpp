@@ -1,0 +1,131 @@
+// ucmc_core.hpp -- host-side internals of the C ABI: the system object, error
+// plumbing and the launcher entry points each model file provides.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/ucmc.h"
+
+namespace ucmc {
+
+// Error carrying a ucmc status code; converted to a return value at the ABI.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] inline void fail(int code, const std::string& msg) { throw Error(code, msg); }
+
+inline void hip_check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) fail(UCMC_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define UCMC_HIP(x) ::ucmc::hip_check((x), #x)
+
+// Owning device buffer.
+struct DevBuf {
+    void* p = nullptr;
+    std::size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void alloc(std::size_t n)
+    {
+        release();
+        if (n == 0) n = 4;
+        UCMC_HIP(hipMalloc(&p, n));
+        bytes = n;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Regular-lattice geometry recognised from a CSR table.
+enum class Geom { General, Square, Cubic, Hyper4 };
+
+struct System {
+    int kind = 0;
+    int device = 0;
+    int N = 1;                 // components (VectorSpin N, SU(N) N)
+    hipStream_t stream = nullptr;
+    std::size_t sites = 0;
+    Geom geom = Geom::General;
+    std::size_t Lx = 0, Ly = 0, L = 0;
+    int schedule = 0;
+    int ncolours = 0;
+    std::vector<std::uint8_t> colour;       // host copy of the colouring
+    std::vector<std::uint32_t> h_offsets;   // host CSR copy (general graphs)
+    std::vector<std::uint32_t> h_bonds;
+    std::uint32_t max_degree = 0;
+
+    // couplings
+    std::int32_t Ji = 1;      // Ising J
+    double J = 1.0;           // vector / XY J
+    double model_beta = 1.0;  // gauge models: beta inside energy_diff_at
+    double K = 0.0;           // plaquette Ising K
+
+    // device tables
+    DevBuf offsets, bonds, edge_id;        // CSR + undirected edge ids (Wolff)
+    std::size_t n_edges = 0;
+    DevBuf colour_sites;                   // sites grouped by colour
+    std::vector<std::size_t> colour_start; // ncolours + 1 offsets into colour_sites
+    DevBuf field[2];                       // [cur] authoritative; [cur^1] scratch
+    int cur = 0;
+    DevBuf scratch;                        // reductions
+    DevBuf scratch2;
+    // Wolff: per-site membership stamps (site is in flip f's cluster iff
+    // stamp == mark_f, so no per-flip clear), the BFS queue, per-flip outputs
+    DevBuf stamp, queue, flip_out;
+    std::uint32_t wolff_mark = 0;
+
+    ~System();
+    std::size_t device_bytes() const;
+};
+
+// Per-thread last error message (ucmc_last_error).
+void set_last_error(const std::string& msg);
+
+// ---- per-model entry points (one translation unit each) ----
+void ising_setup(System& s);
+void ising_metropolis(System& s, const ucmc_metropolis_params& p, ucmc_observables* series,
+                      std::uint8_t* trace_acc, float* trace_dE);
+void ising_observe(System& s, ucmc_observables& out);
+std::uint64_t ising_bench(System& s, const ucmc_metropolis_params& p, int repeats, float& ms);
+
+void vector_setup(System& s);
+void vector_metropolis(System& s, const ucmc_metropolis_params& p, ucmc_observables* series,
+                       std::uint8_t* trace_acc, float* trace_dE);
+void vector_observe(System& s, ucmc_observables& out);
+std::uint64_t vector_bench(System& s, const ucmc_metropolis_params& p, int repeats, float& ms);
+void square_csr(System& s);  // materialise square_lattice(Lx, Ly) for a dims-created system
+void wolff_run(System& s, const ucmc_wolff_params& p, std::uint64_t* sizes,
+               ucmc_observables* series);
+
+// geometry helpers (abi.hip)
+bool detect_square(const std::uint32_t* off, const std::uint32_t* bonds, std::size_t n,
+                   std::size_t& Lx, std::size_t& Ly);
+bool detect_cubic(const std::uint32_t* off, const std::uint32_t* bonds, std::size_t n,
+                  std::size_t& L);
+int greedy_colouring(const std::uint32_t* off, const std::uint32_t* bonds, std::size_t n,
+                     std::vector<std::uint8_t>& colour);
+void upload_colour_lists(System& s);
+std::size_t build_edge_ids(System& s); // uploads edge_id; returns #undirected edges
+
+// Accept threshold of driver.hpp:83-84 as a count of accepted 32-bit draws
+// (see ising.hip): k* such that log(k 2^-32) < -beta dE  <=>  k < k*.
+std::uint64_t accept_threshold(double beta, double dE);
+// driver.hpp:195: k 2^-32 < 1 - exp(-beta dE)  <=>  k < k*.
+std::uint64_t wolff_threshold(double beta, double dE);
+
+}  // namespace ucmc

@@ -1,0 +1,419 @@
+// vector.hip -- O(N) vector spins (example/vector_spin.hpp:14-83) under
+// Metropolis with UniformSphereN<N> or LocalRotation<N>{delta}
+// (dof/vector.hpp:35-122), driven as in driver.hpp:73-87.
+//
+// Field on the device: fp32 SoA, component k of site i at [k * sites + i]
+// (coalesced per component).  One launch per colour class: (x + y + z) & 1 on
+// a periodic simple-cubic lattice with even L, (x + y) & 1 on an even square
+// lattice, a greedy colouring otherwise.  Each thread owns one site of the
+// class and draws its proposal and accept uniforms from its own
+// Pcg32(key, site) stream in the reference's order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "ucmc_core.hpp"
+#include "ucmc_device.hpp"
+
+namespace ucmc {
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+struct VecArgs {
+    float* f;                       // SoA [N][n]
+    std::uint32_t n;
+    const std::uint32_t* offsets;   // CSR (general / square)
+    const std::uint32_t* bonds;
+    const std::uint32_t* sites;     // colour class (CSR / square paths)
+    std::uint32_t count;
+    std::uint32_t L;                // cubic: edge length
+    std::uint32_t colour;           // cubic: parity of the class
+    std::uint64_t key;
+    float J;
+    double beta;
+    float delta;
+    int kind;                       // 0 sphere, 1 rotation
+    unsigned long long* accepts;
+    float* trace_dE;
+    std::uint8_t* trace_acc;
+};
+
+// u in [0, 1) as float, from a 32-bit draw k: (float)(k * 2^-32) may round to
+// 1.0f for k > 2^32 - 2^7; the affine maps below stay in range either way.
+__device__ __forceinline__ float uf(std::uint32_t k) { return static_cast<float>(k) * 0x1.0p-32f; }
+
+// dof/vector.hpp:35-61 in fp32: N == 2 angle, N >= 3 height then recursion.
+template <int N>
+__device__ __forceinline__ void sphere(Pcg& r, float* out)
+{
+    if constexpr (N == 2) {
+        const float phi = static_cast<float>(2.0 * kPi) * uf(r.next());
+        float s, c;
+        sincosf(phi, &s, &c);
+        out[0] = c;
+        out[1] = s;
+    } else {
+        const float z = 2.0f * uf(r.next()) - 1.0f;
+        const float rad = sqrtf(fmaxf(1.0f - z * z, 0.0f));
+        sphere<N - 1>(r, out);
+#pragma unroll
+        for (int k = 0; k < N - 1; ++k) out[k] *= rad;
+        out[N - 1] = z;
+    }
+}
+
+// dof/vector.hpp:79-118 in fp32: uniform oriented 2-frame by projection and
+// rejection, rotation by delta (2u - 1) in that plane.
+template <int N>
+__device__ __forceinline__ void rotate(Pcg& r, float delta, const float* cur, float* out)
+{
+    float u[N], v[N];
+    sphere<N>(r, u);
+    const float eps = static_cast<float>(64.0 * 2.220446049250313e-16);
+    for (;;) {
+        sphere<N>(r, v);
+        float proj = 0.f;
+#pragma unroll
+        for (int k = 0; k < N; ++k) proj += v[k] * u[k];
+        float r2 = 0.f;
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            v[k] -= proj * u[k];
+            r2 += v[k] * v[k];
+        }
+        if (r2 > eps) {
+            const float inv = 1.0f / sqrtf(r2);
+#pragma unroll
+            for (int k = 0; k < N; ++k) v[k] *= inv;
+            break;
+        }
+    }
+    const float th = delta * (2.0f * uf(r.next()) - 1.0f);
+    float sn, c;
+    sincosf(th, &sn, &c);
+    float cu = 0.f, cv = 0.f;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        cu += u[k] * cur[k];
+        cv += v[k] * cur[k];
+    }
+#pragma unroll
+    for (int k = 0; k < N; ++k)
+        out[k] = cur[k] + (c - 1.0f) * (cu * u[k] + cv * v[k]) + sn * (cu * v[k] - cv * u[k]);
+}
+
+template <int N, bool CUBIC, bool TRACE>
+__global__ __launch_bounds__(256) void vector_colour(VecArgs a)
+{
+    const std::uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    bool acc = false;
+    if (t < a.count) {
+        std::uint32_t i, nb[6];
+        int deg = 0;
+        if constexpr (CUBIC) {
+            const std::uint32_t L = a.L, half = L >> 1;
+            const std::uint32_t xh = t % half, yz = t / half, y = yz % L, z = yz / L;
+            const std::uint32_t x = 2 * xh + ((a.colour + y + z) & 1u);
+            i = (z * L + y) * L + x;
+            nb[0] = (z * L + y) * L + (x + 1 == L ? 0 : x + 1);
+            nb[1] = (z * L + y) * L + (x == 0 ? L - 1 : x - 1);
+            nb[2] = (z * L + (y + 1 == L ? 0 : y + 1)) * L + x;
+            nb[3] = (z * L + (y == 0 ? L - 1 : y - 1)) * L + x;
+            nb[4] = ((z + 1 == L ? 0 : z + 1) * L + y) * L + x;
+            nb[5] = ((z == 0 ? L - 1 : z - 1) * L + y) * L + x;
+            deg = 6;
+        } else {
+            i = a.sites[t];
+        }
+        float s[N], p[N], h[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            s[k] = a.f[k * a.n + i];
+            h[k] = 0.f;
+        }
+        if constexpr (CUBIC) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q)
+#pragma unroll
+                for (int k = 0; k < N; ++k) h[k] += a.f[k * a.n + nb[q]];
+        } else {
+            for (std::uint32_t q = a.offsets[i]; q < a.offsets[i + 1]; ++q) {
+                const std::uint32_t j = a.bonds[q];
+#pragma unroll
+                for (int k = 0; k < N; ++k) h[k] += a.f[k * a.n + j];
+            }
+        }
+        (void)deg;
+        Pcg r;
+        r.seed(a.key, i);
+        if (a.kind == 0) sphere<N>(r, p);
+        else rotate<N>(r, a.delta, s, p);
+        // example/vector_spin.hpp:50-60: -J sum_j (p.s_j - s.s_j) = -J (p - s).h
+        float d = 0.f;
+#pragma unroll
+        for (int k = 0; k < N; ++k) d += (p[k] - s[k]) * h[k];
+        const float dE = -a.J * d;
+        const double log_p = -a.beta * static_cast<double>(dE) + 0.0;
+        acc = log_p >= 0.0 || log(r.uni()) < log_p;
+        if (acc) {
+            // keep |S| = 1 against fp32 drift
+            float nn = 0.f;
+#pragma unroll
+            for (int k = 0; k < N; ++k) nn += p[k] * p[k];
+            const float inv = rsqrtf(nn);
+#pragma unroll
+            for (int k = 0; k < N; ++k) a.f[k * a.n + i] = p[k] * inv;
+        }
+        if constexpr (TRACE) {
+            a.trace_dE[i] = dE;
+            a.trace_acc[i] = acc ? 1 : 0;
+        }
+    }
+    if (a.accepts) {
+        const unsigned long long c = __popcll(__ballot(acc));
+        if ((threadIdx.x & 63) == 0 && c) atomicAdd(a.accepts, c);
+    }
+}
+
+// E = -J sum over CSR rows of s_i.s_j / 2 (model_check.cpp:45-51), M = sum s_i.
+template <int N>
+__global__ __launch_bounds__(256) void vector_reduce(const float* f, std::uint32_t n,
+                                                     const std::uint32_t* off,
+                                                     const std::uint32_t* bonds, std::uint32_t L,
+                                                     double* out)
+{
+    double e = 0.0, m[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) m[k] = 0.0;
+    for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float s[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            s[k] = f[k * n + i];
+            m[k] += s[k];
+        }
+        if (L) { // cubic: +x, +y, +z bonds once each
+            const std::uint32_t x = i % L, y = (i / L) % L, z = i / (L * L);
+            const std::uint32_t nb[3] = {(z * L + y) * L + (x + 1) % L, (z * L + (y + 1) % L) * L + x,
+                                         (((z + 1) % L) * L + y) * L + x};
+            for (int q = 0; q < 3; ++q) {
+                double d = 0.0;
+#pragma unroll
+                for (int k = 0; k < N; ++k) d += static_cast<double>(s[k]) * f[k * n + nb[q]];
+                e += d;
+            }
+        } else {
+            double d = 0.0;
+            for (std::uint32_t q = off[i]; q < off[i + 1]; ++q)
+#pragma unroll
+                for (int k = 0; k < N; ++k) d += static_cast<double>(s[k]) * f[k * n + bonds[q]];
+            e += 0.5 * d;
+        }
+    }
+    e = wave_sum(e);
+#pragma unroll
+    for (int k = 0; k < N; ++k) m[k] = wave_sum(m[k]);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(out, e);
+#pragma unroll
+        for (int k = 0; k < N; ++k) atomicAdd(out + 1 + k, m[k]);
+    }
+}
+
+template <int N>
+void launch_colour(System& s, const ucmc_metropolis_params& p, std::uint64_t key, int c,
+                   unsigned long long* dacc, float* tdE, std::uint8_t* tacc)
+{
+    VecArgs a{};
+    a.f = s.field[s.cur].as<float>();
+    a.n = static_cast<std::uint32_t>(s.sites);
+    a.offsets = s.offsets.as<std::uint32_t>();
+    a.bonds = s.bonds.as<std::uint32_t>();
+    a.key = key;
+    a.J = static_cast<float>(s.J);
+    a.beta = p.beta;
+    a.delta = static_cast<float>(p.width);
+    a.kind = p.proposal == UCMC_PROP_SPHERE ? 0 : 1;
+    a.accepts = dacc;
+    a.trace_dE = tdE;
+    a.trace_acc = tacc;
+    const bool cubic = s.geom == Geom::Cubic;
+    if (cubic) {
+        a.L = static_cast<std::uint32_t>(s.L);
+        a.colour = static_cast<std::uint32_t>(c);
+        a.count = static_cast<std::uint32_t>(s.sites / 2);
+    } else {
+        a.sites = s.colour_sites.as<std::uint32_t>() + s.colour_start[c];
+        a.count = static_cast<std::uint32_t>(s.colour_start[c + 1] - s.colour_start[c]);
+    }
+    if (a.count == 0) return;
+    const dim3 grid((a.count + 255) / 256), block(256);
+    if (cubic) {
+        if (tdE) hipLaunchKernelGGL((vector_colour<N, true, true>), grid, block, 0, s.stream, a);
+        else hipLaunchKernelGGL((vector_colour<N, true, false>), grid, block, 0, s.stream, a);
+    } else {
+        if (tdE) hipLaunchKernelGGL((vector_colour<N, false, true>), grid, block, 0, s.stream, a);
+        else hipLaunchKernelGGL((vector_colour<N, false, false>), grid, block, 0, s.stream, a);
+    }
+    UCMC_HIP(hipGetLastError());
+}
+
+void sweep(System& s, const ucmc_metropolis_params& p, std::uint64_t key,
+           unsigned long long* dacc, float* tdE, std::uint8_t* tacc)
+{
+    for (int c = 0; c < s.ncolours; ++c) {
+        switch (s.N) {
+        case 2: launch_colour<2>(s, p, key, c, dacc, tdE, tacc); break;
+        case 3: launch_colour<3>(s, p, key, c, dacc, tdE, tacc); break;
+        case 4: launch_colour<4>(s, p, key, c, dacc, tdE, tacc); break;
+        default: fail(UCMC_E_UNSUPPORTED, "N");
+        }
+    }
+}
+
+void reduce(System& s, double* out)
+{
+    const unsigned blocks = static_cast<unsigned>(std::min<std::size_t>((s.sites + 255) / 256, 1024));
+    const std::uint32_t L = s.geom == Geom::Cubic ? static_cast<std::uint32_t>(s.L) : 0u;
+    auto f = s.field[s.cur].as<float>();
+    const auto n = static_cast<std::uint32_t>(s.sites);
+    switch (s.N) {
+    case 2: hipLaunchKernelGGL(vector_reduce<2>, dim3(blocks), dim3(256), 0, s.stream, f, n, s.offsets.as<std::uint32_t>(), s.bonds.as<std::uint32_t>(), L, out); break;
+    case 3: hipLaunchKernelGGL(vector_reduce<3>, dim3(blocks), dim3(256), 0, s.stream, f, n, s.offsets.as<std::uint32_t>(), s.bonds.as<std::uint32_t>(), L, out); break;
+    case 4: hipLaunchKernelGGL(vector_reduce<4>, dim3(blocks), dim3(256), 0, s.stream, f, n, s.offsets.as<std::uint32_t>(), s.bonds.as<std::uint32_t>(), L, out); break;
+    }
+    UCMC_HIP(hipGetLastError());
+}
+
+void finish(const System& s, const double* w, ucmc_observables& o)
+{
+    o.energy = -s.J * w[0];
+    for (int k = 0; k < s.N; ++k) o.order[k] = w[1 + k];
+}
+
+void check_proposal(const ucmc_metropolis_params& p)
+{
+    if (p.proposal != UCMC_PROP_SPHERE && p.proposal != UCMC_PROP_ROTATION)
+        fail(UCMC_E_UNSUPPORTED, "VectorSpin takes UniformSphereN or LocalRotation");
+    if (p.hits != 1) fail(UCMC_E_UNSUPPORTED, "VectorSpin runs one hit per site");
+}
+
+}  // namespace
+
+void vector_setup(System& s)
+{
+    std::size_t L = 0, Lx = 0, Ly = 0;
+    if (detect_cubic(s.h_offsets.data(), s.h_bonds.data(), s.sites, L) && L % 2 == 0) {
+        s.geom = Geom::Cubic;
+        s.L = L;
+        s.ncolours = 2;
+        s.colour.resize(s.sites);
+        for (std::size_t i = 0; i < s.sites; ++i)
+            s.colour[i] = static_cast<std::uint8_t>(((i % L) + (i / L) % L + i / (L * L)) & 1u);
+        s.schedule = UCMC_SCHED_CUBIC;
+    } else if (detect_square(s.h_offsets.data(), s.h_bonds.data(), s.sites, Lx, Ly) && Lx % 2 == 0 &&
+               Ly % 2 == 0) {
+        s.geom = Geom::Square;
+        s.Lx = Lx;
+        s.Ly = Ly;
+        s.ncolours = 2;
+        s.colour.resize(s.sites);
+        for (std::size_t i = 0; i < s.sites; ++i)
+            s.colour[i] = static_cast<std::uint8_t>(((i % Lx) + (i / Lx)) & 1u);
+        s.schedule = UCMC_SCHED_CHECKERBOARD;
+    } else {
+        s.geom = Geom::General;
+        s.ncolours = greedy_colouring(s.h_offsets.data(), s.h_bonds.data(), s.sites, s.colour);
+        s.schedule = UCMC_SCHED_CSR;
+    }
+    upload_colour_lists(s);
+    s.field[0].alloc(s.sites * s.N * sizeof(float));
+    // example/vector_spin.hpp:27-28: every spin along the last axis
+    std::vector<float> init(s.sites * s.N, 0.f);
+    for (std::size_t i = 0; i < s.sites; ++i) init[(s.N - 1) * s.sites + i] = 1.f;
+    UCMC_HIP(hipMemcpy(s.field[0].p, init.data(), init.size() * 4, hipMemcpyHostToDevice));
+    s.scratch.alloc(64 * sizeof(double));
+}
+
+void vector_metropolis(System& s, const ucmc_metropolis_params& p, ucmc_observables* series,
+                       std::uint8_t* trace_acc, float* trace_dE)
+{
+    check_proposal(p);
+    const bool rec = series && (p.flags & (UCMC_RECORD_ENERGY | UCMC_RECORD_ORDER | UCMC_RECORD_ACCEPTS));
+    const std::size_t stride = 8; // doubles per sweep slot: [accepts(u64), e, m0..m3, pad]
+    DevBuf rec_buf;
+    if (rec) {
+        rec_buf.alloc(p.nsweeps * stride * 8);
+        UCMC_HIP(hipMemsetAsync(rec_buf.p, 0, rec_buf.bytes, s.stream));
+    }
+    DevBuf tb;
+    float* d_dE = nullptr;
+    std::uint8_t* d_acc = nullptr;
+    if (trace_acc) {
+        tb.alloc(s.sites * 5);
+        d_dE = tb.as<float>();
+        d_acc = reinterpret_cast<std::uint8_t*>(d_dE + s.sites);
+    }
+    for (std::uint64_t k = 0; k < p.nsweeps; ++k) {
+        double* slot = rec ? rec_buf.as<double>() + stride * k : nullptr;
+        sweep(s, p, key_of(p.run_seed, p.sweep0 + k),
+              (rec && (p.flags & UCMC_RECORD_ACCEPTS)) ? reinterpret_cast<unsigned long long*>(slot) : nullptr,
+              d_dE, d_acc);
+        if (rec && (p.flags & (UCMC_RECORD_ENERGY | UCMC_RECORD_ORDER))) reduce(s, slot + 1);
+    }
+    if (rec) {
+        std::vector<double> h(p.nsweeps * stride);
+        UCMC_HIP(hipMemcpyAsync(h.data(), rec_buf.p, h.size() * 8, hipMemcpyDeviceToHost, s.stream));
+        UCMC_HIP(hipStreamSynchronize(s.stream));
+        for (std::uint64_t k = 0; k < p.nsweeps; ++k) {
+            ucmc_observables o{};
+            finish(s, &h[stride * k + 1], o);
+            std::memcpy(&o.accepted, &h[stride * k], 8);
+            series[k] = o;
+        }
+    }
+    if (trace_acc) {
+        UCMC_HIP(hipMemcpyAsync(trace_dE, d_dE, s.sites * 4, hipMemcpyDeviceToHost, s.stream));
+        UCMC_HIP(hipMemcpyAsync(trace_acc, d_acc, s.sites, hipMemcpyDeviceToHost, s.stream));
+    }
+    UCMC_HIP(hipStreamSynchronize(s.stream));
+}
+
+void vector_observe(System& s, ucmc_observables& o)
+{
+    UCMC_HIP(hipMemsetAsync(s.scratch.p, 0, 8 * sizeof(double), s.stream));
+    reduce(s, s.scratch.as<double>());
+    double h[8];
+    UCMC_HIP(hipMemcpyAsync(h, s.scratch.p, sizeof(h), hipMemcpyDeviceToHost, s.stream));
+    UCMC_HIP(hipStreamSynchronize(s.stream));
+    finish(s, h, o);
+}
+
+std::uint64_t vector_bench(System& s, const ucmc_metropolis_params& p, int repeats, float& ms)
+{
+    check_proposal(p);
+    hipEvent_t e0, e1;
+    UCMC_HIP(hipEventCreate(&e0));
+    UCMC_HIP(hipEventCreate(&e1));
+    UCMC_HIP(hipStreamSynchronize(s.stream));
+    UCMC_HIP(hipEventRecord(e0, s.stream));
+    std::uint64_t launches = 0;
+    for (int r = 0; r < repeats; ++r)
+        for (std::uint64_t k = 0; k < p.nsweeps; ++k) {
+            sweep(s, p, key_of(p.run_seed, p.sweep0 + r * p.nsweeps + k), nullptr, nullptr, nullptr);
+            launches += s.ncolours;
+        }
+    UCMC_HIP(hipEventRecord(e1, s.stream));
+    UCMC_HIP(hipEventSynchronize(e1));
+    UCMC_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return launches;
+}
+
+}  // namespace ucmc

@@ -1,0 +1,336 @@
+// wolff.hip -- Wolff single-cluster flips (driver.hpp:136-254) on the GPU for
+// the pair models Ising (FixedAxis, dof/ising.hpp:53-62) and VectorSpin<N>
+// (UniformSphereN<N> axis, dof/vector.hpp:35-61).
+//
+// The reference grows the cluster depth-first with one sequential RNG.  The
+// cluster it samples is the connected component of the seed in the random
+// graph of bonds activated with p = 1 - exp(-beta dE_bond), dE_bond taken on
+// the pre-flip spins (driver.hpp:180-200).  Here every undirected edge e draws
+// its own uniform from Pcg32(k_f, 1 + e), so the component does not depend on
+// the traversal order and a parallel breadth-first growth reproduces the CPU
+// replay exactly.
+//
+// One workgroup of 1024 threads runs all flips of a call back to back: each
+// level of the frontier is expanded by all waves; a wave appends its newly
+// claimed sites with one ballot + one LDS atomic.  Membership is a per-site
+// stamp compared with the flip's mark, so no O(N) clear per flip (the clear at
+// driver.hpp:156 is what bounds the reference at large N).  After the growth
+// the boundary-bond energy change and the order-parameter change are reduced,
+// then the cluster is reflected.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "ucmc_core.hpp"
+#include "ucmc_device.hpp"
+
+namespace ucmc {
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr int kThreads = 1024;
+
+struct WolffArgs {
+    void* field;                   // int8 [n] (Ising) or fp32 SoA [N][n]
+    std::uint32_t n;
+    const std::uint32_t* offsets;
+    const std::uint32_t* bonds;
+    const std::uint32_t* edge_id;
+    std::uint32_t* stamp;
+    std::uint32_t* queue;
+    double* out;                   // per flip: [size, dE, dM0..dM3]
+    std::uint64_t run_seed;
+    std::uint64_t flip0;
+    std::uint32_t nflips;
+    std::uint32_t mark0;           // mark of flip f is mark0 + f
+    double beta;
+    double J;
+    std::uint64_t thr_aligned;     // Ising: accepted draws when s_i s_j = +1
+    std::uint64_t thr_anti;        //        and when s_i s_j = -1
+};
+
+// Double-precision products and sums without contraction, matching the
+// oracle's -ffp-contract=off arithmetic.
+__device__ __forceinline__ double dmul(double a, double b) { return __dmul_rn(a, b); }
+__device__ __forceinline__ double dadd(double a, double b) { return __dadd_rn(a, b); }
+
+template <int N>
+__device__ void sphere_d(Pcg& r, double* out)
+{
+    if constexpr (N == 1) {
+        out[0] = r.uni() < 0.5 ? -1.0 : 1.0;
+    } else if constexpr (N == 2) {
+        const double phi = dmul(2.0 * kPi, r.uni());
+        out[0] = cos(phi);
+        out[1] = sin(phi);
+    } else {
+        const double z = dadd(dmul(2.0, r.uni()), -1.0);
+        const double rad = sqrt(dadd(1.0, -dmul(z, z)));
+        sphere_d<N - 1>(r, out);
+        for (int k = 0; k < N - 1; ++k) out[k] = dmul(out[k], rad);
+        out[N - 1] = z;
+    }
+}
+
+// N == 1: Ising (int8 field), else O(N) fp32 SoA.
+template <int N>
+struct Field;
+
+template <>
+struct Field<1> {
+    const std::int8_t* f;
+    __device__ double proj(std::uint32_t i, const double*) const { return f[i]; }
+};
+
+template <int N>
+struct Field {
+    const float* f;
+    std::uint32_t n;
+    __device__ double proj(std::uint32_t i, const double* r) const
+    {
+        double a = 0.0;
+        for (int k = 0; k < N; ++k) a = dadd(a, dmul(static_cast<double>(f[k * n + i]), r[k]));
+        return a;
+    }
+};
+
+template <int N>
+__global__ __launch_bounds__(kThreads) void wolff_flips(WolffArgs a)
+{
+    __shared__ std::uint32_t s_head, s_tail, s_seed;
+    __shared__ double s_r[4];
+    __shared__ double s_red[1 + 4];
+
+    const std::uint32_t tid = threadIdx.x, lane = tid & 63u;
+    Field<N> fld;
+    if constexpr (N == 1) fld.f = static_cast<const std::int8_t*>(a.field);
+    else {
+        fld.f = static_cast<const float*>(a.field);
+        fld.n = a.n;
+    }
+
+    for (std::uint32_t fi = 0; fi < a.nflips; ++fi) {
+        const std::uint32_t mark = a.mark0 + fi;
+        const std::uint64_t key = key_of(a.run_seed, kWolffCounterBase + a.flip0 + fi);
+        if (tid == 0) {
+            // driver.hpp:154-159: the axis first, then the seed site
+            Pcg ctl;
+            ctl.seed(key, 0);
+            double r[4] = {1.0, 0.0, 0.0, 0.0};
+            if constexpr (N > 1) sphere_d<N>(ctl, r);
+            for (int k = 0; k < 4; ++k) s_r[k] = r[k];
+            const std::uint32_t seed =
+                static_cast<std::uint32_t>(dmul(ctl.uni(), static_cast<double>(a.n)));
+            s_seed = seed;
+            a.stamp[seed] = mark;
+            a.queue[0] = seed;
+            s_head = 0;
+            s_tail = 1;
+            for (int k = 0; k < 5; ++k) s_red[k] = 0.0;
+        }
+        __syncthreads();
+        double r[4];
+        for (int k = 0; k < 4; ++k) r[k] = s_r[k];
+
+        // ---- breadth-first growth, one level per iteration ----
+        for (;;) {
+            const std::uint32_t h0 = s_head, t0 = s_tail;
+            __syncthreads();
+            if (h0 == t0) break;
+            for (std::uint32_t base = h0; base < t0; base += kThreads) {
+                const std::uint32_t idx = base + tid;
+                const bool have = idx < t0;
+                const std::uint32_t i = have ? a.queue[idx] : 0u;
+                const std::uint32_t q0 = have ? a.offsets[i] : 0u, q1 = have ? a.offsets[i + 1] : 0u;
+                const double ai = have ? fld.proj(i, r) : 0.0;
+                // iterate to the wave's largest degree so every lane joins each ballot
+                std::uint32_t deg = q1 - q0, wdeg = deg;
+                for (int off = 32; off > 0; off >>= 1) wdeg = max(wdeg, static_cast<std::uint32_t>(__shfl_xor(static_cast<int>(wdeg), off, 64)));
+                for (std::uint32_t d = 0; d < wdeg; ++d) {
+                    bool won = false;
+                    std::uint32_t j = 0;
+                    if (d < deg) {
+                        const std::uint32_t q = q0 + d;
+                        j = a.bonds[q];
+                        const std::uint32_t sj = a.stamp[j];
+                        if (sj != mark) {
+                            const std::uint64_t es = first_state(key, 1ull + a.edge_id[q]);
+                            const std::uint32_t u = pcg_out(es);
+                            bool act;
+                            if constexpr (N == 1) {
+                                const bool aligned = fld.proj(i, r) * fld.proj(j, r) > 0.0;
+                                act = static_cast<std::uint64_t>(u) < (aligned ? a.thr_aligned : a.thr_anti);
+                            } else {
+                                const double aj = fld.proj(j, r);
+                                const double dE = dmul(2.0 * a.J, dmul(ai, aj));
+                                const double p = 1.0 - exp(-a.beta * dE);
+                                act = static_cast<double>(u) * 0x1.0p-32 < p;
+                            }
+                            if (act) won = atomicCAS(&a.stamp[j], sj, mark) == sj;
+                        }
+                    }
+                    const unsigned long long m = __ballot(won);
+                    if (m) {
+                        const int leader = __ffsll(static_cast<long long>(m)) - 1;
+                        std::uint32_t pos = 0;
+                        if (static_cast<int>(lane) == leader) pos = atomicAdd(&s_tail, static_cast<std::uint32_t>(__popcll(m)));
+                        pos = static_cast<std::uint32_t>(__shfl(static_cast<int>(pos), leader, 64));
+                        if (won) {
+                            const unsigned long long below = m & ((1ull << lane) - 1ull);
+                            a.queue[pos + __popcll(below)] = j;
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            if (tid == 0) s_head = t0;
+            __syncthreads();
+        }
+        const std::uint32_t size = s_tail;
+
+        // ---- energy / order change on the pre-flip field ----
+        double de = 0.0, dm[4] = {0.0, 0.0, 0.0, 0.0};
+        for (std::uint32_t idx = tid; idx < size; idx += kThreads) {
+            const std::uint32_t i = a.queue[idx];
+            if constexpr (N == 1) {
+                const int si = fld.f[i];
+                int sum = 0;
+                for (std::uint32_t q = a.offsets[i]; q < a.offsets[i + 1]; ++q) {
+                    const std::uint32_t j = a.bonds[q];
+                    if (a.stamp[j] != mark) sum += fld.f[j];
+                }
+                de += 2.0 * a.J * static_cast<double>(si * sum);
+                dm[0] += -2.0 * si;
+            } else {
+                const double ai = fld.proj(i, r);
+                double h[N];
+                for (int k = 0; k < N; ++k) h[k] = 0.0;
+                for (std::uint32_t q = a.offsets[i]; q < a.offsets[i + 1]; ++q) {
+                    const std::uint32_t j = a.bonds[q];
+                    if (a.stamp[j] != mark)
+                        for (int k = 0; k < N; ++k) h[k] += fld.f[k * a.n + j];
+                }
+                for (int k = 0; k < N; ++k) {
+                    const double s = fld.f[k * a.n + i];
+                    const float sn = static_cast<float>(dadd(s, -dmul(dmul(2.0, ai), r[k])));
+                    de += -a.J * (static_cast<double>(sn) - s) * h[k];
+                    dm[k] += static_cast<double>(sn) - s;
+                }
+            }
+        }
+        de = wave_sum(de);
+        for (int k = 0; k < N; ++k) dm[k] = wave_sum(dm[k]);
+        if (lane == 0) {
+            atomicAdd(&s_red[0], de);
+            for (int k = 0; k < N; ++k) atomicAdd(&s_red[1 + k], dm[k]);
+        }
+        __syncthreads();
+
+        // ---- reflect the cluster: s - 2 (s.r) r (driver.hpp:172-175) ----
+        for (std::uint32_t idx = tid; idx < size; idx += kThreads) {
+            const std::uint32_t i = a.queue[idx];
+            if constexpr (N == 1) {
+                std::int8_t* f = static_cast<std::int8_t*>(a.field);
+                f[i] = static_cast<std::int8_t>(-f[i]);
+            } else {
+                float* f = static_cast<float*>(a.field);
+                const double ai = fld.proj(i, r);
+                for (int k = 0; k < N; ++k) {
+                    const double s = f[k * a.n + i];
+                    f[k * a.n + i] = static_cast<float>(dadd(s, -dmul(dmul(2.0, ai), r[k])));
+                }
+            }
+        }
+        if (tid == 0) {
+            double* o = a.out + 6 * static_cast<std::size_t>(fi);
+            o[0] = static_cast<double>(size);
+            o[1] = s_red[0];
+            for (int k = 0; k < 4; ++k) o[2 + k] = k < N ? s_red[1 + k] : 0.0;
+        }
+        __threadfence();
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+void wolff_run(System& s, const ucmc_wolff_params& p, std::uint64_t* sizes, ucmc_observables* series)
+{
+    if (!(p.beta == p.beta)) fail(UCMC_E_INVALID_ARGUMENT, "beta is NaN");
+    if (p.nflips == 0) return;
+    if (p.nflips > (1u << 30)) fail(UCMC_E_INVALID_ARGUMENT, "too many flips in one call");
+    if (s.kind == UCMC_ISING && !s.offsets.p) square_csr(s);
+    if (!s.edge_id.p) build_edge_ids(s);
+    if (!s.stamp.p) {
+        s.stamp.alloc(s.sites * 4);
+        UCMC_HIP(hipMemsetAsync(s.stamp.p, 0, s.sites * 4, s.stream));
+        s.queue.alloc(s.sites * 4);
+        s.wolff_mark = 0;
+    }
+    // marks are 1 .. 2^32-1; restart them (one clear) before they wrap
+    if (static_cast<std::uint64_t>(s.wolff_mark) + p.nflips >= 0xFFFFFFFFull) {
+        UCMC_HIP(hipMemsetAsync(s.stamp.p, 0, s.sites * 4, s.stream));
+        s.wolff_mark = 0;
+    }
+    ucmc_observables start{};
+    if (series) {
+        if (s.kind == UCMC_ISING) ising_observe(s, start);
+        else vector_observe(s, start);
+    }
+    DevBuf out;
+    out.alloc(p.nflips * 6 * sizeof(double));
+    WolffArgs a{};
+    a.field = s.field[s.cur].p;
+    a.n = static_cast<std::uint32_t>(s.sites);
+    a.offsets = s.offsets.as<std::uint32_t>();
+    a.bonds = s.bonds.as<std::uint32_t>();
+    a.edge_id = s.edge_id.as<std::uint32_t>();
+    a.stamp = s.stamp.as<std::uint32_t>();
+    a.queue = s.queue.as<std::uint32_t>();
+    a.out = out.as<double>();
+    a.run_seed = p.run_seed;
+    a.flip0 = p.flip0;
+    a.nflips = static_cast<std::uint32_t>(p.nflips);
+    a.mark0 = s.wolff_mark + 1;
+    a.beta = p.beta;
+    if (s.kind == UCMC_ISING) {
+        a.J = static_cast<double>(s.Ji);
+        // driver.hpp:195 with dE = energy_diff_bond = 2 J s_i s_j
+        a.thr_aligned = wolff_threshold(p.beta, 2.0 * s.Ji);
+        a.thr_anti = wolff_threshold(p.beta, -2.0 * s.Ji);
+        hipLaunchKernelGGL(wolff_flips<1>, dim3(1), dim3(kThreads), 0, s.stream, a);
+    } else {
+        a.J = s.J;
+        switch (s.N) {
+        case 2: hipLaunchKernelGGL(wolff_flips<2>, dim3(1), dim3(kThreads), 0, s.stream, a); break;
+        case 3: hipLaunchKernelGGL(wolff_flips<3>, dim3(1), dim3(kThreads), 0, s.stream, a); break;
+        case 4: hipLaunchKernelGGL(wolff_flips<4>, dim3(1), dim3(kThreads), 0, s.stream, a); break;
+        default: fail(UCMC_E_UNSUPPORTED, "N");
+        }
+    }
+    UCMC_HIP(hipGetLastError());
+    s.wolff_mark += static_cast<std::uint32_t>(p.nflips);
+    std::vector<double> h(p.nflips * 6);
+    UCMC_HIP(hipMemcpyAsync(h.data(), out.p, h.size() * 8, hipMemcpyDeviceToHost, s.stream));
+    UCMC_HIP(hipStreamSynchronize(s.stream));
+    double e = start.energy;
+    double m[4] = {start.order[0], start.order[1], start.order[2], start.order[3]};
+    for (std::uint64_t f = 0; f < p.nflips; ++f) {
+        if (sizes) sizes[f] = static_cast<std::uint64_t>(h[6 * f]);
+        if (series) {
+            e += h[6 * f + 1];
+            for (int k = 0; k < 4; ++k) m[k] += h[6 * f + 2 + k];
+            ucmc_observables o{};
+            o.energy = e;
+            for (int k = 0; k < 4; ++k) o.order[k] = m[k];
+            o.cluster = static_cast<std::uint64_t>(h[6 * f]);
+            series[f] = o;
+        }
+    }
+}
+
+}  // namespace ucmc
