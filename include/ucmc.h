@@ -74,6 +74,7 @@ typedef struct ucmc_system ucmc_system;
 #define UCMC_SUN_GAUGE     4 /* example/sun_gauge.hpp:16-86    */
 #define UCMC_XY_ANGLE      5 /* XY spin stored as an angle (no reference model) */
 #define UCMC_PLAQ_ISING    6 /* J1 + 4-spin plaquette Ising (no reference model) */
+#define UCMC_XY_PT         7 /* batch of XY-angle replicas under parallel tempering */
 
 /* proposals (dof/ising.hpp:39-62, dof/vector.hpp:35-122, dof/u1.hpp:35-50,
  * dof/sun.hpp:237-278) */
@@ -133,6 +134,7 @@ typedef struct {
     uint64_t sweep0;    /* counter of the first sweep */
     uint64_t nsweeps;
     uint32_t flags;
+    uint32_t reunitarize_every; /* SU(3): FP64 reunitarisation after every k-th sweep (0 = never) */
 } ucmc_metropolis_params;
 
 typedef struct {
@@ -141,7 +143,21 @@ typedef struct {
     uint64_t flip0;     /* counter of the first flip */
     uint64_t nflips;
     uint32_t flags;
+    uint64_t touch_target; /* > 0: stop once the flipped sites add up to this
+                              (Wolff::sweep, driver.hpp:213-217); sizes of the
+                              flips not run are reported as 0 */
 } ucmc_wolff_params;
+
+/* Parallel tempering over a batch of XY-angle replicas (no counterpart in
+ * the reference, whose drivers run one chain each). */
+typedef struct {
+    double   width;      /* U1Shift width on the angles */
+    uint64_t run_seed;   /* replica r streams: key(run_seed, 2^61 + r); swaps: key(run_seed, 2^60) */
+    uint64_t sweep0;
+    uint64_t nsweeps;
+    uint32_t tile;       /* tile edge; 0 = the autotuned (or default) one */
+    uint32_t use_graph;  /* replay the sweeps from a captured hipGraph */
+} ucmc_pt_params;
 
 /* ---- version / errors ---- */
 int         ucmc_abi_version(void);
@@ -194,6 +210,56 @@ int ucmc_wolff_run(ucmc_system* sys, const ucmc_wolff_params* p, uint64_t* clust
                    ucmc_observables* series);
 /* Observables of the current field. */
 int ucmc_observe(ucmc_system* sys, ucmc_observables* out);
+/* SU(3): project every link back onto SU(3) in FP64 (Gram-Schmidt on rows 0
+ * and 1, row 2 = conj(row0 x row1)).  The reference keeps double links and
+ * never needs it (test/su_check.cpp:96-97); fp32 links drift without it. */
+int ucmc_reunitarize(ucmc_system* sys);
+
+/* ---- parallel tempering ---- */
+/* nreplicas periodic L[r] x L[r] XY replicas (E = -J sum cos(t_i - t_j));
+ * ladder[r] groups replicas that exchange temperatures (non-decreasing in r);
+ * betas[k] is the temperature of slot k, and replica r starts in slot r.
+ * Angles start at 0; the field layout is the replicas' angles back to back. */
+int ucmc_pt_create(size_t nreplicas, const size_t* L, const uint32_t* ladder, const double* betas,
+                   double J, int device, ucmc_system** out);
+/* nsweeps sweeps; after each sweep, neighbouring slots (k, k+1), k of the
+ * sweep's parity, exchange replicas with probability
+ * min(1, exp((beta_k - beta_k+1)(E_a - E_b))).  energy_series (nullable)
+ * receives nsweeps x nreplicas energies after each sweep (before its swaps);
+ * temperature_of (nullable) the final slot of every replica. */
+int ucmc_pt_run(ucmc_system* sys, const ucmc_pt_params* p, double* energy_series,
+                uint32_t* temperature_of);
+/* Cumulative accepted site updates per replica and accepted / attempted swaps per slot. */
+int ucmc_pt_stats(const ucmc_system* sys, uint64_t* accepts, uint64_t* swap_accepts,
+                  uint64_t* swap_tries);
+/* Times candidate tile edges (16, 32, 64) on a copy of the state and keeps
+ * the fastest; the state is restored afterwards. */
+int ucmc_pt_autotune(ucmc_system* sys, double width, uint32_t* tile, double* best_ms_per_sweep);
+
+/* ---- self-learning Monte Carlo (plaquette-Ising systems) ---- */
+/* Plain 4-colour Metropolis at beta for `burn` sweeps, then nconfigs
+ * configurations `every` sweeps apart reduced on the device to the training
+ * rows [1, -C1/N, -C2/N, -C3/N | E/N] (C_k: pair sums of shells (1,0), (1,1),
+ * (2,0)), kept on the device.  correlations (nullable): nconfigs x
+ * (C1, C2, C3, P, M) with P the plaquette sum. */
+int ucmc_slmc_collect(ucmc_system* sys, double beta, uint64_t burn, uint64_t nconfigs,
+                      uint64_t every, uint64_t run_seed, double* correlations);
+/* Least-squares fit theta = (E0/N, J1, J2, J3) of E_eff = E0 - sum_k J_k C_k
+ * through the normal equations: X^T X by bf16 MFMA (hi/lo split), solved in
+ * FP64.  gram (nullable) receives the 16 x 16 fp32 X^T X. */
+int ucmc_slmc_fit_normal(ucmc_system* sys, float* gram, double* theta);
+/* Forward + backward of the loss sum_c (x_c.theta - y_c)^2 over the stored
+ * rows: out[0] = the loss sum, out[1..4] = its gradient. */
+int ucmc_slmc_loss_grad(ucmc_system* sys, const double* theta, double* out);
+/* `iters` Adam steps (lr, 0.9, 0.999) on the mean loss, from theta (in/out). */
+int ucmc_slmc_fit_adam(ucmc_system* sys, int iters, double lr, double* theta, double* mean_loss);
+/* nflips SLMC steps: Wolff cluster on the effective couplings J_eff[3] (the
+ * growth uses keyed edge uniforms, Pcg32(k_f, 1 + 6 owner + slot)), accepted
+ * with min(1, exp(-beta (dE_true - dE_eff))) from the control stream's second
+ * draw (k_f = key(run_seed, 3 2^61 + flip0 + f)).  series (nullable): true
+ * energy, magnetisation, cluster size and 0/1 acceptance per step. */
+int ucmc_slmc_run(ucmc_system* sys, const double* J_eff, double beta, uint64_t run_seed,
+                  uint64_t flip0, uint64_t nflips, ucmc_observables* series, uint8_t* accepted);
 
 /* ---- measurement ---- */
 /* Runs the same launches as ucmc_metropolis_run (no recording) `repeats`

@@ -523,3 +523,99 @@ void orc_fpi_metropolis_keyed(size_t L, int8_t* s, double J1, double K, double b
         if (accepts_out) accepts_out[k] = acc;
     }
 }
+
+/* Pair sums of the three shells (1,0), (1,1), (2,0), the plaquette sum and
+ * the magnetisation of a periodic L x L configuration. */
+void orc_fpi_correlations(size_t L, const int8_t* s, double* out5)
+{
+    long long c1 = 0, c2 = 0, c3 = 0, pp = 0, m = 0;
+    for (size_t y = 0; y < L; ++y)
+        for (size_t x = 0; x < L; ++x) {
+            const int si = s[fpi_id(L, x, y)];
+            const int r = s[fpi_id(L, x + 1, y)], d = s[fpi_id(L, x, y + 1)];
+            c1 += si * (r + d);
+            c2 += si * (s[fpi_id(L, x + 1, y + 1)] + s[fpi_id(L, x + 1, y + L - 1)]);
+            c3 += si * (s[fpi_id(L, x + 2, y)] + s[fpi_id(L, x, y + 2)]);
+            pp += si * r * d * s[fpi_id(L, x + 1, y + 1)];
+            m += si;
+        }
+    out5[0] = (double)c1;
+    out5[1] = (double)c2;
+    out5[2] = (double)c3;
+    out5[3] = (double)pp;
+    out5[4] = (double)m;
+}
+
+/* Keyed SLMC step replay: Wolff growth on the effective couplings Jeff over
+ * the 12 neighbours of shells (1,0), (1,1), (2,0), with the activation rule of
+ * driver.hpp:192-196 on the bond dE 2 Jeff_k s_i s_j and the uniform of
+ * undirected edge e = 6 owner + slot drawn from Pcg32(k_f, 1 + e); then the
+ * Metropolis-Hastings correction min(1, exp(-beta (dE_true - dE_eff))) with
+ * the second draw of Pcg32(k_f, 0) (the first picks the seed site). */
+void orc_slmc_keyed(size_t L, int8_t* s, double J1, double K, const double* Jeff, double beta,
+                    uint64_t run_seed, uint64_t flip0, size_t nflips, uint64_t* size_out,
+                    uint8_t* accepted_out)
+{
+    static const int dx[12] = {1, 0, -1, 0, 1, 1, -1, -1, 2, 0, -2, 0};
+    static const int dy[12] = {0, 1, 0, -1, 1, -1, -1, 1, 0, 2, 0, -2};
+    static const int shell[12] = {0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 2};
+    static const int slot[12] = {0, 1, 0, 1, 2, 3, 2, 3, 4, 5, 4, 5};
+    static const int fwd[12] = {1, 1, 0, 0, 1, 1, 0, 0, 1, 1, 0, 0};
+    const size_t n = L * L;
+    uint8_t* in = (uint8_t*)calloc(n, 1);
+    uint32_t* q = (uint32_t*)malloc(n * sizeof(uint32_t));
+    for (size_t f = 0; f < nflips; ++f) {
+        const uint64_t key = orc_key(run_seed, (UINT64_C(3) << 61) + flip0 + f);
+        orc_pcg32 ctl;
+        orc_pcg32_init(&ctl, key, 0);
+        const size_t seed = (size_t)(orc_pcg32_uni(&ctl) * (double)n);
+        memset(in, 0, n);
+        size_t head = 0, tail = 0;
+        in[seed] = 1;
+        q[tail++] = (uint32_t)seed;
+        while (head < tail) {
+            const uint32_t i = q[head++];
+            const size_t x = i % L, y = i / L;
+            for (int d = 0; d < 12; ++d) {
+                const uint32_t j = (uint32_t)fpi_id(L, x + 2 * L + (size_t)(long)dx[d], y + 2 * L + (size_t)(long)dy[d]);
+                if (in[j]) continue;
+                const uint64_t owner = fwd[d] ? i : j;
+                orc_pcg32 er;
+                orc_pcg32_init(&er, key, 1 + 6 * owner + (uint64_t)slot[d]);
+                const double dE = 2.0 * Jeff[shell[d]] * (double)(s[i] * s[j]);
+                if (orc_pcg32_uni(&er) < 1.0 - exp(-beta * dE)) {
+                    in[j] = 1;
+                    q[tail++] = j;
+                }
+            }
+        }
+        double dt = 0.0, de = 0.0;
+        for (size_t k = 0; k < tail; ++k) {
+            const uint32_t i = q[k];
+            const size_t x = i % L, y = i / L;
+            for (int d = 0; d < 12; ++d) {
+                const uint32_t j = (uint32_t)fpi_id(L, x + 2 * L + (size_t)(long)dx[d], y + 2 * L + (size_t)(long)dy[d]);
+                if (in[j]) continue;
+                const double b = 2.0 * s[i] * s[j];
+                de += Jeff[shell[d]] * b;
+                if (d < 4) dt += J1 * b;
+            }
+        }
+        /* plaquettes with an odd number of sites in the cluster change sign */
+        for (size_t y0 = 0; y0 < L; ++y0)
+            for (size_t x0 = 0; x0 < L; ++x0) {
+                const size_t cs[4] = {fpi_id(L, x0, y0), fpi_id(L, x0 + 1, y0), fpi_id(L, x0, y0 + 1),
+                                      fpi_id(L, x0 + 1, y0 + 1)};
+                const int inside = in[cs[0]] + in[cs[1]] + in[cs[2]] + in[cs[3]];
+                if (inside & 1) dt += 2.0 * K * (double)(s[cs[0]] * s[cs[1]] * s[cs[2]] * s[cs[3]]);
+            }
+        const double log_p = -beta * (dt - de) + 0.0;
+        const int acc = log_p >= 0.0 || log(orc_pcg32_uni(&ctl)) < log_p;
+        if (acc)
+            for (size_t k = 0; k < tail; ++k) s[q[k]] = (int8_t)(-s[q[k]]);
+        if (size_out) size_out[f] = tail;
+        if (accepted_out) accepted_out[f] = (uint8_t)acc;
+    }
+    free(in);
+    free(q);
+}

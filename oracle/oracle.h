@@ -218,6 +218,14 @@ double orc_fpi_energy_diff_flip(size_t L, const int8_t* s, double J1, double K, 
 void   orc_fpi_metropolis_keyed(size_t L, int8_t* s, double J1, double K, double beta,
                                 uint64_t run_seed, uint64_t sweep0, size_t nsweeps,
                                 uint64_t* accepts_out);
+/* out5 = (C1, C2, C3, P, M): pair sums of shells (1,0), (1,1), (2,0), the
+ * plaquette sum and the magnetisation. */
+void   orc_fpi_correlations(size_t L, const int8_t* s, double* out5);
+/* Keyed SLMC step replay (Wolff on Jeff[3] + Metropolis-Hastings correction
+ * with the true model). */
+void   orc_slmc_keyed(size_t L, int8_t* s, double J1, double K, const double* Jeff, double beta,
+                      uint64_t run_seed, uint64_t flip0, size_t nflips, uint64_t* size_out,
+                      uint8_t* accepted_out);
 
 #ifdef __cplusplus
 }

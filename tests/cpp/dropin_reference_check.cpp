@@ -1,0 +1,161 @@
+// dropin_reference_check.cpp -- the reference's own test scenarios
+// (test/model_check.cpp:171-222, test/graph_check.cpp:153-192,
+// test/u1_check.cpp:162-229, test/su_check.cpp:68-133) with the reference's
+// model, proposal and RNG types, driven by ucmc::gpu::Metropolis / Wolff
+// instead of the reference's drivers.  Built by oracle/Makefile (it needs the
+// reference headers) into oracle/_ref/dropin_reference_check; run on a GPU by
+// tests/test_gpu_dropin.py.  Prints the measured values and "ok".
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <numbers>
+
+#include <dof/ising.hpp>
+#include <dof/sun.hpp>
+#include <dof/u1.hpp>
+#include <dof/vector.hpp>
+#include <example/ising.hpp>
+#include <example/sun_gauge.hpp>
+#include <example/u1_gauge.hpp>
+#include <example/vector_spin.hpp>
+#include <lattice/square.hpp>
+#include <rng.hpp>
+
+#include <ucmc/gpu.hpp>
+
+#define EXPECT(c)                                                                   \
+    do {                                                                            \
+        if (!(c)) {                                                                 \
+            std::fprintf(stderr, "FAILED %s at %s:%d\n", #c, __FILE__, __LINE__);   \
+            std::exit(1);                                                           \
+        }                                                                           \
+    } while (0)
+
+template <class M>
+double total_energy(const M& model)
+{
+    double sum = 0.0;
+    for (std::size_t i = 0; i < model.sites(); ++i) sum += model.energy_at(i);
+    return 0.5 * sum;
+}
+
+int main()
+{
+    constexpr std::size_t L = 4, N = L * L;
+    const auto graph = square_lattice(L, L);
+    constexpr double ising_exact = -0.8440540736;  // model_check.cpp:172
+
+    {   // model_check.cpp:173-184
+        Ising model(graph);
+        ucmc::gpu::Metropolis metro(model, Flip{}, Pcg32(11, 1), 0.3);
+        metro.run(2000);
+        double sum = 0.0;
+        metro.sample(4000, [&](const Ising& m) { sum += total_energy(m); });
+        const double avg = sum / (4000.0 * double(N));
+        std::printf("Ising Metropolis: %.5f\n", avg);
+        EXPECT(std::abs(avg - ising_exact) < 0.03);
+    }
+    {   // model_check.cpp:185-196
+        Ising model(graph);
+        ucmc::gpu::Wolff wolf(model, FixedAxis{}, Pcg32(12, 1), 0.3);
+        for (int k = 0; k < 2000; ++k) wolf.flip();
+        double sum = 0.0;
+        wolf.sample(20000, [&](const Ising& m) { sum += total_energy(m); });
+        const double avg = sum / (20000.0 * double(N));
+        std::printf("Ising Wolff:      %.5f (cluster %.2f)\n", avg, wolf.mean_cluster_size());
+        EXPECT(std::abs(avg - ising_exact) < 0.05);
+    }
+    double heis_metro = 0.0;
+    {   // model_check.cpp:199-210
+        VectorSpin<3> model(graph);
+        ucmc::gpu::Metropolis metro(model, UniformSphereN<3>{}, Pcg32(13, 1), 0.3);
+        metro.run(2000);
+        double sum = 0.0;
+        metro.sample(4000, [&](const VectorSpin<3>& m) { sum += total_energy(m); });
+        heis_metro = sum / (4000.0 * double(N));
+        std::printf("Heisenberg Metropolis: %.5f\n", heis_metro);
+    }
+    {   // model_check.cpp:211-221
+        VectorSpin<3> model(graph);
+        ucmc::gpu::Wolff wolf(model, UniformSphereN<3>{}, Pcg32(14, 1), 0.3);
+        for (int k = 0; k < 2000; ++k) wolf.flip();
+        double sum = 0.0;
+        wolf.sample(20000, [&](const VectorSpin<3>& m) { sum += total_energy(m); });
+        const double avg = sum / (20000.0 * double(N));
+        std::printf("Heisenberg Wolff:      %.5f (cluster %.2f)\n", avg, wolf.mean_cluster_size());
+        EXPECT(std::abs(avg - heis_metro) < 0.05);
+    }
+    {   // LocalRotation proposal and the device fast path
+        VectorSpin<3> model(graph);
+        ucmc::gpu::Metropolis metro(model, LocalRotation<3>{0.8}, Pcg32(15, 1), 0.3);
+        metro.run(2000);
+        double sum = 0.0;
+        metro.sample_device(8000, [&](const ucmc_observables& o) { sum += o.energy; });
+        const double avg = sum / (8000.0 * double(N));
+        std::printf("Heisenberg LocalRotation (device observables): %.5f\n", avg);
+        EXPECT(std::abs(avg - heis_metro) < 0.05);
+    }
+    {   // u1_check.cpp: hot and cold at beta 0.5 (model and driver beta, as there)
+        auto chain = [](double beta, bool hot, std::uint64_t seed) {
+            U1Gauge g(4, beta);
+            Pcg32 init(seed, 1);
+            if (hot)
+                for (std::size_t i = 0; i < g.sites(); ++i) {
+                    U1Link p;
+                    p.theta = 2.0 * std::numbers::pi_v<double> * init.uni();
+                    g.update(i, &p);
+                }
+            ucmc::gpu::Metropolis metro(g, U1Shift{0.7}, Pcg32(seed, 2), beta);
+            metro.run(100);
+            double sum = 0.0;
+            std::size_t n = 0;
+            metro.sample(400, [&](const U1Gauge& m) { sum += m.mean_plaquette(); ++n; });
+            return sum / double(n);
+        };
+        const double hot05 = chain(0.5, true, 50), cold05 = chain(0.5, false, 60);
+        const double cold10 = chain(10.0, false, 20);
+        std::printf("U1 beta=0.5 hot=%.5f cold=%.5f, beta=10 cold=%.5f\n", hot05, cold05, cold10);
+        EXPECT(std::abs(hot05 - cold05) < 0.02);
+        EXPECT(cold10 > 0.9);
+    }
+    {   // su_check.cpp:109-133 (SU(2) L=4 and SU(3) L=3 -> device needs even L: SU(3) at L=4)
+        auto chain3 = [](double beta, bool hot, std::uint64_t seed) {
+            SUGauge<3> g(4, beta);
+            Pcg32 init(seed, 1);
+            if (hot) {
+                SUShift<3> big{3.0};
+                for (std::size_t i = 0; i < g.sites(); ++i) {
+                    SULink<3> p;
+                    for (int k = 0; k < 5; ++k) {
+                        big.sample(g.data() + i, init, &p);
+                        g.update(i, &p);
+                    }
+                }
+            }
+            ucmc::gpu::Metropolis m(g, SUShift<3>{0.3}, Pcg32(seed, 2), beta);
+            m.run(100);
+            double sum = 0.0;
+            std::size_t n = 0;
+            m.sample(200, [&](const SUGauge<3>& model) { sum += model.mean_plaquette(); ++n; });
+            for (std::size_t i = 0; i < g.sites(); i += 37) EXPECT(g.data()[i].unitarity_error() < 1e-4);
+            return sum / double(n);
+        };
+        const double h0 = chain3(0.0, true, 60), c0 = chain3(0.0, false, 70);
+        const double h4 = chain3(4.0, true, 80), c4 = chain3(4.0, false, 90);
+        std::printf("SU3 beta=0 hot=%.5f cold=%.5f, beta=4 hot=%.5f cold=%.5f\n", h0, c0, h4, c4);
+        EXPECT(std::abs(h0) < 0.05 && std::abs(c0) < 0.05);
+        EXPECT(std::abs(h4 - c4) < 0.05);
+    }
+    {   // exception types at the boundary mirror the reference's
+        bool threw = false;
+        try {
+            ucmc_system* h = nullptr;
+            ucmc::gpu::check(ucmc_u1_gauge_create(0, 1.0, 0, &h));
+        } catch (const std::invalid_argument&) {
+            threw = true;
+        }
+        EXPECT(threw);
+    }
+    std::puts("ok");
+    return 0;
+}

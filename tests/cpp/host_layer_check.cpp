@@ -1,0 +1,194 @@
+// host_layer_check.cpp -- the C++23 host layer (include/ucmc/gpu.hpp) with
+// user-written models shaped like the reference's ModelLike / WolffModel
+// concepts (model.hpp:70-86, driver.hpp:117-128), on the GPU: a periodic
+// Ising ring against its exact transfer-matrix energy, and an XY rotor model
+// where Metropolis (LocalRotation-like proposal) and Wolff must agree.
+// Built by universal_classical_montecarlo_amd/build.py; run by
+// tests/test_gpu_dropin.py.  Prints the measured values and "ok".
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include <ucmc/gpu.hpp>
+
+#define EXPECT(c)                                                                 \
+    do {                                                                          \
+        if (!(c)) {                                                               \
+            std::fprintf(stderr, "FAILED %s at %s:%d\n", #c, __FILE__, __LINE__); \
+            std::exit(1);                                                         \
+        }                                                                         \
+    } while (0)
+
+struct Bond { std::uint32_t j; };
+
+// E = -J sum_i s_i s_{i+1} on a ring of n spins.
+class RingIsing
+{
+public:
+    using value_type = std::int8_t;
+    using bond_type = Bond;
+    static constexpr std::size_t size = 1;
+    RingIsing(std::size_t n, int J) : J_(J), s_(n, 1), rows_(2 * n)
+    {
+        for (std::size_t i = 0; i < n; ++i) {
+            rows_[2 * i].j = static_cast<std::uint32_t>((i + 1) % n);
+            rows_[2 * i + 1].j = static_cast<std::uint32_t>((i + n - 1) % n);
+        }
+    }
+    std::size_t sites() const noexcept { return s_.size(); }
+    const std::int8_t* data() const noexcept { return s_.data(); }
+    std::size_t degree(std::size_t) const noexcept { return 2; }
+    const Bond* bonds(std::size_t i) const noexcept { return rows_.data() + 2 * i; }
+    int energy_at(std::size_t i) const noexcept
+    {
+        return -J_ * s_[i] * (s_[rows_[2 * i].j] + s_[rows_[2 * i + 1].j]);
+    }
+    int energy_diff_at(std::size_t i, const std::int8_t* p) const noexcept
+    {
+        const int sum = s_[rows_[2 * i].j] + s_[rows_[2 * i + 1].j];
+        return -J_ * (*p - s_[i]) * sum;
+    }
+    int energy_diff_bond(std::size_t i, const Bond& b, const std::int8_t* p) const noexcept
+    {
+        return -J_ * (*p - s_[b.j]) * s_[i];
+    }
+    void update(std::size_t i, const std::int8_t* p) noexcept { s_[i] = *p; }
+
+private:
+    int J_;
+    std::vector<std::int8_t> s_;
+    std::vector<Bond> rows_;
+};
+
+// XY rotors as 2-vectors on a periodic L x L square lattice.
+class Rotors
+{
+public:
+    using value_type = double;
+    using bond_type = Bond;
+    static constexpr std::size_t size = 2;
+    explicit Rotors(std::size_t L) : L_(L), s_(2 * L * L, 0.0), rows_(4 * L * L)
+    {
+        for (std::size_t i = 0; i < L * L; ++i) {
+            s_[2 * i + 1] = 1.0;
+            const std::size_t x = i % L, y = i / L;
+            rows_[4 * i + 0].j = static_cast<std::uint32_t>(y * L + (x + 1) % L);
+            rows_[4 * i + 1].j = static_cast<std::uint32_t>(((y + 1) % L) * L + x);
+            rows_[4 * i + 2].j = static_cast<std::uint32_t>(y * L + (x + L - 1) % L);
+            rows_[4 * i + 3].j = static_cast<std::uint32_t>(((y + L - 1) % L) * L + x);
+        }
+    }
+    std::size_t sites() const noexcept { return L_ * L_; }
+    const double* data() const noexcept { return s_.data(); }
+    std::size_t degree(std::size_t) const noexcept { return 4; }
+    const Bond* bonds(std::size_t i) const noexcept { return rows_.data() + 4 * i; }
+    double energy_diff_at(std::size_t i, const double* p) const noexcept
+    {
+        double d = 0.0;
+        for (int a = 0; a < 4; ++a) {
+            const double* t = s_.data() + 2 * rows_[4 * i + a].j;
+            d += -((p[0] - s_[2 * i]) * t[0] + (p[1] - s_[2 * i + 1]) * t[1]);
+        }
+        return d;
+    }
+    double energy_diff_bond(std::size_t i, const Bond& b, const double* p) const noexcept
+    {
+        const double* f = s_.data() + 2 * i;
+        const double* o = s_.data() + 2 * b.j;
+        return -((p[0] - o[0]) * f[0] + (p[1] - o[1]) * f[1]);
+    }
+    void update(std::size_t i, const double* p) noexcept
+    {
+        s_[2 * i] = p[0];
+        s_[2 * i + 1] = p[1];
+    }
+    double total_energy() const
+    {
+        double e = 0.0;
+        for (std::size_t i = 0; i < sites(); ++i)
+            for (int a = 0; a < 2; ++a) {
+                const double* t = s_.data() + 2 * rows_[4 * i + a].j;
+                e += -(s_[2 * i] * t[0] + s_[2 * i + 1] * t[1]);
+            }
+        return e;
+    }
+
+private:
+    std::size_t L_;
+    std::vector<double> s_;
+    std::vector<Bond> rows_;
+};
+
+struct Rng  // anything with uni() will do for the drivers
+{
+    std::uint64_t x;
+    double uni() { x = x * 6364136223846793005ull + 1442695040888963407ull; return (x >> 32) * 0x1.0p-32; }
+};
+struct Flip {};
+struct FixedAxis {};
+struct Rotate { double delta; };
+struct SphereAxis {};
+
+int main()
+{
+    {   // exact: Z = (2 cosh bJ)^N + (2 sinh bJ)^N
+        const std::size_t n = 64;
+        const double beta = 0.7;
+        const double c = std::cosh(beta), s = std::sinh(beta);
+        const double lp = std::pow(2 * c, n), lm = std::pow(2 * s, n);
+        const double exact = -(lp * std::tanh(beta) + lm / std::tanh(beta)) / (lp + lm);
+        RingIsing ring(n, 1);
+        ucmc::gpu::Metropolis metro(ring, Flip{}, Rng{7}, beta);
+        EXPECT(metro.info().schedule == UCMC_SCHED_CSR);
+        metro.run(1000);
+        double sum = 0.0;
+        metro.sample(20000, [&](const RingIsing& m) {
+            double e = 0.0;
+            for (std::size_t i = 0; i < m.sites(); ++i) e += m.energy_at(i);
+            sum += 0.5 * e;
+        });
+        const double avg = sum / (20000.0 * n);
+        std::printf("ring Metropolis %.5f exact %.5f\n", avg, exact);
+        EXPECT(std::abs(avg - exact) < 0.01);
+        double wsum = 0.0;
+        ucmc::gpu::Wolff wolf(ring, FixedAxis{}, Rng{8}, beta);
+        wolf.sample_device(20000, [&](const ucmc_observables& o) { wsum += o.energy; });
+        const double wavg = wsum / (20000.0 * n);
+        std::printf("ring Wolff %.5f (cluster %.2f)\n", wavg, wolf.mean_cluster_size());
+        EXPECT(std::abs(wavg - exact) < 0.01);
+    }
+    {
+        Rotors rot(8);
+        ucmc::gpu::Metropolis metro(rot, Rotate{1.0}, Rng{3}, 0.8);
+        EXPECT(metro.info().schedule == UCMC_SCHED_CHECKERBOARD);
+        metro.run(2000);
+        double ms = 0.0;
+        metro.sample_device(10000, [&](const ucmc_observables& o) { ms += o.energy; });
+        const double em = ms / (10000.0 * 64);
+        metro.sync();
+        Rotors rot2(8);
+        ucmc::gpu::Wolff wolf(rot2, SphereAxis{}, Rng{4}, 0.8);
+        wolf.run(200);
+        double ws = 0.0;
+        wolf.sample(4000, [&](const Rotors& m) { ws += m.total_energy(); });
+        const double ew = ws / (4000.0 * 64);
+        std::printf("rotors Metropolis %.5f Wolff %.5f\n", em, ew);
+        EXPECT(std::abs(em - ew) < 0.03);
+    }
+    {   // errors keep the reference's exception types
+        std::vector<std::uint32_t> off{0, 1, 2}, b{0, 0};
+        const ucmc_pair_graph g{2, off.data(), b.data()};
+        ucmc_system* h = nullptr;
+        bool threw = false;
+        try {
+            ucmc::gpu::check(ucmc_ising_create(&g, 1, 0, &h));
+        } catch (const std::invalid_argument&) {
+            threw = true;
+        }
+        EXPECT(threw);
+    }
+    std::puts("ok");
+    return 0;
+}

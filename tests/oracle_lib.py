@@ -109,6 +109,8 @@ def oracle():
     _sig(L, "orc_fpi_energy", f64, sz, i8p, f64, f64)
     _sig(L, "orc_fpi_energy_diff_flip", f64, sz, i8p, f64, f64, sz)
     _sig(L, "orc_fpi_metropolis_keyed", None, sz, i8p, f64, f64, f64, u64, u64, sz, ou64)
+    _sig(L, "orc_fpi_correlations", None, sz, i8p, f64p)
+    _sig(L, "orc_slmc_keyed", None, sz, i8p, f64, f64, f64p, f64, u64, u64, sz, ou64, ou8)
     # helpers that take an orc_pcg32* are wrapped through a small struct
     L.orc_pcg32_init.restype = None
     L.orc_pcg32_init.argtypes = [C.c_void_p, u64, u64]

@@ -1,0 +1,49 @@
+"""CPU checks of the drop-in boundary: the HIP library loads without a GPU,
+exports every function include/ucmc.h declares, and its host-side pieces
+(key schedule, argument checks that fire before any device work) behave as
+documented.  No compute call runs here."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle_lib import oracle
+from universal_classical_montecarlo_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = []
+    for h in os.listdir(os.path.join(ROOT, "include")):
+        if h.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", h)).read()
+            names += re.findall(r"^\s*(?:int|const char\*|uint64_t)\s+(ucmc_\w+)\s*\(", txt, re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = abi.lib()
+    names = declared_functions()
+    assert len(names) >= 30
+    for n in names:
+        assert hasattr(lib, n), f"{n} declared in include/ucmc.h but not exported"
+    assert sorted(abi.EXPORTED) == names
+
+
+def test_abi_version_and_key():
+    lib = abi.lib()
+    assert lib.ucmc_abi_version() == 1
+    for seed, c in [(0, 0), (2026, 7), (2**63 + 11, 2**62 + 5)]:
+        assert lib.ucmc_key(seed, c) == oracle().orc_key(seed, c)
+
+
+def test_no_device_is_reported_not_hidden():
+    n = abi.device_count()
+    if n == 0:
+        off = np.array([0, 1, 2], np.uint32)
+        bonds = np.array([1, 0], np.uint32)
+        with pytest.raises(abi.UcmcError) as e:
+            abi.System.ising(off, bonds, 1)
+        assert e.value.code == abi.UCMC_E_NO_DEVICE

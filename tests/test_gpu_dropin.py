@@ -1,0 +1,32 @@
+"""The C++23 drop-in layer on the GPU: tests/cpp/host_layer_check.cpp (user
+models shaped like the reference's concepts) and, when it was built next to
+the reference sources, tests/cpp/dropin_reference_check.cpp (the reference's
+own Ising / VectorSpin / U1Gauge / SUGauge types and test scenarios, with
+ucmc::gpu::Metropolis / Wolff in place of the reference's drivers)."""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HOST = os.path.join(ROOT, "universal_classical_montecarlo_amd", "_build", "host_layer_check")
+DROPIN = os.path.join(ROOT, "oracle", "_ref", "dropin_reference_check")
+
+pytestmark = pytest.mark.gpu
+
+
+def run(path):
+    r = subprocess.run([path], capture_output=True, text=True, timeout=240)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.strip().endswith("ok")
+
+
+def test_host_layer_user_models(gpu):
+    assert os.path.exists(HOST), "build() did not produce host_layer_check"
+    run(HOST)
+
+
+@pytest.mark.skipif(not os.path.exists(DROPIN), reason="built only where the reference sources are")
+def test_dropin_reference_scenarios(gpu):
+    run(DROPIN)

@@ -1,0 +1,121 @@
+"""GPU parity of the 4D gauge models: U1Gauge + U1Shift and SUGauge<N> +
+SUShift<N> (example/u1_gauge.hpp, example/sun_gauge.hpp, driven as in
+driver.hpp:73-87) against the oracle's keyed replay over the 8 (direction x
+parity) colours, with `hits` hits per link.
+
+fp32 links and staples on the device, double in the oracle: dE within 1e-5
+relative (absolute floor 1e-5), decisions identical, links within 1e-5."""
+import numpy as np
+import pytest
+
+from oracle_lib import Pcg32, hypercubic, oracle
+from universal_classical_montecarlo_amd import abi
+
+pytestmark = pytest.mark.gpu
+RTOL = 1e-5
+
+
+def f32(a):
+    return np.asarray(a, np.float64).astype(np.float32).astype(np.float64)
+
+
+def hot_u1(L, seed):
+    rng = Pcg32(seed, 1)
+    n = 4 * L ** 4
+    return f32([2.0 * np.pi * rng.uni() - np.pi for _ in range(n)])
+
+
+def hot_sun(N, L, seed, width=3.0, steps=2):
+    """su_check.cpp:74-82 hot start recipe (SUShift{3.0} steps per link)."""
+    o = oracle()
+    rng = Pcg32(seed, 1)
+    n = 4 * L ** 4
+    out = np.zeros(n * 2 * N * N)
+    for i in range(n):
+        u = np.zeros(2 * N * N)
+        u[[2 * (k * N + k) for k in range(N)]] = 1.0
+        for _ in range(steps):
+            v = np.zeros(2 * N * N)
+            o.orc_sun_shift(N, width, u, rng.ptr, v)
+            u = v
+        out[i * 2 * N * N:(i + 1) * 2 * N * N] = u
+    return f32(out)
+
+
+def assert_close_dE(got, want):
+    err = np.abs(got.astype(np.float64) - want)
+    tol = RTOL * np.maximum(np.abs(want), 1.0)
+    assert np.all(err <= tol), f"max dE err {err.max()}"
+
+
+@pytest.mark.parametrize("L,beta,width,hits", [(2, 1.3, 1.0, 1), (4, 0.9, 0.7, 1)])
+def test_u1_trace(gpu, L, beta, width, hits):
+    print(f"u1 L={L}", flush=True)
+    sysm = abi.System.u1_gauge(L, beta)
+    th = hot_u1(L, 5)
+    sysm.upload(th)
+    dE, acc = sysm.trace(1.0, 17, sweep0=3, proposal=abi.PROP_U1_SHIFT, width=width, hits=hits)
+    want = th.copy()
+    n = 4 * L ** 4
+    tr = np.zeros(n)
+    oracle().orc_u1_metropolis_keyed(L, want, beta, 1.0, width, 17, 3, 1, None, tr)
+    assert_close_dE(dE, tr)
+    got = sysm.download()
+    d = np.abs(np.angle(np.exp(1j * (got - want))))
+    assert d.max() < 1e-5
+    sysm.close()
+
+
+@pytest.mark.parametrize("N,L,beta,width,hits", [(3, 2, 5.7, 0.2, 4), (3, 4, 5.7, 0.2, 4),
+                                                 (2, 2, 2.0, 0.3, 1), (3, 4, 4.0, 0.3, 1)])
+def test_sun_trace(gpu, N, L, beta, width, hits):
+    print(f"su{N} L={L} hits={hits}", flush=True)
+    sysm = abi.System.sun_gauge(N, L, beta)
+    links = hot_sun(N, L, 11)
+    sysm.upload(links)
+    dE, acc = sysm.trace(1.0, 23, sweep0=1, proposal=abi.PROP_SU_SHIFT, width=width, hits=hits)
+    want = links.copy()
+    n = 4 * L ** 4
+    tr = np.zeros(n * hits)
+    oracle().orc_sun_metropolis_keyed(N, L, want, beta, 1.0, width, hits, 23, 1, 1, None, tr)
+    assert_close_dE(dE, tr)
+    got = sysm.download()
+    assert np.max(np.abs(got - want)) < 2e-5
+    o = sysm.observe()
+    cells, *_ = hypercubic(L)
+    assert abs(o.plaquette - oracle().orc_sun_mean_plaquette(N, 6 * L ** 4, cells, got)) < 1e-6
+    sysm.close()
+
+
+def test_sun_cold_start_and_errors(gpu):
+    sysm = abi.System.sun_gauge(3, 4, 5.7)
+    o = sysm.observe()
+    assert o.plaquette == pytest.approx(1.0, abs=1e-7)  # identity links
+    assert o.energy == pytest.approx(0.0, abs=1e-4)
+    with pytest.raises(abi.UcmcError):
+        sysm.metropolis(1.0, 1, 0, proposal=abi.PROP_U1_SHIFT, width=0.2)
+    sysm.close()
+    with pytest.raises(abi.UcmcError) as e:
+        abi.System.sun_gauge(3, 0, 5.7)
+    assert e.value.code == abi.UCMC_E_INVALID_ARGUMENT
+
+
+def test_su3_reunitarize_and_chain(gpu):
+    """After many fp32 sweeps the FP64 reunitarisation restores unitarity; the
+    FP64 plaquette of a 16^4 chain at beta = 5.7 is close to the FP64 oracle's
+    plaquette of the same links."""
+    L = 8
+    sysm = abi.System.sun_gauge(3, L, 5.7)
+    ser = sysm.metropolis(1.0, 20, 3, proposal=abi.PROP_SU_SHIFT, width=0.2, hits=4,
+                          flags=abi.RECORD_ENERGY | abi.RECORD_ACCEPTS, record=True,
+                          reunitarize_every=10)
+    links = sysm.download()
+    o = oracle()
+    err = max(o.orc_sun_unitarity_error(3, links[18 * i:18 * i + 18].copy()) for i in range(0, 4 * L ** 4, 97))
+    assert err < 5e-6
+    cells, *_ = hypercubic(L)
+    p64 = o.orc_sun_mean_plaquette(3, 6 * L ** 4, cells, links)
+    assert abs(ser[-1].plaquette - p64) < 1e-6
+    acc = ser[-1].accepted / (4 * 4 * L ** 4)
+    assert 0.3 < acc < 0.95
+    sysm.close()

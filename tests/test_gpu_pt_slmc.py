@@ -1,0 +1,189 @@
+"""GPU parity of the XY-angle model, the batched parallel-tempering launch and
+the self-learning Monte Carlo pieces (plaquette Ising Metropolis, training
+reductions, MFMA normal equations, forward / backward + Adam, the MH-corrected
+Wolff step) against the oracle and against direct host computations."""
+import numpy as np
+import pytest
+
+from oracle_lib import checkerboard, csr_square, oracle, random_spins
+from universal_classical_montecarlo_amd import abi
+
+pytestmark = pytest.mark.gpu
+RTOL = 1e-5
+
+
+def f32(a):
+    return np.asarray(a, np.float64).astype(np.float32).astype(np.float64)
+
+
+# ------------------------------------------------------------ XY angles --
+@pytest.mark.parametrize("L,beta,width", [(16, 1.1, 1.0), (10, 0.8, 2.0)])
+def test_xy_angle_trace(gpu, L, beta, width):
+    off, bonds = csr_square(L, L)
+    sysm = abi.System.xy_angle(off, bonds, 1.0)
+    colour = sysm.colouring()
+    rng = np.random.default_rng(L)
+    t = f32(rng.uniform(-np.pi, np.pi, L * L))
+    sysm.upload(t)
+    dE, acc = sysm.trace(beta, 5, sweep0=1, proposal=abi.PROP_U1_SHIFT, width=width)
+    want = t.copy()
+    tr = np.zeros(L * L)
+    wacc = np.zeros(1, np.uint64)
+    oracle().orc_xy_metropolis_keyed(L * L, off, bonds, want, 1.0, beta, width, colour,
+                                     int(colour.max()) + 1, 5, 1, 1, wacc, tr)
+    err = np.abs(dE.astype(np.float64) - tr)
+    assert np.all(err <= RTOL * np.maximum(np.abs(tr), 1.0)), err.max()
+    assert int(acc.sum()) == int(wacc[0])
+    got = sysm.download()
+    assert np.max(np.abs(np.angle(np.exp(1j * (got - want))))) < 1e-5
+    sysm.close()
+
+
+# --------------------------------------------------- parallel tempering --
+def pt_replay(Ls, ladders, betas, fields, width, run_seed, sweep0, nsweeps):
+    """Oracle replay of the batched PT launch: each replica a keyed XY sweep at
+    its current temperature, then the exchange step with the swap stream."""
+    o = oracle()
+    R = len(Ls)
+    temp_of = list(range(R))
+    rep_at = list(range(R))
+    starts = [r for r in range(R) if r == 0 or ladders[r] != ladders[r - 1]] + [R]
+    key = lambda c: o.orc_key(run_seed, c)
+    E = np.zeros((nsweeps, R))
+    for k in range(nsweeps):
+        sweep = sweep0 + k
+        for r in range(R):
+            L = Ls[r]
+            off, bonds = csr_square(L, L)
+            o.orc_xy_metropolis_keyed(L * L, off, bonds, fields[r], 1.0, betas[temp_of[r]], width,
+                                      checkerboard((L, L)), 2, key((1 << 61) + r), sweep, 1, None,
+                                      None)
+            t = fields[r].reshape(L, L)
+            E[k, r] = -(np.cos(t - np.roll(t, -1, 1)).sum() + np.cos(t - np.roll(t, -1, 0)).sum())
+        pkey = o.orc_key(key(1 << 60), sweep)
+        for ld in range(len(starts) - 1):
+            s0, s1 = starts[ld], starts[ld + 1]
+            kk = s0 + (sweep & 1)
+            while kk + 1 < s1:
+                ra, rb = rep_at[kk], rep_at[kk + 1]
+                log_p = (betas[kk] - betas[kk + 1]) * (E[k, ra] - E[k, rb])
+                u = np.zeros(1, np.uint32)
+                o.orc_pcg32_block(pkey, kk, 1, u)
+                if log_p >= 0.0 or np.log(u[0] * 2.0 ** -32) < log_p:
+                    rep_at[kk], rep_at[kk + 1] = rb, ra
+                    temp_of[ra], temp_of[rb] = kk + 1, kk
+                kk += 2
+    return E, temp_of
+
+
+@pytest.mark.parametrize("tile,use_graph", [(16, False), (8, True), (32, False)])
+def test_pt_batch_matches_replay(gpu, tile, use_graph):
+    Ls = [8, 8, 8, 16, 16, 32]
+    ladders = [0, 0, 0, 1, 1, 2]
+    betas = [0.9, 1.1, 1.3, 0.8, 1.4, 1.0]
+    sysm = abi.System.pt(Ls, ladders, betas)
+    rng = np.random.default_rng(3)
+    fields = [f32(rng.uniform(-np.pi, np.pi, L * L)) for L in Ls]
+    sysm.upload(np.concatenate(fields))
+    nsweeps = 6
+    E, temp = sysm.pt_run(1.0, nsweeps, 99, sweep0=4, tile=tile, use_graph=use_graph, record=True)
+    replayed = [f.copy() for f in fields]
+    wE, wtemp = pt_replay(Ls, ladders, betas, replayed, 1.0, 99, 4, nsweeps)
+    assert np.allclose(E, wE, rtol=1e-5, atol=1e-4)
+    assert temp.tolist() == wtemp
+    got = sysm.download()
+    want = np.concatenate(replayed)
+    assert np.max(np.abs(np.angle(np.exp(1j * (got - want))))) < 1e-4
+    acc, sacc, stry = sysm.pt_stats()
+    assert acc.sum() > 0 and stry.sum() > 0
+    sysm.close()
+
+
+def test_pt_baseline_config_runs(gpu):
+    """256 replicas, L in {32..512}, 64 temperatures per
+    ladder (the 32-replica ladders use every other beta); autotuned tiles."""
+    Ls, ladders, betas = [], [], []
+    grid = 0.8 + 0.6 * np.arange(64) / 63
+    for ld, (L, count) in enumerate([(32, 64), (64, 64), (128, 64), (256, 32), (512, 32)]):
+        bs = grid if count == 64 else grid[::2]
+        Ls += [L] * count
+        ladders += [ld] * count
+        betas += list(bs)
+    sysm = abi.System.pt(Ls, ladders, betas)
+    assert sysm.sites == 11862016
+    tile, ms = sysm.pt_autotune(1.0)
+    assert tile in (16, 32, 64) and ms > 0
+    E, temp = sysm.pt_run(1.0, 20, 1, use_graph=True, record=True)
+    assert sorted(temp.tolist()) == list(range(256))
+    # every replica's energy is a physical XY energy per bond in [-1, 1]
+    n_bonds = 2 * np.array(Ls, float) ** 2
+    assert np.all(np.abs(E[-1] / n_bonds) <= 1.0)
+    acc, sacc, stry = sysm.pt_stats()
+    assert 0 < sacc.sum() <= stry.sum()
+    sysm.close()
+
+
+# ------------------------------------------------------------------ SLMC --
+def test_plaquette_ising_bit_exact(gpu):
+    L = 16
+    sysm = abi.System.plaquette_ising(L, 1.0, 0.2)
+    assert sysm.info.schedule == abi.SCHED_FOUR
+    s = random_spins(L * L, 4, 1)
+    sysm.upload(s)
+    ser = sysm.metropolis(0.42, 5, 11, sweep0=2,
+                          flags=abi.RECORD_ENERGY | abi.RECORD_ACCEPTS | abi.RECORD_ORDER, record=True)
+    want = s.copy()
+    acc = np.zeros(5, np.uint64)
+    oracle().orc_fpi_metropolis_keyed(L, want, 1.0, 0.2, 0.42, 11, 2, 5, acc)
+    got = sysm.download()
+    assert np.array_equal(got, want)
+    assert [o.accepted for o in ser] == acc.tolist()
+    assert ser[-1].energy == pytest.approx(oracle().orc_fpi_energy(L, want, 1.0, 0.2), abs=1e-9)
+    assert ser[-1].order[0] == want.astype(np.int64).sum()
+    sysm.close()
+
+
+def test_slmc_training_kernels(gpu):
+    """Device reductions vs the oracle, the MFMA normal equations vs a float64
+    least-squares solve, and the forward / backward gradient vs numpy."""
+    L = 32
+    sysm = abi.System.plaquette_ising(L, 1.0, 0.2)
+    corr = sysm.slmc_collect(0.42, 200, 400, 2, 5)
+    assert corr.shape == (400, 5)
+    N = L * L
+    X = np.stack([np.ones(400), -corr[:, 0] / N, -corr[:, 1] / N, -corr[:, 2] / N], axis=1)
+    y = (-1.0 * corr[:, 0] - 0.2 * corr[:, 3]) / N
+    X32, y32 = f32(X), f32(y)
+    theta_ref, *_ = np.linalg.lstsq(X32, y32, rcond=None)
+    theta, G = sysm.slmc_fit_normal()
+    assert np.allclose(G[:4, :4], X32.T @ X32, rtol=2e-4, atol=1e-3)
+    assert np.allclose(G[:4, 15], X32.T @ y32, rtol=2e-4, atol=1e-3)
+    assert np.allclose(theta, theta_ref, atol=2e-3)
+    # gradient of sum (X theta - y)^2 at an arbitrary point
+    th = np.array([0.1, 0.9, 0.05, -0.02])
+    loss, grad = sysm.slmc_loss_grad(th)
+    r = X32 @ th - y32
+    assert loss == pytest.approx((r ** 2).sum(), rel=1e-9)
+    assert np.allclose(grad, 2.0 * X32.T @ r, rtol=1e-9, atol=1e-9)
+    # Adam reaches the least-squares solution
+    th_adam, mloss = sysm.slmc_fit_adam(3000, 0.01, np.zeros(4))
+    assert mloss <= ((X32 @ theta_ref - y32) ** 2).mean() * 1.05 + 1e-9
+    sysm.close()
+
+
+def test_slmc_step_matches_replay(gpu):
+    L = 16
+    sysm = abi.System.plaquette_ising(L, 1.0, 0.2)
+    s = random_spins(L * L, 6, 2)
+    sysm.upload(s)
+    J = np.array([1.05, 0.07, -0.01])
+    ser, acc = sysm.slmc_run(J, 0.42, 80, 31, flip0=3)
+    want = s.copy()
+    sz = np.zeros(80, np.uint64)
+    wacc = np.zeros(80, np.uint8)
+    oracle().orc_slmc_keyed(L, want, 1.0, 0.2, J, 0.42, 31, 3, 80, sz, wacc)
+    assert np.array_equal(acc, wacc)
+    assert [o.cluster for o in ser] == sz.tolist()
+    assert np.array_equal(sysm.download(), want)
+    assert ser[-1].energy == pytest.approx(oracle().orc_fpi_energy(L, want, 1.0, 0.2), abs=1e-9)
+    sysm.close()

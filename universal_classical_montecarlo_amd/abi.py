@@ -35,7 +35,9 @@ EXPORTED = [
     "ucmc_sun_gauge_create", "ucmc_xy_angle_create", "ucmc_plaquette_ising_create",
     "ucmc_system_destroy", "ucmc_system_info", "ucmc_colouring", "ucmc_field_upload",
     "ucmc_field_download", "ucmc_metropolis_run", "ucmc_metropolis_trace", "ucmc_wolff_run",
-    "ucmc_observe", "ucmc_metropolis_bench",
+    "ucmc_observe", "ucmc_reunitarize", "ucmc_metropolis_bench", "ucmc_pt_create",
+    "ucmc_pt_run", "ucmc_pt_stats", "ucmc_pt_autotune", "ucmc_slmc_collect",
+    "ucmc_slmc_fit_normal", "ucmc_slmc_loss_grad", "ucmc_slmc_fit_adam", "ucmc_slmc_run",
 ]
 
 
@@ -58,12 +60,18 @@ class Observables(C.Structure):
 class MetropolisParams(C.Structure):
     _fields_ = [("beta", C.c_double), ("proposal", C.c_int), ("width", C.c_double),
                 ("hits", C.c_int), ("run_seed", C.c_uint64), ("sweep0", C.c_uint64),
-                ("nsweeps", C.c_uint64), ("flags", C.c_uint32)]
+                ("nsweeps", C.c_uint64), ("flags", C.c_uint32),
+                ("reunitarize_every", C.c_uint32)]
 
 
 class WolffParams(C.Structure):
     _fields_ = [("beta", C.c_double), ("run_seed", C.c_uint64), ("flip0", C.c_uint64),
-                ("nflips", C.c_uint64), ("flags", C.c_uint32)]
+                ("nflips", C.c_uint64), ("flags", C.c_uint32), ("touch_target", C.c_uint64)]
+
+
+class PtParams(C.Structure):
+    _fields_ = [("width", C.c_double), ("run_seed", C.c_uint64), ("sweep0", C.c_uint64),
+                ("nsweeps", C.c_uint64), ("tile", C.c_uint32), ("use_graph", C.c_uint32)]
 
 
 class UcmcError(RuntimeError):
@@ -108,6 +116,19 @@ def lib():
     L.ucmc_metropolis_trace.argtypes = [vp, C.POINTER(MetropolisParams), C.c_void_p, C.c_void_p]
     L.ucmc_wolff_run.argtypes = [vp, C.POINTER(WolffParams), C.c_void_p, C.c_void_p]
     L.ucmc_observe.argtypes = [vp, C.POINTER(Observables)]
+    L.ucmc_reunitarize.argtypes = [vp]
+    L.ucmc_pt_create.argtypes = [C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                                 C.c_int, C.POINTER(vp)]
+    L.ucmc_pt_run.argtypes = [vp, C.POINTER(PtParams), C.c_void_p, C.c_void_p]
+    L.ucmc_pt_stats.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.ucmc_slmc_collect.argtypes = [vp, C.c_double, C.c_uint64, C.c_uint64, C.c_uint64,
+                                    C.c_uint64, C.c_void_p]
+    L.ucmc_slmc_fit_normal.argtypes = [vp, C.c_void_p, C.c_void_p]
+    L.ucmc_slmc_loss_grad.argtypes = [vp, C.c_void_p, C.c_void_p]
+    L.ucmc_slmc_fit_adam.argtypes = [vp, C.c_int, C.c_double, C.c_void_p, C.POINTER(C.c_double)]
+    L.ucmc_slmc_run.argtypes = [vp, C.c_void_p, C.c_double, C.c_uint64, C.c_uint64, C.c_uint64,
+                                C.c_void_p, C.c_void_p]
+    L.ucmc_pt_autotune.argtypes = [vp, C.c_double, C.POINTER(C.c_uint32), C.POINTER(C.c_double)]
     L.ucmc_metropolis_bench.argtypes = [vp, C.POINTER(MetropolisParams), C.c_int,
                                         C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
     _lib = L
@@ -195,6 +216,73 @@ class System:
         check(lib().ucmc_plaquette_ising_create(L, float(J1), float(K), device, C.byref(h)))
         return cls(h, np.int8, 1)
 
+    @classmethod
+    def pt(cls, Ls, ladders, betas, J=1.0, device=0):
+        L = np.ascontiguousarray(Ls, dtype=np.uint64)
+        ld = np.ascontiguousarray(ladders, dtype=np.uint32)
+        b = np.ascontiguousarray(betas, dtype=np.float64)
+        h = C.c_void_p()
+        check(lib().ucmc_pt_create(len(L), _ptr(L), _ptr(ld), _ptr(b), float(J), device,
+                                   C.byref(h)))
+        obj = cls(h, np.float64, 1)
+        obj.nreps = len(L)
+        return obj
+
+    def pt_run(self, width, nsweeps, run_seed, sweep0=0, tile=0, use_graph=False, record=False):
+        p = PtParams(float(width), int(run_seed), int(sweep0), int(nsweeps), int(tile),
+                     int(bool(use_graph)))
+        series = np.zeros((nsweeps, self.nreps)) if record else None
+        temp = np.zeros(self.nreps, np.uint32)
+        check(lib().ucmc_pt_run(self.h, C.byref(p), None if series is None else _ptr(series),
+                                _ptr(temp)))
+        return series, temp
+
+    def pt_stats(self):
+        a = np.zeros(self.nreps, np.uint64)
+        sa = np.zeros(self.nreps, np.uint64)
+        st = np.zeros(self.nreps, np.uint64)
+        check(lib().ucmc_pt_stats(self.h, _ptr(a), _ptr(sa), _ptr(st)))
+        return a, sa, st
+
+    def pt_autotune(self, width):
+        t = C.c_uint32()
+        ms = C.c_double()
+        check(lib().ucmc_pt_autotune(self.h, float(width), C.byref(t), C.byref(ms)))
+        return t.value, ms.value
+
+    # -- self-learning Monte Carlo --
+    def slmc_collect(self, beta, burn, nconfigs, every, run_seed):
+        corr = np.zeros((nconfigs, 5))
+        check(lib().ucmc_slmc_collect(self.h, float(beta), int(burn), int(nconfigs), int(every),
+                                      int(run_seed), _ptr(corr)))
+        return corr
+
+    def slmc_fit_normal(self):
+        G = np.zeros((16, 16), np.float32)
+        th = np.zeros(4)
+        check(lib().ucmc_slmc_fit_normal(self.h, _ptr(G), _ptr(th)))
+        return th, G
+
+    def slmc_loss_grad(self, theta):
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        out = np.zeros(5)
+        check(lib().ucmc_slmc_loss_grad(self.h, _ptr(th), _ptr(out)))
+        return out[0], out[1:].copy()
+
+    def slmc_fit_adam(self, iters, lr, theta0):
+        th = np.ascontiguousarray(theta0, dtype=np.float64).copy()
+        loss = C.c_double()
+        check(lib().ucmc_slmc_fit_adam(self.h, int(iters), float(lr), _ptr(th), C.byref(loss)))
+        return th, loss.value
+
+    def slmc_run(self, J_eff, beta, nflips, run_seed, flip0=0, record=True):
+        J = np.ascontiguousarray(J_eff, dtype=np.float64)
+        acc = np.zeros(nflips, np.uint8)
+        series = (Observables * nflips)() if record else None
+        check(lib().ucmc_slmc_run(self.h, _ptr(J), float(beta), int(run_seed), int(flip0),
+                                  int(nflips), series, _ptr(acc)))
+        return series, acc
+
     def close(self):
         if self.h:
             check(lib().ucmc_system_destroy(self.h))
@@ -223,13 +311,14 @@ class System:
 
     # -- drivers --
     @staticmethod
-    def _mparams(beta, proposal, width, hits, run_seed, sweep0, nsweeps, flags):
+    def _mparams(beta, proposal, width, hits, run_seed, sweep0, nsweeps, flags, reunit=0):
         return MetropolisParams(float(beta), int(proposal), float(width), int(hits),
-                                int(run_seed), int(sweep0), int(nsweeps), int(flags))
+                                int(run_seed), int(sweep0), int(nsweeps), int(flags), int(reunit))
 
     def metropolis(self, beta, nsweeps, run_seed, sweep0=0, proposal=PROP_FLIP, width=0.0,
-                   hits=1, flags=0, record=False):
-        p = self._mparams(beta, proposal, width, hits, run_seed, sweep0, nsweeps, flags)
+                   hits=1, flags=0, record=False, reunitarize_every=0):
+        p = self._mparams(beta, proposal, width, hits, run_seed, sweep0, nsweeps, flags,
+                          reunitarize_every)
         if record:
             series = (Observables * nsweeps)()
             check(lib().ucmc_metropolis_run(self.h, C.byref(p), series))
@@ -239,17 +328,21 @@ class System:
 
     def trace(self, beta, run_seed, sweep0=0, proposal=PROP_FLIP, width=0.0, hits=1):
         p = self._mparams(beta, proposal, width, hits, run_seed, sweep0, 1, 0)
-        dE = np.empty(self.sites, np.float32)
-        acc = np.empty(self.sites, np.uint8)
+        dE = np.empty(self.sites * hits, np.float32)
+        acc = np.empty(self.sites * hits, np.uint8)
         check(lib().ucmc_metropolis_trace(self.h, C.byref(p), _ptr(dE), _ptr(acc)))
         return dE, acc
 
-    def wolff(self, beta, nflips, run_seed, flip0=0, flags=0, record=False):
-        p = WolffParams(float(beta), int(run_seed), int(flip0), int(nflips), int(flags))
+    def wolff(self, beta, nflips, run_seed, flip0=0, flags=0, record=False, touch_target=0):
+        p = WolffParams(float(beta), int(run_seed), int(flip0), int(nflips), int(flags),
+                        int(touch_target))
         sizes = np.zeros(nflips, np.uint64)
         series = (Observables * nflips)() if record else None
         check(lib().ucmc_wolff_run(self.h, C.byref(p), _ptr(sizes), series))
         return sizes, series
+
+    def reunitarize(self):
+        check(lib().ucmc_reunitarize(self.h))
 
     def observe(self):
         o = Observables()
@@ -257,8 +350,9 @@ class System:
         return o
 
     def bench(self, beta, nsweeps, run_seed, repeats=1, proposal=PROP_FLIP, width=0.0, hits=1,
-              flags=0):
-        p = self._mparams(beta, proposal, width, hits, run_seed, 0, nsweeps, flags)
+              flags=0, reunitarize_every=0):
+        p = self._mparams(beta, proposal, width, hits, run_seed, 0, nsweeps, flags,
+                          reunitarize_every)
         ms = C.c_double()
         n = C.c_uint64()
         check(lib().ucmc_metropolis_bench(self.h, C.byref(p), repeats, C.byref(ms), C.byref(n)))

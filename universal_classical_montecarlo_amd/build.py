@@ -42,10 +42,17 @@ def build_library(jobs=8):
     return LIB
 
 
+def build_host_programs():
+    """C++23 test programs over the host layer (include/ucmc/gpu.hpp), linked
+    against libucmc.so; host code only, compiled with g++."""
+    os.makedirs(OBJ, exist_ok=True)
+    out = os.path.join(OBJ, "host_layer_check")
+    _run(["g++", "-std=c++23", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"),
+          "-I" + os.path.join(HERE, "include"), os.path.join(ROOT, "tests", "cpp", "host_layer_check.cpp"),
+          "-L" + HERE, "-lucmc", "-Wl,-rpath,$ORIGIN/..", "-o", out])
+    return [out]
+
+
 if __name__ == "__main__":
     print(build_library())
-
-
-def build_host_programs():
-    """C++23 programs over the host layer (include/ucmc/gpu.hpp); none yet."""
-    return []
+    print(build_host_programs())

@@ -365,36 +365,182 @@ int ucmc_vector_create(int N, const ucmc_pair_graph* graph, double J, int device
 int ucmc_u1_gauge_create(size_t L, double beta, int device, ucmc_system** out)
 {
     return guard([&] {
-        (void)L; (void)beta; (void)device;
-        if (out) *out = nullptr;
-        fail(UCMC_E_UNSUPPORTED, "U1Gauge not built yet");
+        if (!out) fail(UCMC_E_INVALID_ARGUMENT, "null out");
+        *out = nullptr;
+        auto* s = new ucmc_system();
+        try {
+            s->kind = UCMC_U1_GAUGE;
+            s->L = L;
+            s->model_beta = beta;
+            open_device(*s, device);
+            gauge_setup(*s);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
     });
 }
 
 int ucmc_sun_gauge_create(int N, size_t L, double beta, int device, ucmc_system** out)
 {
     return guard([&] {
-        (void)N; (void)L; (void)beta; (void)device;
-        if (out) *out = nullptr;
-        fail(UCMC_E_UNSUPPORTED, "SUGauge not built yet");
+        if (!out) fail(UCMC_E_INVALID_ARGUMENT, "null out");
+        *out = nullptr;
+        if (N != 2 && N != 3) fail(UCMC_E_UNSUPPORTED, "SUGauge<N> on device needs N in {2, 3}");
+        auto* s = new ucmc_system();
+        try {
+            s->kind = UCMC_SUN_GAUGE;
+            s->N = N;
+            s->L = L;
+            s->model_beta = beta;
+            open_device(*s, device);
+            gauge_setup(*s);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
     });
 }
 
 int ucmc_xy_angle_create(const ucmc_pair_graph* graph, double J, int device, ucmc_system** out)
 {
     return guard([&] {
-        (void)graph; (void)J; (void)device;
-        if (out) *out = nullptr;
-        fail(UCMC_E_UNSUPPORTED, "XY angle model not built yet");
+        if (!out) fail(UCMC_E_INVALID_ARGUMENT, "null out");
+        *out = nullptr;
+        auto* s = new ucmc_system();
+        try {
+            s->kind = UCMC_XY_ANGLE;
+            s->J = J;
+            open_device(*s, device);
+            take_graph(*s, graph);
+            xy_setup(*s);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+int ucmc_pt_create(size_t nreplicas, const size_t* L, const uint32_t* ladder, const double* betas,
+                   double J, int device, ucmc_system** out)
+{
+    return guard([&] {
+        if (!out) fail(UCMC_E_INVALID_ARGUMENT, "null out");
+        *out = nullptr;
+        auto* s = new ucmc_system();
+        try {
+            s->kind = UCMC_XY_PT;
+            s->J = J;
+            open_device(*s, device);
+            pt_setup(*s, nreplicas, L, ladder, betas);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+int ucmc_pt_run(ucmc_system* sys, const ucmc_pt_params* p, double* energy_series,
+                uint32_t* temperature_of)
+{
+    return guard([&] {
+        System* s = checked(sys);
+        if (!p) fail(UCMC_E_INVALID_ARGUMENT, "null params");
+        if (s->kind != UCMC_XY_PT) fail(UCMC_E_INVALID_ARGUMENT, "not a parallel-tempering system");
+        pt_run(*s, *p, energy_series, temperature_of);
+    });
+}
+
+int ucmc_pt_stats(const ucmc_system* sys, uint64_t* accepts, uint64_t* swap_accepts,
+                  uint64_t* swap_tries)
+{
+    return guard([&] {
+        if (!sys || sys->kind != UCMC_XY_PT) fail(UCMC_E_INVALID_ARGUMENT, "not a parallel-tempering system");
+        UCMC_HIP(hipSetDevice(sys->device));
+        pt_stats(*sys, accepts, swap_accepts, swap_tries);
+    });
+}
+
+int ucmc_pt_autotune(ucmc_system* sys, double width, uint32_t* tile, double* best_ms_per_sweep)
+{
+    return guard([&] {
+        System* s = checked(sys);
+        if (s->kind != UCMC_XY_PT) fail(UCMC_E_INVALID_ARGUMENT, "not a parallel-tempering system");
+        pt_autotune(*s, width, tile, best_ms_per_sweep);
     });
 }
 
 int ucmc_plaquette_ising_create(size_t L, double J1, double K, int device, ucmc_system** out)
 {
     return guard([&] {
-        (void)L; (void)J1; (void)K; (void)device;
-        if (out) *out = nullptr;
-        fail(UCMC_E_UNSUPPORTED, "plaquette Ising not built yet");
+        if (!out) fail(UCMC_E_INVALID_ARGUMENT, "null out");
+        *out = nullptr;
+        if (!(J1 == J1) || !(K == K)) fail(UCMC_E_INVALID_ARGUMENT, "coupling is NaN");
+        auto* s = new ucmc_system();
+        try {
+            s->kind = UCMC_PLAQ_ISING;
+            s->L = L;
+            s->J = J1;
+            s->K = K;
+            open_device(*s, device);
+            fpi_setup(*s);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+namespace {
+System* plaq(ucmc_system* sys)
+{
+    System* s = checked(sys);
+    if (s->kind != UCMC_PLAQ_ISING) fail(UCMC_E_INVALID_ARGUMENT, "SLMC needs a plaquette-Ising system");
+    return s;
+}
+}  // namespace
+
+int ucmc_slmc_collect(ucmc_system* sys, double beta, uint64_t burn, uint64_t nconfigs,
+                      uint64_t every, uint64_t run_seed, double* correlations)
+{
+    return guard([&] { slmc_collect(*plaq(sys), beta, burn, nconfigs, every, run_seed, correlations); });
+}
+
+int ucmc_slmc_fit_normal(ucmc_system* sys, float* gram, double* theta)
+{
+    return guard([&] {
+        if (!theta) fail(UCMC_E_INVALID_ARGUMENT, "null theta");
+        slmc_gram(*plaq(sys), gram, theta);
+    });
+}
+
+int ucmc_slmc_loss_grad(ucmc_system* sys, const double* theta, double* out)
+{
+    return guard([&] {
+        if (!theta || !out) fail(UCMC_E_INVALID_ARGUMENT, "null argument");
+        slmc_loss_grad_host(*plaq(sys), theta, out);
+    });
+}
+
+int ucmc_slmc_fit_adam(ucmc_system* sys, int iters, double lr, double* theta, double* mean_loss)
+{
+    return guard([&] {
+        if (!theta || iters < 0) fail(UCMC_E_INVALID_ARGUMENT, "bad argument");
+        slmc_adam_fit(*plaq(sys), iters, lr, theta, mean_loss);
+    });
+}
+
+int ucmc_slmc_run(ucmc_system* sys, const double* J_eff, double beta, uint64_t run_seed,
+                  uint64_t flip0, uint64_t nflips, ucmc_observables* series, uint8_t* accepted)
+{
+    return guard([&] {
+        if (!J_eff) fail(UCMC_E_INVALID_ARGUMENT, "null couplings");
+        slmc_run(*plaq(sys), J_eff, beta, run_seed, flip0, nflips, series, accepted);
     });
 }
 
@@ -446,6 +592,12 @@ int ucmc_field_upload(ucmc_system* sys, const void* host, size_t bytes)
             UCMC_HIP(hipMemcpyAsync(s->field[s->cur].p, host, bytes, hipMemcpyHostToDevice, s->stream));
             break;
         }
+        case UCMC_U1_GAUGE:
+        case UCMC_SUN_GAUGE:
+            if (bytes != s->sites * field_doubles_per_site(*s) * sizeof(double))
+                fail(UCMC_E_INVALID_ARGUMENT, "field size does not match the model");
+            gauge_upload(*s, static_cast<const double*>(host));
+            break;
         default: {
             const std::size_t per = field_doubles_per_site(*s);
             if (bytes != s->sites * per * sizeof(double))
@@ -476,6 +628,12 @@ int ucmc_field_download(ucmc_system* sys, void* host, size_t bytes)
             UCMC_HIP(hipMemcpyAsync(host, s->field[s->cur].p, bytes, hipMemcpyDeviceToHost, s->stream));
             UCMC_HIP(hipStreamSynchronize(s->stream));
             break;
+        case UCMC_U1_GAUGE:
+        case UCMC_SUN_GAUGE:
+            if (bytes != s->sites * field_doubles_per_site(*s) * sizeof(double))
+                fail(UCMC_E_INVALID_ARGUMENT, "field size does not match the model");
+            gauge_download(*s, static_cast<double*>(host));
+            break;
         default: {
             const std::size_t per = field_doubles_per_site(*s);
             if (bytes != s->sites * per * sizeof(double))
@@ -503,6 +661,10 @@ int ucmc_metropolis_run(ucmc_system* sys, const ucmc_metropolis_params* p,
         switch (s->kind) {
         case UCMC_ISING: ising_metropolis(*s, *p, series, nullptr, nullptr); break;
         case UCMC_VECTOR: vector_metropolis(*s, *p, series, nullptr, nullptr); break;
+        case UCMC_U1_GAUGE:
+        case UCMC_SUN_GAUGE: gauge_metropolis(*s, *p, series, nullptr, nullptr); break;
+        case UCMC_XY_ANGLE: xy_metropolis(*s, *p, series, nullptr, nullptr); break;
+        case UCMC_PLAQ_ISING: fpi_metropolis(*s, *p, series, nullptr, nullptr); break;
         default: fail(UCMC_E_UNSUPPORTED, "model kind");
         }
     });
@@ -519,6 +681,10 @@ int ucmc_metropolis_trace(ucmc_system* sys, const ucmc_metropolis_params* p, flo
         switch (s->kind) {
         case UCMC_ISING: ising_metropolis(*s, *p, nullptr, accepted, dE); break;
         case UCMC_VECTOR: vector_metropolis(*s, *p, nullptr, accepted, dE); break;
+        case UCMC_U1_GAUGE:
+        case UCMC_SUN_GAUGE: gauge_metropolis(*s, *p, nullptr, accepted, dE); break;
+        case UCMC_XY_ANGLE: xy_metropolis(*s, *p, nullptr, accepted, dE); break;
+        case UCMC_PLAQ_ISING: fpi_metropolis(*s, *p, nullptr, accepted, dE); break;
         default: fail(UCMC_E_UNSUPPORTED, "model kind");
         }
     });
@@ -547,8 +713,20 @@ int ucmc_observe(ucmc_system* sys, ucmc_observables* out)
         switch (s->kind) {
         case UCMC_ISING: ising_observe(*s, *out); break;
         case UCMC_VECTOR: vector_observe(*s, *out); break;
+        case UCMC_U1_GAUGE:
+        case UCMC_SUN_GAUGE: gauge_observe(*s, *out); break;
+        case UCMC_XY_ANGLE: xy_observe(*s, *out); break;
+        case UCMC_PLAQ_ISING: fpi_observe(*s, *out); break;
         default: fail(UCMC_E_UNSUPPORTED, "model kind");
         }
+    });
+}
+
+int ucmc_reunitarize(ucmc_system* sys)
+{
+    return guard([&] {
+        System* s = checked(sys);
+        gauge_reunitarize(*s);
     });
 }
 
@@ -563,6 +741,10 @@ int ucmc_metropolis_bench(ucmc_system* sys, const ucmc_metropolis_params* p, int
         switch (s->kind) {
         case UCMC_ISING: n = ising_bench(*s, *p, repeats, ms); break;
         case UCMC_VECTOR: n = vector_bench(*s, *p, repeats, ms); break;
+        case UCMC_U1_GAUGE:
+        case UCMC_SUN_GAUGE: n = gauge_bench(*s, *p, repeats, ms); break;
+        case UCMC_XY_ANGLE: n = xy_bench(*s, *p, repeats, ms); break;
+        case UCMC_PLAQ_ISING: n = fpi_bench(*s, *p, repeats, ms); break;
         default: fail(UCMC_E_UNSUPPORTED, "model kind");
         }
         *elapsed_ms = ms;

@@ -1,0 +1,765 @@
+// gauge.hip -- 4D lattice gauge theories on the GPU: U1Gauge
+// (example/u1_gauge.hpp:16-85) with U1Shift (dof/u1.hpp:35-50) and SUGauge<N>
+// (example/sun_gauge.hpp:16-86) with SUShift<N> (dof/sun.hpp:237-278), over the
+// table of hypercubic_plaquettes(L) (lattice/hypercubic.hpp:15-64), generated
+// on the fly: link (mu, c) with c = (t, z, y, x), direction mu bumps c[mu].
+//
+// The reference recomputes every incident plaquette before and after
+// (example/sun_gauge.hpp:37-51).  Here each link sums its 6 staples once,
+//   A = sum_{nu != mu} U_nu(c+mu) U_mu(c+nu)^+ U_nu(c)^+
+//                    + U_nu(c-nu+mu)^+ U_mu(c-nu)^+ U_nu(c-nu),
+// so that each incident plaquette is Re Tr(U A_p) / N, and every hit reuses it:
+//   dE = (beta / N) Re Tr((U - U') A).
+// Links of one colour (mu, parity of t+z+y+x) share no plaquette for even L,
+// so the 8 colour classes run as 8 launches, one thread per link, `hits`
+// Metropolis hits each drawing from the link's own Pcg32(key, link).
+//
+// Storage: fp32, 2 N^2 planes (re, im of the row-major entries) for SU(N), one
+// angle plane for U(1); within a plane, links are ordered
+// mu L^4 + parity L^4/2 + (linear(c) >> 1) so one colour class is contiguous.
+// FP64 is used only for the plaquette / action sums and the periodic
+// reunitarisation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "ucmc_core.hpp"
+#include "ucmc_device.hpp"
+
+namespace ucmc {
+
+namespace {
+
+struct Geo4 {
+    std::uint32_t L, L4, half;  // half = L4 / 2
+};
+
+// Reference link id (hypercubic.hpp:37-39) of direction mu at c = (t, z, y, x).
+__host__ __device__ inline std::uint32_t lin4(const Geo4& g, const std::uint32_t* c)
+{
+    return ((c[0] * g.L + c[1]) * g.L + c[2]) * g.L + c[3];
+}
+__host__ __device__ inline std::uint32_t dev_index(const Geo4& g, std::uint32_t mu, const std::uint32_t* c)
+{
+    const std::uint32_t par = (c[0] + c[1] + c[2] + c[3]) & 1u;
+    return mu * g.L4 + par * g.half + (lin4(g, c) >> 1);
+}
+__host__ inline std::uint32_t dev_of_ref(const Geo4& g, std::uint32_t link)
+{
+    const std::uint32_t mu = link / g.L4;
+    std::uint32_t r = link % g.L4, c[4];
+    for (int k = 3; k >= 0; --k) {
+        c[k] = r % g.L;
+        r /= g.L;
+    }
+    return dev_index(g, mu, c);
+}
+__host__ __device__ inline void coords_of(const Geo4& g, std::uint32_t par, std::uint32_t h, std::uint32_t* c)
+{
+    const std::uint32_t hl = g.L >> 1;
+    std::uint32_t row = h / hl;
+    const std::uint32_t k = h % hl;
+    c[2] = row % g.L;
+    row /= g.L;
+    c[1] = row % g.L;
+    c[0] = row / g.L;
+    c[3] = 2 * k + ((c[0] + c[1] + c[2] + par) & 1u);
+}
+__device__ __forceinline__ void shift(const Geo4& g, std::uint32_t* c, int mu, int d)
+{
+    c[mu] = d > 0 ? (c[mu] + 1 == g.L ? 0 : c[mu] + 1) : (c[mu] == 0 ? g.L - 1 : c[mu] - 1);
+}
+
+// ------------------------------------------------------------- SU(N) ops --
+template <int N>
+struct Mat {
+    float r[N * N], i[N * N];
+};
+
+template <int N>
+__device__ __forceinline__ void load(const float* f, std::uint32_t nl, std::uint32_t idx, Mat<N>& m)
+{
+#pragma unroll
+    for (int k = 0; k < N * N; ++k) {
+        m.r[k] = f[(2 * k) * static_cast<std::size_t>(nl) + idx];
+        m.i[k] = f[(2 * k + 1) * static_cast<std::size_t>(nl) + idx];
+    }
+}
+template <int N>
+__device__ __forceinline__ void store(float* f, std::uint32_t nl, std::uint32_t idx, const Mat<N>& m)
+{
+#pragma unroll
+    for (int k = 0; k < N * N; ++k) {
+        f[(2 * k) * static_cast<std::size_t>(nl) + idx] = m.r[k];
+        f[(2 * k + 1) * static_cast<std::size_t>(nl) + idx] = m.i[k];
+    }
+}
+// C = A op(B), op = identity or dagger
+template <int N, bool DAGB>
+__device__ __forceinline__ void mul(const Mat<N>& a, const Mat<N>& b, Mat<N>& c)
+{
+#pragma unroll
+    for (int x = 0; x < N; ++x)
+#pragma unroll
+        for (int y = 0; y < N; ++y) {
+            float sr = 0.f, si = 0.f;
+#pragma unroll
+            for (int k = 0; k < N; ++k) {
+                const float ar = a.r[x * N + k], ai = a.i[x * N + k];
+                const float br = DAGB ? b.r[y * N + k] : b.r[k * N + y];
+                const float bi = DAGB ? -b.i[y * N + k] : b.i[k * N + y];
+                sr += ar * br - ai * bi;
+                si += ar * bi + ai * br;
+            }
+            c.r[x * N + y] = sr;
+            c.i[x * N + y] = si;
+        }
+}
+// C = A^+ B
+template <int N>
+__device__ __forceinline__ void mul_dag_a(const Mat<N>& a, const Mat<N>& b, Mat<N>& c)
+{
+#pragma unroll
+    for (int x = 0; x < N; ++x)
+#pragma unroll
+        for (int y = 0; y < N; ++y) {
+            float sr = 0.f, si = 0.f;
+#pragma unroll
+            for (int k = 0; k < N; ++k) {
+                const float ar = a.r[k * N + x], ai = -a.i[k * N + x];
+                const float br = b.r[k * N + y], bi = b.i[k * N + y];
+                sr += ar * br - ai * bi;
+                si += ar * bi + ai * br;
+            }
+            c.r[x * N + y] = sr;
+            c.i[x * N + y] = si;
+        }
+}
+// Re Tr(A B)
+template <int N>
+__device__ __forceinline__ float retr_mul(const Mat<N>& a, const Mat<N>& b)
+{
+    float t = 0.f;
+#pragma unroll
+    for (int x = 0; x < N; ++x)
+#pragma unroll
+        for (int k = 0; k < N; ++k) t += a.r[x * N + k] * b.r[k * N + x] - a.i[x * N + k] * b.i[k * N + x];
+    return t;
+}
+template <int N>
+__device__ __forceinline__ float norm_inf(const Mat<N>& a)
+{
+    float n = 0.f;
+#pragma unroll
+    for (int x = 0; x < N; ++x) {
+        float row = 0.f;
+#pragma unroll
+        for (int y = 0; y < N; ++y) row += sqrtf(a.r[x * N + y] * a.r[x * N + y] + a.i[x * N + y] * a.i[x * N + y]);
+        n = fmaxf(n, row);
+    }
+    return n;
+}
+
+// dof/sun.hpp:192-220 in fp32: halve while the inf-norm exceeds 0.5, Taylor
+// series until the term norm drops below fp32 resolution (the reference stops
+// at 1e-16 in double), square back.
+template <int N>
+__device__ __forceinline__ void expm(const Mat<N>& a, Mat<N>& out)
+{
+    float n = norm_inf(a);
+    int sq = 0;
+    while (n > 0.5f && sq < 60) {
+        n *= 0.5f;
+        ++sq;
+    }
+    const float scale = ldexpf(1.0f, -sq);
+    Mat<N> b, term, tmp;
+#pragma unroll
+    for (int k = 0; k < N * N; ++k) {
+        b.r[k] = a.r[k] * scale;
+        b.i[k] = a.i[k] * scale;
+        out.r[k] = term.r[k] = (k % (N + 1) == 0) ? 1.f : 0.f;
+        out.i[k] = term.i[k] = 0.f;
+    }
+    for (int k = 1; k <= 30; ++k) {
+        mul<N, false>(term, b, tmp);
+        const float inv = 1.0f / static_cast<float>(k);
+#pragma unroll
+        for (int q = 0; q < N * N; ++q) {
+            term.r[q] = tmp.r[q] * inv;
+            term.i[q] = tmp.i[q] * inv;
+            out.r[q] += term.r[q];
+            out.i[q] += term.i[q];
+        }
+        if (norm_inf(term) < 1e-8f) break;
+    }
+    for (int k = 0; k < sq; ++k) {
+        mul<N, false>(out, out, tmp);
+        out = tmp;
+    }
+}
+
+__device__ __forceinline__ float uf(std::uint32_t k) { return static_cast<float>(k) * 0x1.0p-32f; }
+
+// dof/sun.hpp:245-274: X traceless Hermitian from N diagonal draws, then (a, b)
+// per i < j, scaled by width; U' = U expm(i X).
+template <int N>
+__device__ __forceinline__ void su_shift(Pcg& r, float width, const Mat<N>& u, Mat<N>& out)
+{
+    Mat<N> a;
+    float tr = 0.f, d[N];
+#pragma unroll
+    for (int x = 0; x < N; ++x) {
+        d[x] = width * (2.0f * uf(r.next()) - 1.0f);
+        tr += d[x];
+    }
+#pragma unroll
+    for (int k = 0; k < N * N; ++k) a.r[k] = a.i[k] = 0.f;
+#pragma unroll
+    for (int x = 0; x < N; ++x)
+#pragma unroll
+        for (int y = x + 1; y < N; ++y) {
+            const float av = width * (2.0f * uf(r.next()) - 1.0f);
+            const float bv = width * (2.0f * uf(r.next()) - 1.0f);
+            // x_xy = av + i bv, x_yx = av - i bv; a = i x = (-im, re)
+            a.r[x * N + y] = -bv;
+            a.i[x * N + y] = av;
+            a.r[y * N + x] = bv;
+            a.i[y * N + x] = av;
+        }
+#pragma unroll
+    for (int x = 0; x < N; ++x) {
+        a.r[x * N + x] = 0.f;
+        a.i[x * N + x] = d[x] - tr / static_cast<float>(N);
+    }
+    Mat<N> e;
+    expm<N>(a, e);
+    mul<N, false>(u, e, out);
+}
+
+struct SunArgs {
+    float* f;
+    Geo4 g;
+    std::uint32_t nl;        // links
+    std::uint32_t mu, par;   // colour class
+    std::uint64_t key;
+    float width;
+    float beta_over_n;       // model beta / N (inside energy_diff_at)
+    double driver_beta;
+    int hits;
+    unsigned long long* accepts;
+    float* trace_dE;         // [link(ref id)][hit]
+    std::uint8_t* trace_acc;
+};
+
+template <int N, bool TRACE>
+__global__ __launch_bounds__(256) void sun_colour(SunArgs a)
+{
+    const std::uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+    std::uint32_t nacc = 0;
+    if (h < a.g.half) {
+        std::uint32_t c[4];
+        coords_of(a.g, a.par, h, c);
+        const int mu = static_cast<int>(a.mu);
+        const std::uint32_t self = a.mu * a.g.L4 + a.par * a.g.half + h;
+        Mat<N> A, t1, t2, U;
+#pragma unroll
+        for (int k = 0; k < N * N; ++k) A.r[k] = A.i[k] = 0.f;
+        for (int nu = 0; nu < 4; ++nu) {
+            if (nu == mu) continue;
+            std::uint32_t cm[4] = {c[0], c[1], c[2], c[3]}, cn[4] = {c[0], c[1], c[2], c[3]};
+            shift(a.g, cm, mu, +1);
+            shift(a.g, cn, nu, +1);
+            // forward: U_nu(c+mu) U_mu(c+nu)^+ U_nu(c)^+
+            Mat<N> x, y, z;
+            load<N>(a.f, a.nl, dev_index(a.g, nu, cm), x);
+            load<N>(a.f, a.nl, dev_index(a.g, mu, cn), y);
+            load<N>(a.f, a.nl, dev_index(a.g, nu, c), z);
+            mul<N, true>(x, y, t1);
+            mul<N, true>(t1, z, t2);
+#pragma unroll
+            for (int k = 0; k < N * N; ++k) {
+                A.r[k] += t2.r[k];
+                A.i[k] += t2.i[k];
+            }
+            // backward: U_nu(c-nu+mu)^+ U_mu(c-nu)^+ U_nu(c-nu)
+            std::uint32_t cb[4] = {c[0], c[1], c[2], c[3]};
+            shift(a.g, cb, nu, -1);
+            std::uint32_t cbm[4] = {cb[0], cb[1], cb[2], cb[3]};
+            shift(a.g, cbm, mu, +1);
+            load<N>(a.f, a.nl, dev_index(a.g, nu, cbm), x);
+            load<N>(a.f, a.nl, dev_index(a.g, mu, cb), y);
+            load<N>(a.f, a.nl, dev_index(a.g, nu, cb), z);
+            // (y x)^+ z = x^+ y^+ z
+            mul<N, false>(y, x, t1);
+            mul_dag_a<N>(t1, z, t2);
+#pragma unroll
+            for (int k = 0; k < N * N; ++k) {
+                A.r[k] += t2.r[k];
+                A.i[k] += t2.i[k];
+            }
+        }
+        load<N>(a.f, a.nl, self, U);
+        Pcg r;
+        const std::uint32_t ref_id = a.mu * a.g.L4 + lin4(a.g, c);
+        r.seed(a.key, ref_id);
+        for (int hit = 0; hit < a.hits; ++hit) {
+            Mat<N> Up;
+            su_shift<N>(r, a.width, U, Up);
+            Mat<N> D;
+#pragma unroll
+            for (int k = 0; k < N * N; ++k) {
+                D.r[k] = U.r[k] - Up.r[k];
+                D.i[k] = U.i[k] - Up.i[k];
+            }
+            const float dE = a.beta_over_n * retr_mul<N>(D, A);
+            const double log_p = -a.driver_beta * static_cast<double>(dE) + 0.0;
+            const bool acc = log_p >= 0.0 || log(r.uni()) < log_p;
+            if (acc) {
+                U = Up;
+                ++nacc;
+            }
+            if constexpr (TRACE) {
+                a.trace_dE[static_cast<std::size_t>(ref_id) * a.hits + hit] = dE;
+                a.trace_acc[static_cast<std::size_t>(ref_id) * a.hits + hit] = acc ? 1 : 0;
+            }
+        }
+        if (nacc) store<N>(a.f, a.nl, self, U);
+    }
+    if (a.accepts) {
+        unsigned long long v = nacc;
+        v = wave_sum_u64(v);
+        if ((threadIdx.x & 63) == 0 && v) atomicAdd(a.accepts, v);
+    }
+}
+
+// --------------------------------------------------------------- U(1) ---
+struct U1Args {
+    float* th;
+    Geo4 g;
+    std::uint32_t mu, par;
+    std::uint64_t key;
+    float width;
+    float beta;              // model beta
+    double driver_beta;
+    int hits;
+    unsigned long long* accepts;
+    float* trace_dE;
+    std::uint8_t* trace_acc;
+};
+
+template <bool TRACE>
+__global__ __launch_bounds__(256) void u1_colour(U1Args a)
+{
+    const std::uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+    std::uint32_t nacc = 0;
+    if (h < a.g.half) {
+        std::uint32_t c[4];
+        coords_of(a.g, a.par, h, c);
+        const int mu = static_cast<int>(a.mu);
+        const std::uint32_t self = a.mu * a.g.L4 + a.par * a.g.half + h;
+        // W = sum_p exp(i phi_p) with cos(theta_p) = cos(theta + phi_p)
+        float wr = 0.f, wi = 0.f;
+        for (int nu = 0; nu < 4; ++nu) {
+            if (nu == mu) continue;
+            std::uint32_t cm[4] = {c[0], c[1], c[2], c[3]}, cn[4] = {c[0], c[1], c[2], c[3]};
+            shift(a.g, cm, mu, +1);
+            shift(a.g, cn, nu, +1);
+            const float phf = a.th[dev_index(a.g, nu, cm)] - a.th[dev_index(a.g, mu, cn)] - a.th[dev_index(a.g, nu, c)];
+            std::uint32_t cb[4] = {c[0], c[1], c[2], c[3]};
+            shift(a.g, cb, nu, -1);
+            std::uint32_t cbm[4] = {cb[0], cb[1], cb[2], cb[3]};
+            shift(a.g, cbm, mu, +1);
+            const float phb = -a.th[dev_index(a.g, nu, cbm)] - a.th[dev_index(a.g, mu, cb)] + a.th[dev_index(a.g, nu, cb)];
+            float s, co;
+            sincosf(phf, &s, &co);
+            wr += co;
+            wi += s;
+            sincosf(phb, &s, &co);
+            wr += co;
+            wi += s;
+        }
+        float t = a.th[self];
+        Pcg r;
+        const std::uint32_t ref_id = a.mu * a.g.L4 + lin4(a.g, c);
+        r.seed(a.key, ref_id);
+        const float two_pi = static_cast<float>(2.0 * 3.14159265358979323846);
+        for (int hit = 0; hit < a.hits; ++hit) {
+            // dof/u1.hpp:40-46
+            const float tp = remainderf(t + a.width * (2.0f * uf(r.next()) - 1.0f), two_pi);
+            float s0, c0, s1, c1;
+            sincosf(t, &s0, &c0);
+            sincosf(tp, &s1, &c1);
+            // beta sum_p [cos(t + phi) - cos(t' + phi)] = beta Re((e^{it} - e^{it'}) W)
+            const float dE = a.beta * ((c0 - c1) * wr - (s0 - s1) * wi);
+            const double log_p = -a.driver_beta * static_cast<double>(dE) + 0.0;
+            const bool acc = log_p >= 0.0 || log(r.uni()) < log_p;
+            if (acc) {
+                t = tp;
+                ++nacc;
+            }
+            if constexpr (TRACE) {
+                a.trace_dE[static_cast<std::size_t>(ref_id) * a.hits + hit] = dE;
+                a.trace_acc[static_cast<std::size_t>(ref_id) * a.hits + hit] = acc ? 1 : 0;
+            }
+        }
+        if (nacc) a.th[self] = t;
+    }
+    if (a.accepts) {
+        unsigned long long v = wave_sum_u64(nacc);
+        if ((threadIdx.x & 63) == 0 && v) atomicAdd(a.accepts, v);
+    }
+}
+
+// ------------------------------------------------- FP64 plaquette sums --
+// sum over the 6 L^4 plaquettes (mu < nu, c) of Re Tr(U0 U1 U2^+ U3^+) / N
+// (products in fp32, sum in FP64).
+template <int N>
+__global__ __launch_bounds__(256) void sun_plaquette_sum(const float* f, Geo4 g, std::uint32_t nl, double* out)
+{
+    double acc = 0.0;
+    for (std::uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < g.L4; s += gridDim.x * blockDim.x) {
+        std::uint32_t c[4], r = s;
+        for (int k = 3; k >= 0; --k) {
+            c[k] = r % g.L;
+            r /= g.L;
+        }
+        for (int mu = 0; mu < 4; ++mu)
+            for (int nu = mu + 1; nu < 4; ++nu) {
+                std::uint32_t cm[4] = {c[0], c[1], c[2], c[3]}, cn[4] = {c[0], c[1], c[2], c[3]};
+                shift(g, cm, mu, +1);
+                shift(g, cn, nu, +1);
+                Mat<N> u0, u1, u2, u3, t1, t2;
+                load<N>(f, nl, dev_index(g, mu, c), u0);
+                load<N>(f, nl, dev_index(g, nu, cm), u1);
+                load<N>(f, nl, dev_index(g, mu, cn), u2);
+                load<N>(f, nl, dev_index(g, nu, c), u3);
+                mul<N, false>(u0, u1, t1);
+                mul<N, true>(t1, u2, t2);
+                // Re Tr(t2 u3^+) = sum_xk Re(t2_xk conj(u3_xk))
+                float t = 0.f;
+#pragma unroll
+                for (int k = 0; k < N * N; ++k) t += t2.r[k] * u3.r[k] + t2.i[k] * u3.i[k];
+                acc += static_cast<double>(t) / N;
+            }
+    }
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) atomicAdd(out, acc);
+}
+
+__global__ __launch_bounds__(256) void u1_plaquette_sum(const float* th, Geo4 g, double* out)
+{
+    double acc = 0.0;
+    for (std::uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < g.L4; s += gridDim.x * blockDim.x) {
+        std::uint32_t c[4], r = s;
+        for (int k = 3; k >= 0; --k) {
+            c[k] = r % g.L;
+            r /= g.L;
+        }
+        for (int mu = 0; mu < 4; ++mu)
+            for (int nu = mu + 1; nu < 4; ++nu) {
+                std::uint32_t cm[4] = {c[0], c[1], c[2], c[3]}, cn[4] = {c[0], c[1], c[2], c[3]};
+                shift(g, cm, mu, +1);
+                shift(g, cn, nu, +1);
+                const float p = th[dev_index(g, mu, c)] + th[dev_index(g, nu, cm)] - th[dev_index(g, mu, cn)] -
+                                th[dev_index(g, nu, c)];
+                acc += cos(static_cast<double>(p));
+            }
+    }
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) atomicAdd(out, acc);
+}
+
+// FP64 reunitarisation of SU(3): normalise row 0, orthogonalise and normalise
+// row 1, row 2 = conj(row0 x row1).
+__global__ __launch_bounds__(256) void su3_reunitarize(float* f, std::uint32_t nl)
+{
+    const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nl) return;
+    double re[9], im[9];
+    for (int k = 0; k < 9; ++k) {
+        re[k] = f[(2 * k) * static_cast<std::size_t>(nl) + i];
+        im[k] = f[(2 * k + 1) * static_cast<std::size_t>(nl) + i];
+    }
+    double n0 = 0.0;
+    for (int j = 0; j < 3; ++j) n0 += re[j] * re[j] + im[j] * im[j];
+    n0 = 1.0 / sqrt(n0);
+    for (int j = 0; j < 3; ++j) {
+        re[j] *= n0;
+        im[j] *= n0;
+    }
+    double pr = 0.0, pi = 0.0;
+    for (int j = 0; j < 3; ++j) {
+        pr += re[j] * re[3 + j] + im[j] * im[3 + j];
+        pi += re[j] * im[3 + j] - im[j] * re[3 + j];
+    }
+    for (int j = 0; j < 3; ++j) {
+        const double ar = re[j], ai = im[j];
+        re[3 + j] -= pr * ar - pi * ai;
+        im[3 + j] -= pr * ai + pi * ar;
+    }
+    double n1 = 0.0;
+    for (int j = 0; j < 3; ++j) n1 += re[3 + j] * re[3 + j] + im[3 + j] * im[3 + j];
+    n1 = 1.0 / sqrt(n1);
+    for (int j = 0; j < 3; ++j) {
+        re[3 + j] *= n1;
+        im[3 + j] *= n1;
+    }
+    for (int k = 0; k < 3; ++k) {
+        const int l = (k + 1) % 3, m = (k + 2) % 3;
+        const double cr = (re[l] * re[3 + m] - im[l] * im[3 + m]) - (re[m] * re[3 + l] - im[m] * im[3 + l]);
+        const double ci = (re[l] * im[3 + m] + im[l] * re[3 + m]) - (re[m] * im[3 + l] + im[m] * re[3 + l]);
+        re[6 + k] = cr;
+        im[6 + k] = -ci;
+    }
+    for (int k = 0; k < 9; ++k) {
+        f[(2 * k) * static_cast<std::size_t>(nl) + i] = static_cast<float>(re[k]);
+        f[(2 * k + 1) * static_cast<std::size_t>(nl) + i] = static_cast<float>(im[k]);
+    }
+}
+
+Geo4 geo(const System& s)
+{
+    Geo4 g;
+    g.L = static_cast<std::uint32_t>(s.L);
+    g.L4 = g.L * g.L * g.L * g.L;
+    g.half = g.L4 / 2;
+    return g;
+}
+
+void launch_colour(System& s, const ucmc_metropolis_params& p, std::uint64_t key, int colour,
+                   unsigned long long* dacc, float* tdE, std::uint8_t* tacc)
+{
+    const Geo4 g = geo(s);
+    const std::uint32_t mu = static_cast<std::uint32_t>(colour / 2), par = static_cast<std::uint32_t>(colour % 2);
+    const dim3 grid((g.half + 255) / 256), block(256);
+    if (s.kind == UCMC_U1_GAUGE) {
+        U1Args a{};
+        a.th = s.field[s.cur].as<float>();
+        a.g = g;
+        a.mu = mu;
+        a.par = par;
+        a.key = key;
+        a.width = static_cast<float>(p.width);
+        a.beta = static_cast<float>(s.model_beta);
+        a.driver_beta = p.beta;
+        a.hits = p.hits;
+        a.accepts = dacc;
+        a.trace_dE = tdE;
+        a.trace_acc = tacc;
+        if (tdE) hipLaunchKernelGGL(u1_colour<true>, grid, block, 0, s.stream, a);
+        else hipLaunchKernelGGL(u1_colour<false>, grid, block, 0, s.stream, a);
+    } else {
+        SunArgs a{};
+        a.f = s.field[s.cur].as<float>();
+        a.g = g;
+        a.nl = static_cast<std::uint32_t>(s.sites);
+        a.mu = mu;
+        a.par = par;
+        a.key = key;
+        a.width = static_cast<float>(p.width);
+        a.beta_over_n = static_cast<float>(s.model_beta / s.N);
+        a.driver_beta = p.beta;
+        a.hits = p.hits;
+        a.accepts = dacc;
+        a.trace_dE = tdE;
+        a.trace_acc = tacc;
+        if (s.N == 3) {
+            if (tdE) hipLaunchKernelGGL((sun_colour<3, true>), grid, block, 0, s.stream, a);
+            else hipLaunchKernelGGL((sun_colour<3, false>), grid, block, 0, s.stream, a);
+        } else {
+            if (tdE) hipLaunchKernelGGL((sun_colour<2, true>), grid, block, 0, s.stream, a);
+            else hipLaunchKernelGGL((sun_colour<2, false>), grid, block, 0, s.stream, a);
+        }
+    }
+    UCMC_HIP(hipGetLastError());
+}
+
+void plaquette_sum(System& s, double* out)
+{
+    const Geo4 g = geo(s);
+    const unsigned blocks = std::min<unsigned>((g.L4 + 255) / 256, 1024);
+    if (s.kind == UCMC_U1_GAUGE)
+        hipLaunchKernelGGL(u1_plaquette_sum, dim3(blocks), dim3(256), 0, s.stream, s.field[s.cur].as<float>(), g, out);
+    else if (s.N == 3)
+        hipLaunchKernelGGL(sun_plaquette_sum<3>, dim3(blocks), dim3(256), 0, s.stream, s.field[s.cur].as<float>(), g,
+                           static_cast<std::uint32_t>(s.sites), out);
+    else
+        hipLaunchKernelGGL(sun_plaquette_sum<2>, dim3(blocks), dim3(256), 0, s.stream, s.field[s.cur].as<float>(), g,
+                           static_cast<std::uint32_t>(s.sites), out);
+    UCMC_HIP(hipGetLastError());
+}
+
+void reunit_launch(System& s)
+{
+    const std::uint32_t nl = static_cast<std::uint32_t>(s.sites);
+    hipLaunchKernelGGL(su3_reunitarize, dim3((nl + 255) / 256), dim3(256), 0, s.stream, s.field[s.cur].as<float>(), nl);
+    UCMC_HIP(hipGetLastError());
+}
+
+void finish(const System& s, double sum, ucmc_observables& o)
+{
+    const double cells = 6.0 * static_cast<double>(geo(s).L4);
+    o.plaquette = sum / cells;
+    o.energy = s.model_beta * (cells - sum); // total_energy: beta sum_p (1 - P)
+}
+
+void check_proposal(const System& s, const ucmc_metropolis_params& p)
+{
+    if (s.kind == UCMC_U1_GAUGE && p.proposal != UCMC_PROP_U1_SHIFT)
+        fail(UCMC_E_UNSUPPORTED, "U1Gauge takes U1Shift");
+    if (s.kind == UCMC_SUN_GAUGE && p.proposal != UCMC_PROP_SU_SHIFT)
+        fail(UCMC_E_UNSUPPORTED, "SUGauge takes SUShift");
+}
+
+}  // namespace
+
+void gauge_setup(System& s)
+{
+    if (s.L == 0) fail(UCMC_E_INVALID_ARGUMENT, "hypercubic_plaquettes: L must be positive");
+    if (s.L % 2 != 0 || s.L < 2) fail(UCMC_E_UNSUPPORTED, "device gauge update needs even L");
+    if (4.0 * std::pow(static_cast<double>(s.L), 4) > 4294967295.0)
+        fail(UCMC_E_OVERFLOW, "hypercubic_plaquettes: site count does not fit uint32_t");
+    const std::size_t L4 = s.L * s.L * s.L * s.L;
+    s.sites = 4 * L4;
+    s.geom = Geom::Hyper4;
+    s.ncolours = 8;
+    s.schedule = UCMC_SCHED_LINK8;
+    s.colour.resize(s.sites);
+    for (std::size_t i = 0; i < s.sites; ++i) {
+        std::size_t r = i % L4, par = 0;
+        for (int k = 0; k < 4; ++k) {
+            par += r % s.L;
+            r /= s.L;
+        }
+        s.colour[i] = static_cast<std::uint8_t>(2 * (i / L4) + (par & 1));
+    }
+    const std::size_t planes = s.kind == UCMC_U1_GAUGE ? 1 : 2 * s.N * s.N;
+    s.field[0].alloc(s.sites * planes * sizeof(float));
+    std::vector<float> init(s.sites * planes, 0.f);
+    if (s.kind == UCMC_SUN_GAUGE) // example/sun_gauge.hpp:26: identity (cold) start
+        for (int x = 0; x < s.N; ++x) std::fill_n(init.begin() + (2 * (x * s.N + x)) * s.sites, s.sites, 1.f);
+    UCMC_HIP(hipMemcpy(s.field[0].p, init.data(), init.size() * 4, hipMemcpyHostToDevice));
+    s.scratch.alloc(64);
+}
+
+void gauge_upload(System& s, const double* host)
+{
+    const Geo4 g = geo(s);
+    const std::size_t planes = s.kind == UCMC_U1_GAUGE ? 1 : 2 * s.N * s.N;
+    std::vector<float> f(s.sites * planes);
+    for (std::uint32_t l = 0; l < s.sites; ++l) {
+        const std::uint32_t d = dev_of_ref(g, l);
+        for (std::size_t k = 0; k < planes; ++k) f[k * s.sites + d] = static_cast<float>(host[l * planes + k]);
+    }
+    UCMC_HIP(hipMemcpy(s.field[s.cur].p, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+}
+
+void gauge_download(System& s, double* host)
+{
+    const Geo4 g = geo(s);
+    const std::size_t planes = s.kind == UCMC_U1_GAUGE ? 1 : 2 * s.N * s.N;
+    std::vector<float> f(s.sites * planes);
+    UCMC_HIP(hipStreamSynchronize(s.stream));
+    UCMC_HIP(hipMemcpy(f.data(), s.field[s.cur].p, f.size() * 4, hipMemcpyDeviceToHost));
+    for (std::uint32_t l = 0; l < s.sites; ++l) {
+        const std::uint32_t d = dev_of_ref(g, l);
+        for (std::size_t k = 0; k < planes; ++k) host[l * planes + k] = f[k * s.sites + d];
+    }
+}
+
+void gauge_metropolis(System& s, const ucmc_metropolis_params& p, ucmc_observables* series,
+                      std::uint8_t* trace_acc, float* trace_dE)
+{
+    check_proposal(s, p);
+    const bool rec = series && (p.flags & (UCMC_RECORD_ENERGY | UCMC_RECORD_ORDER | UCMC_RECORD_ACCEPTS));
+    DevBuf rec_buf;
+    if (rec) {
+        rec_buf.alloc(p.nsweeps * 2 * 8);
+        UCMC_HIP(hipMemsetAsync(rec_buf.p, 0, rec_buf.bytes, s.stream));
+    }
+    DevBuf tb;
+    float* d_dE = nullptr;
+    std::uint8_t* d_acc = nullptr;
+    if (trace_acc) {
+        tb.alloc(s.sites * p.hits * 5);
+        d_dE = tb.as<float>();
+        d_acc = reinterpret_cast<std::uint8_t*>(d_dE + s.sites * p.hits);
+    }
+    for (std::uint64_t k = 0; k < p.nsweeps; ++k) {
+        const std::uint64_t key = key_of(p.run_seed, p.sweep0 + k);
+        unsigned long long* slot = rec ? rec_buf.as<unsigned long long>() + 2 * k : nullptr;
+        for (int c = 0; c < 8; ++c)
+            launch_colour(s, p, key, c, (rec && (p.flags & UCMC_RECORD_ACCEPTS)) ? slot : nullptr, d_dE, d_acc);
+        if (p.reunitarize_every && s.kind == UCMC_SUN_GAUGE && s.N == 3 && (p.sweep0 + k + 1) % p.reunitarize_every == 0)
+            reunit_launch(s);
+        if (rec && (p.flags & (UCMC_RECORD_ENERGY | UCMC_RECORD_ORDER)))
+            plaquette_sum(s, reinterpret_cast<double*>(slot + 1));
+    }
+    if (rec) {
+        std::vector<unsigned long long> h(p.nsweeps * 2);
+        UCMC_HIP(hipMemcpyAsync(h.data(), rec_buf.p, h.size() * 8, hipMemcpyDeviceToHost, s.stream));
+        UCMC_HIP(hipStreamSynchronize(s.stream));
+        for (std::uint64_t k = 0; k < p.nsweeps; ++k) {
+            ucmc_observables o{};
+            double sum;
+            std::memcpy(&sum, &h[2 * k + 1], 8);
+            finish(s, sum, o);
+            o.accepted = h[2 * k];
+            series[k] = o;
+        }
+    }
+    if (trace_acc) {
+        UCMC_HIP(hipMemcpyAsync(trace_dE, d_dE, s.sites * p.hits * 4, hipMemcpyDeviceToHost, s.stream));
+        UCMC_HIP(hipMemcpyAsync(trace_acc, d_acc, s.sites * p.hits, hipMemcpyDeviceToHost, s.stream));
+    }
+    UCMC_HIP(hipStreamSynchronize(s.stream));
+}
+
+void gauge_observe(System& s, ucmc_observables& o)
+{
+    UCMC_HIP(hipMemsetAsync(s.scratch.p, 0, 8, s.stream));
+    plaquette_sum(s, s.scratch.as<double>());
+    double sum = 0.0;
+    UCMC_HIP(hipMemcpyAsync(&sum, s.scratch.p, 8, hipMemcpyDeviceToHost, s.stream));
+    UCMC_HIP(hipStreamSynchronize(s.stream));
+    finish(s, sum, o);
+}
+
+void gauge_reunitarize(System& s)
+{
+    if (s.kind != UCMC_SUN_GAUGE || s.N != 3) fail(UCMC_E_UNSUPPORTED, "reunitarisation is for SU(3)");
+    reunit_launch(s);
+    UCMC_HIP(hipStreamSynchronize(s.stream));
+}
+
+std::uint64_t gauge_bench(System& s, const ucmc_metropolis_params& p, int repeats, float& ms)
+{
+    check_proposal(s, p);
+    hipEvent_t e0, e1;
+    UCMC_HIP(hipEventCreate(&e0));
+    UCMC_HIP(hipEventCreate(&e1));
+    UCMC_HIP(hipStreamSynchronize(s.stream));
+    UCMC_HIP(hipEventRecord(e0, s.stream));
+    std::uint64_t launches = 0;
+    for (int r = 0; r < repeats; ++r)
+        for (std::uint64_t k = 0; k < p.nsweeps; ++k) {
+            const std::uint64_t key = key_of(p.run_seed, p.sweep0 + r * p.nsweeps + k);
+            for (int c = 0; c < 8; ++c) launch_colour(s, p, key, c, nullptr, nullptr, nullptr);
+            launches += 8;
+            if (p.reunitarize_every && s.kind == UCMC_SUN_GAUGE && s.N == 3 &&
+                (p.sweep0 + r * p.nsweeps + k + 1) % p.reunitarize_every == 0)
+                reunit_launch(s);
+        }
+    UCMC_HIP(hipEventRecord(e1, s.stream));
+    UCMC_HIP(hipEventSynchronize(e1));
+    UCMC_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return launches;
+}
+
+}  // namespace ucmc

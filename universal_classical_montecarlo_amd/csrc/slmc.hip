@@ -1,0 +1,709 @@
+// slmc.hip -- self-learning Monte Carlo on the GPU for the frustrated
+// plaquette Ising model
+//   E = -J1 sum_<ij> s_i s_j - K sum_plaq s_i s_j s_k s_l      (periodic L x L)
+// (no reference model; the building blocks are the reference's Ising
+// energy_diff_at form, example/ising.hpp:46-56, its Metropolis accept rule,
+// driver.hpp:83-84, and its Wolff growth rule, driver.hpp:180-200).
+//
+//  * plain Metropolis: 4 colour classes (x & 1) + 2 (y & 1) -- the plaquette
+//    couples diagonal sites, so a checkerboard is not a valid schedule; the
+//    accept rule is exact through a 25-entry threshold table.
+//  * training data: plain-Metropolis configurations reduced on the device to
+//    the pair correlations of three shells, C_k = sum_{shell k} s_i s_j
+//    (shells (1,0), (1,1), (2,0)), and the true energy.
+//  * fit of E_eff = E0 - sum_k J_k C_k: the normal equations X^T X theta =
+//    X^T y are accumulated with bf16 MFMA (v_mfma_f32_16x16x32_bf16, each fp32
+//    feature split into a bf16 high part and a bf16 residual so the product is
+//    accurate to ~2^-16) and solved in FP64; or the same least-squares loss is
+//    minimised with Adam by hand-written forward / backward kernels.
+//  * SLMC step: a Wolff cluster grown with the effective couplings over the 12
+//    neighbours (keyed per undirected edge as in wolff.hip), then accepted with
+//    min(1, exp(-beta [dE_true - dE_eff])), so the exact model stays the judge.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "ucmc_core.hpp"
+#include "ucmc_device.hpp"
+
+namespace ucmc {
+
+namespace {
+
+constexpr std::uint64_t kSlmcCounterBase = 3ull << 61;
+
+// --------------------------------------------------- plain Metropolis --
+struct FpiArgs {
+    std::int8_t* s;
+    std::uint32_t L;
+    std::uint32_t cx, cy;     // colour class: x & 1 == cx, y & 1 == cy
+    std::uint64_t key;
+    const std::uint32_t* tm1; // [25]: index (si * nn + 4) / 2 * 5 + (pq + 4) / 2
+    double twoJ1, twoK;
+    unsigned long long* accepts;
+    float* trace_dE;
+    std::uint8_t* trace_acc;
+};
+
+__device__ __forceinline__ int at(const std::int8_t* s, std::uint32_t L, std::uint32_t x, std::uint32_t y)
+{
+    return s[static_cast<std::size_t>(y % L) * L + (x % L)];
+}
+
+template <bool TRACE>
+__global__ __launch_bounds__(256) void fpi_colour(FpiArgs a)
+{
+    const std::uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+    const std::uint32_t half = a.L / 2;
+    bool acc = false;
+    if (h < half * half) {
+        const std::uint32_t x = 2 * (h % half) + a.cx, y = 2 * (h / half) + a.cy, L = a.L;
+        const std::size_t i = static_cast<std::size_t>(y) * L + x;
+        const int si = a.s[i];
+        const std::uint32_t xm = x + L - 1, xp = x + 1, ym = y + L - 1, yp = y + 1;
+        const int nn = at(a.s, L, xp, y) + at(a.s, L, xm, y) + at(a.s, L, x, yp) + at(a.s, L, x, ym);
+        // the four plaquettes holding i: lower-left corners (x-1|x, y-1|y)
+        const int c_mm = at(a.s, L, xm, ym), c_0m = at(a.s, L, x, ym), c_pm = at(a.s, L, xp, ym);
+        const int c_m0 = at(a.s, L, xm, y), c_p0 = at(a.s, L, xp, y);
+        const int c_mp = at(a.s, L, xm, yp), c_0p = at(a.s, L, x, yp), c_pp = at(a.s, L, xp, yp);
+        const int pq = si * (c_mm * c_0m * c_m0 + c_0m * c_pm * c_p0 + c_m0 * c_mp * c_0p + c_p0 * c_0p * c_pp);
+        const int hn = si * nn;
+        const std::uint32_t u = pcg_out(first_state(a.key, i));
+        acc = u <= a.tm1[((hn + 4) / 2) * 5 + (pq + 4) / 2];
+        if (acc) a.s[i] = static_cast<std::int8_t>(-si);
+        if constexpr (TRACE) {
+            a.trace_dE[i] = static_cast<float>(a.twoJ1 * hn + a.twoK * pq);
+            a.trace_acc[i] = acc ? 1 : 0;
+        }
+    }
+    if (a.accepts) {
+        const unsigned long long c = __popcll(__ballot(acc));
+        if ((threadIdx.x & 63) == 0 && c) atomicAdd(a.accepts, c);
+    }
+}
+
+// Per-configuration reductions: out[0..2] = C1, C2, C3 (pair sums of the
+// three shells), out[3] = P (plaquette sum), out[4] = M.  One block per config.
+__global__ __launch_bounds__(256) void fpi_correlations(const std::int8_t* cfgs, std::uint32_t L,
+                                                        std::size_t stride, long long* out)
+{
+    const std::int8_t* s = cfgs + blockIdx.x * stride;
+    long long c1 = 0, c2 = 0, c3 = 0, pp = 0, m = 0;
+    for (std::uint32_t i = threadIdx.x; i < L * L; i += blockDim.x) {
+        const std::uint32_t x = i % L, y = i / L;
+        const int si = s[i];
+        const int r = at(s, L, x + 1, y), d = at(s, L, x, y + 1);
+        c1 += si * (r + d);
+        c2 += si * (at(s, L, x + 1, y + 1) + at(s, L, x + 1, y + L - 1));
+        c3 += si * (at(s, L, x + 2, y) + at(s, L, x, y + 2));
+        pp += si * r * d * at(s, L, x + 1, y + 1);
+        m += si;
+    }
+    __shared__ long long red[5];
+    if (threadIdx.x < 5) red[threadIdx.x] = 0;
+    __syncthreads();
+    c1 = static_cast<long long>(wave_sum_u64(static_cast<unsigned long long>(c1)));
+    c2 = static_cast<long long>(wave_sum_u64(static_cast<unsigned long long>(c2)));
+    c3 = static_cast<long long>(wave_sum_u64(static_cast<unsigned long long>(c3)));
+    pp = static_cast<long long>(wave_sum_u64(static_cast<unsigned long long>(pp)));
+    m = static_cast<long long>(wave_sum_u64(static_cast<unsigned long long>(m)));
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(&red[0]), static_cast<unsigned long long>(c1));
+        atomicAdd(reinterpret_cast<unsigned long long*>(&red[1]), static_cast<unsigned long long>(c2));
+        atomicAdd(reinterpret_cast<unsigned long long*>(&red[2]), static_cast<unsigned long long>(c3));
+        atomicAdd(reinterpret_cast<unsigned long long*>(&red[3]), static_cast<unsigned long long>(pp));
+        atomicAdd(reinterpret_cast<unsigned long long*>(&red[4]), static_cast<unsigned long long>(m));
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) out[blockIdx.x * 5 + threadIdx.x] = red[threadIdx.x];
+}
+
+// Feature rows x = [1, -C1/N, -C2/N, -C3/N, 0 ... 0, E/N] (16 fp32 per config,
+// column 15 carries the target so X^T X holds X^T y and y^T y).
+__global__ void slmc_features(const long long* corr, std::uint32_t n, double N, double J1, double K,
+                              float* X)
+{
+    const std::uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const long long* q = corr + 5 * static_cast<std::size_t>(c);
+    float* x = X + 16 * static_cast<std::size_t>(c);
+    for (int k = 0; k < 16; ++k) x[k] = 0.f;
+    x[0] = 1.f;
+    x[1] = static_cast<float>(-q[0] / N);
+    x[2] = static_cast<float>(-q[1] / N);
+    x[3] = static_cast<float>(-q[2] / N);
+    x[15] = static_cast<float>((-J1 * q[0] - K * q[3]) / N);
+}
+
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ short to_bf16(float v)
+{
+    // round to nearest even
+    std::uint32_t b = __float_as_uint(v);
+    b += 0x7FFFu + ((b >> 16) & 1u);
+    return static_cast<short>(b >> 16);
+}
+__device__ __forceinline__ float from_bf16(short h)
+{
+    return __uint_as_float(static_cast<std::uint32_t>(static_cast<unsigned short>(h)) << 16);
+}
+
+// G = X^T X (16 x 16, fp32) over n configs with bf16 MFMA: each wave owns a
+// slice of 32-config chunks; lane l supplies feature (l % 16) of configs
+// 8 (l / 16) .. +7 of the chunk, which is both the A (16 x 32) and the
+// B (32 x 16) operand.  The result lane l holds G[4 (l / 16) + i][l % 16].
+__global__ __launch_bounds__(256) void slmc_gram_mfma(const float* X, std::uint32_t n, float* G)
+{
+    const std::uint32_t lane = threadIdx.x & 63u, wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const std::uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    const std::uint32_t f = lane & 15u, grp = lane >> 4;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    const std::uint32_t chunks = (n + 31) / 32;
+    for (std::uint32_t ch = wave; ch < chunks; ch += nwaves) {
+        bf16x8 hi, lo;
+        for (int j = 0; j < 8; ++j) {
+            const std::uint32_t c = ch * 32 + grp * 8 + j;
+            const float v = c < n ? X[16 * static_cast<std::size_t>(c) + f] : 0.f;
+            const short h = to_bf16(v);
+            hi[j] = h;
+            lo[j] = to_bf16(v - from_bf16(h));
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi, hi, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi, lo, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lo, hi, acc, 0, 0, 0);
+    }
+    for (int i = 0; i < 4; ++i) atomicAdd(&G[(4 * grp + i) * 16 + f], acc[i]);
+}
+
+// Solves the 4 x 4 normal equations (features 0..3, right-hand side column 15)
+// by Gaussian elimination with partial pivoting in FP64.
+__global__ void slmc_solve(const float* G, double* theta)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double A[4][5];
+    for (int r = 0; r < 4; ++r) {
+        for (int c = 0; c < 4; ++c) A[r][c] = G[r * 16 + c];
+        A[r][4] = G[r * 16 + 15];
+    }
+    for (int c = 0; c < 4; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < 4; ++r)
+            if (fabs(A[r][c]) > fabs(A[piv][c])) piv = r;
+        for (int k = 0; k < 5; ++k) {
+            const double t = A[c][k];
+            A[c][k] = A[piv][k];
+            A[piv][k] = t;
+        }
+        for (int r = 0; r < 4; ++r) {
+            if (r == c || A[c][c] == 0.0) continue;
+            const double f = A[r][c] / A[c][c];
+            for (int k = c; k < 5; ++k) A[r][k] -= f * A[c][k];
+        }
+    }
+    for (int r = 0; r < 4; ++r) theta[r] = A[r][r] != 0.0 ? A[r][4] / A[r][r] : 0.0;
+}
+
+// Forward + backward of the least-squares loss L = mean_c (x_c.theta - y_c)^2:
+// out[0] += sum (pred - y)^2, out[1..4] += sum 2 (pred - y) x_c (FP64 sums).
+__global__ __launch_bounds__(256) void slmc_loss_grad(const float* X, std::uint32_t n, const double* theta,
+                                                      double* out)
+{
+    double l = 0.0, g[4] = {0.0, 0.0, 0.0, 0.0};
+    for (std::uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        const float* x = X + 16 * static_cast<std::size_t>(c);
+        double pred = 0.0;
+        for (int k = 0; k < 4; ++k) pred += theta[k] * x[k];
+        const double r = pred - x[15];
+        l += r * r;
+        for (int k = 0; k < 4; ++k) g[k] += 2.0 * r * x[k];
+    }
+    l = wave_sum(l);
+    for (int k = 0; k < 4; ++k) g[k] = wave_sum(g[k]);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(out, l);
+        for (int k = 0; k < 4; ++k) atomicAdd(out + 1 + k, g[k]);
+    }
+}
+
+// Adam (Kingma & Ba 2015) on theta with the mean gradient out[1..4] / n.
+__global__ void slmc_adam(double* theta, double* m, double* v, const double* lg, std::uint32_t n, int t,
+                          double lr, double b1, double b2, double eps)
+{
+    const int k = threadIdx.x;
+    if (k >= 4) return;
+    const double g = lg[1 + k] / n;
+    m[k] = b1 * m[k] + (1.0 - b1) * g;
+    v[k] = b2 * v[k] + (1.0 - b2) * g * g;
+    const double mh = m[k] / (1.0 - pow(b1, t)), vh = v[k] / (1.0 - pow(b2, t));
+    theta[k] -= lr * mh / (sqrt(vh) + eps);
+}
+
+// ------------------------------------------------------------ SLMC step --
+// Neighbour direction d (0..11) of (x, y): shells (1,0), (1,1), (2,0); the
+// undirected edge of a forward direction is owned by (x, y), of a backward
+// one by the neighbour: id = 6 owner + slot.
+__constant__ int kDx[12] = {1, 0, -1, 0, 1, 1, -1, -1, 2, 0, -2, 0};
+__constant__ int kDy[12] = {0, 1, 0, -1, 1, -1, -1, 1, 0, 2, 0, -2};
+__constant__ int kShell[12] = {0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 2};
+__constant__ int kSlot[12] = {0, 1, 0, 1, 2, 3, 2, 3, 4, 5, 4, 5};
+__constant__ int kForward[12] = {1, 1, 0, 0, 1, 1, 0, 0, 1, 1, 0, 0};
+
+struct SlmcArgs {
+    std::int8_t* s;
+    std::uint32_t L;
+    std::uint32_t* stamp;
+    std::uint32_t* queue;
+    double* out;              // per step: [size, accepted, dE_true, dE_eff, dM, pad]
+    std::uint64_t run_seed, flip0;
+    std::uint32_t nflips, mark0;
+    double beta, J1, K;
+    double Jeff[3];
+    std::uint64_t thr[3][2];  // [shell][aligned]
+};
+
+__device__ __forceinline__ std::uint32_t nbr(std::uint32_t L, std::uint32_t i, int d)
+{
+    const std::uint32_t x = i % L, y = i / L;
+    const std::uint32_t nx = (x + L + 2 * L + kDx[d]) % L, ny = (y + L + 2 * L + kDy[d]) % L;
+    return ny * L + nx;
+}
+
+__global__ __launch_bounds__(1024) void slmc_flips(SlmcArgs a)
+{
+    __shared__ std::uint32_t s_head, s_tail;
+    __shared__ double s_red[4];
+    __shared__ int s_accept;
+    const std::uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const std::uint32_t L = a.L, n = L * L;
+    for (std::uint32_t fi = 0; fi < a.nflips; ++fi) {
+        const std::uint32_t mark = a.mark0 + fi;
+        const std::uint64_t key = key_of(a.run_seed, kSlmcCounterBase + a.flip0 + fi);
+        if (tid == 0) {
+            Pcg ctl;
+            ctl.seed(key, 0);
+            const std::uint32_t seed = static_cast<std::uint32_t>(ctl.uni() * static_cast<double>(n));
+            a.stamp[seed] = mark;
+            a.queue[0] = seed;
+            s_head = 0;
+            s_tail = 1;
+            for (int k = 0; k < 4; ++k) s_red[k] = 0.0;
+        }
+        __syncthreads();
+        for (;;) {
+            const std::uint32_t h0 = s_head, t0 = s_tail;
+            __syncthreads();
+            if (h0 == t0) break;
+            for (std::uint32_t base = h0; base < t0; base += blockDim.x) {
+                const std::uint32_t idx = base + tid;
+                const bool have = idx < t0;
+                const std::uint32_t i = have ? a.queue[idx] : 0u;
+                const int si = have ? a.s[i] : 0;
+                for (int d = 0; d < 12; ++d) {
+                    bool won = false;
+                    std::uint32_t j = 0;
+                    if (have) {
+                        j = nbr(L, i, d);
+                        const std::uint32_t sj = a.stamp[j];
+                        if (sj != mark) {
+                            const std::uint32_t owner = kForward[d] ? i : j;
+                            const std::uint64_t e = 6ull * owner + kSlot[d];
+                            const std::uint32_t u = pcg_out(first_state(key, 1ull + e));
+                            const bool aligned = si * a.s[j] > 0;
+                            if (static_cast<std::uint64_t>(u) < a.thr[kShell[d]][aligned ? 1 : 0])
+                                won = atomicCAS(&a.stamp[j], sj, mark) == sj;
+                        }
+                    }
+                    const unsigned long long m = __ballot(won);
+                    if (m) {
+                        const int leader = __ffsll(static_cast<long long>(m)) - 1;
+                        std::uint32_t pos = 0;
+                        if (static_cast<int>(lane) == leader) pos = atomicAdd(&s_tail, static_cast<std::uint32_t>(__popcll(m)));
+                        pos = static_cast<std::uint32_t>(__shfl(static_cast<int>(pos), leader, 64));
+                        if (won) a.queue[pos + __popcll(m & ((1ull << lane) - 1ull))] = j;
+                    }
+                }
+            }
+            __syncthreads();
+            if (tid == 0) s_head = t0;
+            __syncthreads();
+        }
+        const std::uint32_t size = s_tail;
+        // dE of flipping the cluster, true and effective, on the pre-flip field
+        double dt = 0.0, de = 0.0, dm = 0.0;
+        for (std::uint32_t idx = tid; idx < size; idx += blockDim.x) {
+            const std::uint32_t i = a.queue[idx];
+            const int si = a.s[i];
+            dm += -2.0 * si;
+            for (int d = 0; d < 12; ++d) {
+                const std::uint32_t j = nbr(L, i, d);
+                if (a.stamp[j] == mark) continue;
+                const double b = 2.0 * si * a.s[j];
+                de += a.Jeff[kShell[d]] * b;
+                if (d < 4) dt += a.J1 * b;
+            }
+            // plaquettes holding i, each counted by its first in-cluster corner
+            const std::uint32_t x = i % L, y = i / L;
+            for (int py = -1; py <= 0; ++py)
+                for (int px = -1; px <= 0; ++px) {
+                    const std::uint32_t x0 = (x + L + px) % L, y0 = (y + L + py) % L;
+                    const std::uint32_t cs[4] = {y0 * L + x0, y0 * L + (x0 + 1) % L, ((y0 + 1) % L) * L + x0,
+                                                 ((y0 + 1) % L) * L + (x0 + 1) % L};
+                    int inside = 0, first = -1, prod = 1;
+                    for (int q = 0; q < 4; ++q) {
+                        const bool in = a.stamp[cs[q]] == mark;
+                        inside += in;
+                        if (in && first < 0) first = q;
+                        prod *= a.s[cs[q]];
+                    }
+                    if (cs[first] == i && (inside & 1)) dt += 2.0 * a.K * prod;
+                }
+        }
+        dt = wave_sum(dt);
+        de = wave_sum(de);
+        dm = wave_sum(dm);
+        if (lane == 0) {
+            atomicAdd(&s_red[0], dt);
+            atomicAdd(&s_red[1], de);
+            atomicAdd(&s_red[2], dm);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            // min(1, exp(-beta (dE_true - dE_eff))), the draw after the seed draw
+            Pcg ctl;
+            ctl.seed(key, 0);
+            (void)ctl.next();
+            const double log_p = -a.beta * (s_red[0] - s_red[1]) + 0.0;
+            s_accept = (log_p >= 0.0 || log(ctl.uni()) < log_p) ? 1 : 0;
+            double* o = a.out + 6 * static_cast<std::size_t>(fi);
+            o[0] = size;
+            o[1] = s_accept;
+            o[2] = s_red[0];
+            o[3] = s_red[1];
+            o[4] = s_red[2];
+        }
+        __syncthreads();
+        if (s_accept)
+            for (std::uint32_t idx = tid; idx < size; idx += blockDim.x) {
+                const std::uint32_t i = a.queue[idx];
+                a.s[i] = static_cast<std::int8_t>(-a.s[i]);
+            }
+        __threadfence();
+        __syncthreads();
+    }
+}
+
+void fpi_tables(const System& s, double beta, std::vector<std::uint32_t>& t)
+{
+    t.assign(25, 0);
+    for (int hn = -4; hn <= 4; hn += 2)
+        for (int pq = -4; pq <= 4; pq += 2) {
+            // the oracle's dE expression (orc_fpi_energy_diff_flip)
+            const double dE = 2.0 * s.J * static_cast<double>(hn) + 2.0 * s.K * static_cast<double>(pq);
+            t[((hn + 4) / 2) * 5 + (pq + 4) / 2] = static_cast<std::uint32_t>(accept_threshold(beta, dE) - 1);
+        }
+}
+
+void fpi_sweep(System& s, std::uint64_t key, const std::uint32_t* tbl, unsigned long long* dacc, float* tdE,
+               std::uint8_t* tacc)
+{
+    const std::uint32_t L = static_cast<std::uint32_t>(s.L), half = L / 2;
+    for (int c = 0; c < 4; ++c) {
+        FpiArgs a{};
+        a.s = s.field[s.cur].as<std::int8_t>();
+        a.L = L;
+        a.cx = static_cast<std::uint32_t>(c & 1);
+        a.cy = static_cast<std::uint32_t>(c >> 1);
+        a.key = key;
+        a.tm1 = tbl;
+        a.twoJ1 = 2.0 * s.J;
+        a.twoK = 2.0 * s.K;
+        a.accepts = dacc;
+        a.trace_dE = tdE;
+        a.trace_acc = tacc;
+        const dim3 grid((half * half + 255) / 256);
+        if (tdE) hipLaunchKernelGGL(fpi_colour<true>, grid, dim3(256), 0, s.stream, a);
+        else hipLaunchKernelGGL(fpi_colour<false>, grid, dim3(256), 0, s.stream, a);
+        UCMC_HIP(hipGetLastError());
+    }
+}
+
+void corr_launch(System& s, const std::int8_t* cfgs, std::uint32_t count, long long* out)
+{
+    hipLaunchKernelGGL(fpi_correlations, dim3(count), dim3(256), 0, s.stream, cfgs,
+                       static_cast<std::uint32_t>(s.L), s.sites, out);
+    UCMC_HIP(hipGetLastError());
+}
+
+void obs_from_corr(const System& s, const long long* q, ucmc_observables& o)
+{
+    o.energy = -s.J * static_cast<double>(q[0]) - s.K * static_cast<double>(q[3]);
+    o.order[0] = static_cast<double>(q[4]);
+}
+
+}  // namespace
+
+// --------------------------------------------------------------- host ---
+void fpi_setup(System& s)
+{
+    if (s.L < 4 || s.L % 2) fail(UCMC_E_INVALID_ARGUMENT, "plaquette Ising needs even L >= 4");
+    s.sites = s.L * s.L;
+    s.geom = Geom::Square;
+    s.Lx = s.Ly = s.L;
+    s.ncolours = 4;
+    s.schedule = UCMC_SCHED_FOUR;
+    s.colour.resize(s.sites);
+    for (std::size_t i = 0; i < s.sites; ++i)
+        s.colour[i] = static_cast<std::uint8_t>(((i % s.L) & 1) + 2 * ((i / s.L) & 1));
+    s.field[0].alloc(s.sites);
+    UCMC_HIP(hipMemset(s.field[0].p, 1, s.sites));
+    s.scratch.alloc(256);
+    s.scratch2.alloc(25 * 4);
+}
+
+void fpi_metropolis(System& s, const ucmc_metropolis_params& p, ucmc_observables* series,
+                    std::uint8_t* trace_acc, float* trace_dE)
+{
+    if (p.proposal != UCMC_PROP_FLIP || p.hits != 1) fail(UCMC_E_UNSUPPORTED, "plaquette Ising takes Flip, one hit");
+    std::vector<std::uint32_t> t;
+    fpi_tables(s, p.beta, t);
+    UCMC_HIP(hipMemcpyAsync(s.scratch2.p, t.data(), 100, hipMemcpyHostToDevice, s.stream));
+    const bool rec = series && (p.flags & (UCMC_RECORD_ENERGY | UCMC_RECORD_ORDER | UCMC_RECORD_ACCEPTS));
+    DevBuf rb;
+    if (rec) {
+        rb.alloc(p.nsweeps * 6 * 8);
+        UCMC_HIP(hipMemsetAsync(rb.p, 0, rb.bytes, s.stream));
+    }
+    DevBuf tb;
+    float* d_dE = nullptr;
+    std::uint8_t* d_acc = nullptr;
+    if (trace_acc) {
+        tb.alloc(s.sites * 5);
+        d_dE = tb.as<float>();
+        d_acc = reinterpret_cast<std::uint8_t*>(d_dE + s.sites);
+    }
+    for (std::uint64_t k = 0; k < p.nsweeps; ++k) {
+        unsigned long long* slot = rec ? rb.as<unsigned long long>() + 6 * k : nullptr;
+        fpi_sweep(s, key_of(p.run_seed, p.sweep0 + k), s.scratch2.as<std::uint32_t>(),
+                  (rec && (p.flags & UCMC_RECORD_ACCEPTS)) ? slot : nullptr, d_dE, d_acc);
+        if (rec && (p.flags & (UCMC_RECORD_ENERGY | UCMC_RECORD_ORDER)))
+            corr_launch(s, s.field[s.cur].as<std::int8_t>(), 1, reinterpret_cast<long long*>(slot + 1));
+    }
+    if (rec) {
+        std::vector<long long> h(p.nsweeps * 6);
+        UCMC_HIP(hipMemcpyAsync(h.data(), rb.p, h.size() * 8, hipMemcpyDeviceToHost, s.stream));
+        UCMC_HIP(hipStreamSynchronize(s.stream));
+        for (std::uint64_t k = 0; k < p.nsweeps; ++k) {
+            ucmc_observables o{};
+            obs_from_corr(s, &h[6 * k + 1], o);
+            o.accepted = static_cast<std::uint64_t>(h[6 * k]);
+            series[k] = o;
+        }
+    }
+    if (trace_acc) {
+        UCMC_HIP(hipMemcpyAsync(trace_dE, d_dE, s.sites * 4, hipMemcpyDeviceToHost, s.stream));
+        UCMC_HIP(hipMemcpyAsync(trace_acc, d_acc, s.sites, hipMemcpyDeviceToHost, s.stream));
+    }
+    UCMC_HIP(hipStreamSynchronize(s.stream));
+}
+
+void fpi_observe(System& s, ucmc_observables& o)
+{
+    UCMC_HIP(hipMemsetAsync(s.scratch.p, 0, 40, s.stream));
+    corr_launch(s, s.field[s.cur].as<std::int8_t>(), 1, s.scratch.as<long long>());
+    long long h[5];
+    UCMC_HIP(hipMemcpyAsync(h, s.scratch.p, 40, hipMemcpyDeviceToHost, s.stream));
+    UCMC_HIP(hipStreamSynchronize(s.stream));
+    obs_from_corr(s, h, o);
+}
+
+std::uint64_t fpi_bench(System& s, const ucmc_metropolis_params& p, int repeats, float& ms)
+{
+    std::vector<std::uint32_t> t;
+    fpi_tables(s, p.beta, t);
+    UCMC_HIP(hipMemcpyAsync(s.scratch2.p, t.data(), 100, hipMemcpyHostToDevice, s.stream));
+    hipEvent_t e0, e1;
+    UCMC_HIP(hipEventCreate(&e0));
+    UCMC_HIP(hipEventCreate(&e1));
+    UCMC_HIP(hipStreamSynchronize(s.stream));
+    UCMC_HIP(hipEventRecord(e0, s.stream));
+    for (int r = 0; r < repeats; ++r)
+        for (std::uint64_t k = 0; k < p.nsweeps; ++k)
+            fpi_sweep(s, key_of(p.run_seed, p.sweep0 + r * p.nsweeps + k), s.scratch2.as<std::uint32_t>(), nullptr,
+                      nullptr, nullptr);
+    UCMC_HIP(hipEventRecord(e1, s.stream));
+    UCMC_HIP(hipEventSynchronize(e1));
+    UCMC_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return static_cast<std::uint64_t>(repeats) * p.nsweeps * 4;
+}
+
+// Collect nconfigs configurations every `every` sweeps after `burn`, reduce
+// them to features on the device and keep X (nconfigs x 16 fp32) in s.stamp's
+// sibling buffer flip_out; optionally copy the correlations out.
+void slmc_collect(System& s, double beta, std::uint64_t burn, std::uint64_t nconfigs, std::uint64_t every,
+                  std::uint64_t run_seed, double* corr_out)
+{
+    if (nconfigs == 0 || every == 0) fail(UCMC_E_INVALID_ARGUMENT, "nconfigs and every must be positive");
+    std::vector<std::uint32_t> t;
+    fpi_tables(s, beta, t);
+    UCMC_HIP(hipMemcpyAsync(s.scratch2.p, t.data(), 100, hipMemcpyHostToDevice, s.stream));
+    std::uint64_t sweep = 0;
+    for (; sweep < burn; ++sweep) fpi_sweep(s, key_of(run_seed, sweep), s.scratch2.as<std::uint32_t>(), nullptr, nullptr, nullptr);
+    DevBuf corr;
+    corr.alloc(nconfigs * 5 * 8);
+    // correlations of each kept configuration, straight from the live field
+    for (std::uint64_t c = 0; c < nconfigs; ++c) {
+        for (std::uint64_t k = 0; k < every; ++k, ++sweep)
+            fpi_sweep(s, key_of(run_seed, sweep), s.scratch2.as<std::uint32_t>(), nullptr, nullptr, nullptr);
+        corr_launch(s, s.field[s.cur].as<std::int8_t>(), 1, corr.as<long long>() + 5 * c);
+    }
+    s.flip_out.alloc(nconfigs * 16 * sizeof(float));
+    const auto n = static_cast<std::uint32_t>(nconfigs);
+    hipLaunchKernelGGL(slmc_features, dim3((n + 255) / 256), dim3(256), 0, s.stream, corr.as<long long>(), n,
+                       static_cast<double>(s.sites), s.J, s.K, s.flip_out.as<float>());
+    UCMC_HIP(hipGetLastError());
+    s.n_edges = nconfigs; // number of stored training rows
+    if (corr_out) {
+        std::vector<long long> h(nconfigs * 5);
+        UCMC_HIP(hipMemcpyAsync(h.data(), corr.p, h.size() * 8, hipMemcpyDeviceToHost, s.stream));
+        UCMC_HIP(hipStreamSynchronize(s.stream));
+        for (std::size_t k = 0; k < h.size(); ++k) corr_out[k] = static_cast<double>(h[k]);
+    }
+    UCMC_HIP(hipStreamSynchronize(s.stream));
+}
+
+void slmc_gram(System& s, float* G_out, double* theta_out)
+{
+    if (!s.flip_out.p || s.n_edges == 0) fail(UCMC_E_INVALID_ARGUMENT, "no training data collected");
+    DevBuf G, th;
+    G.alloc(256 * 4);
+    th.alloc(4 * 8);
+    UCMC_HIP(hipMemsetAsync(G.p, 0, 256 * 4, s.stream));
+    const auto n = static_cast<std::uint32_t>(s.n_edges);
+    const std::uint32_t chunks = (n + 31) / 32;
+    const std::uint32_t blocks = std::max<std::uint32_t>(1, std::min<std::uint32_t>((chunks + 3) / 4, 256));
+    hipLaunchKernelGGL(slmc_gram_mfma, dim3(blocks), dim3(256), 0, s.stream, s.flip_out.as<float>(), n, G.as<float>());
+    hipLaunchKernelGGL(slmc_solve, dim3(1), dim3(64), 0, s.stream, G.as<float>(), th.as<double>());
+    UCMC_HIP(hipGetLastError());
+    if (G_out) UCMC_HIP(hipMemcpyAsync(G_out, G.p, 256 * 4, hipMemcpyDeviceToHost, s.stream));
+    if (theta_out) UCMC_HIP(hipMemcpyAsync(theta_out, th.p, 32, hipMemcpyDeviceToHost, s.stream));
+    UCMC_HIP(hipStreamSynchronize(s.stream));
+}
+
+void slmc_loss_grad_host(System& s, const double* theta, double* out5)
+{
+    if (!s.flip_out.p || s.n_edges == 0) fail(UCMC_E_INVALID_ARGUMENT, "no training data collected");
+    DevBuf th, o;
+    th.alloc(32);
+    o.alloc(40);
+    UCMC_HIP(hipMemcpyAsync(th.p, theta, 32, hipMemcpyHostToDevice, s.stream));
+    UCMC_HIP(hipMemsetAsync(o.p, 0, 40, s.stream));
+    const auto n = static_cast<std::uint32_t>(s.n_edges);
+    hipLaunchKernelGGL(slmc_loss_grad, dim3(std::min<std::uint32_t>((n + 255) / 256, 256)), dim3(256), 0, s.stream,
+                       s.flip_out.as<float>(), n, th.as<double>(), o.as<double>());
+    UCMC_HIP(hipGetLastError());
+    UCMC_HIP(hipMemcpyAsync(out5, o.p, 40, hipMemcpyDeviceToHost, s.stream));
+    UCMC_HIP(hipStreamSynchronize(s.stream));
+}
+
+void slmc_adam_fit(System& s, int iters, double lr, double* theta, double* loss)
+{
+    if (!s.flip_out.p || s.n_edges == 0) fail(UCMC_E_INVALID_ARGUMENT, "no training data collected");
+    DevBuf th, m, v, o;
+    th.alloc(32);
+    m.alloc(32);
+    v.alloc(32);
+    o.alloc(40);
+    UCMC_HIP(hipMemcpyAsync(th.p, theta, 32, hipMemcpyHostToDevice, s.stream));
+    UCMC_HIP(hipMemsetAsync(m.p, 0, 32, s.stream));
+    UCMC_HIP(hipMemsetAsync(v.p, 0, 32, s.stream));
+    const auto n = static_cast<std::uint32_t>(s.n_edges);
+    const dim3 grid(std::min<std::uint32_t>((n + 255) / 256, 256));
+    for (int t = 1; t <= iters; ++t) {
+        UCMC_HIP(hipMemsetAsync(o.p, 0, 40, s.stream));
+        hipLaunchKernelGGL(slmc_loss_grad, grid, dim3(256), 0, s.stream, s.flip_out.as<float>(), n, th.as<double>(),
+                           o.as<double>());
+        hipLaunchKernelGGL(slmc_adam, dim3(1), dim3(64), 0, s.stream, th.as<double>(), m.as<double>(), v.as<double>(),
+                           o.as<double>(), n, t, lr, 0.9, 0.999, 1e-12);
+    }
+    UCMC_HIP(hipMemsetAsync(o.p, 0, 40, s.stream));
+    hipLaunchKernelGGL(slmc_loss_grad, grid, dim3(256), 0, s.stream, s.flip_out.as<float>(), n, th.as<double>(),
+                       o.as<double>());
+    UCMC_HIP(hipGetLastError());
+    double h[5];
+    UCMC_HIP(hipMemcpyAsync(theta, th.p, 32, hipMemcpyDeviceToHost, s.stream));
+    UCMC_HIP(hipMemcpyAsync(h, o.p, 40, hipMemcpyDeviceToHost, s.stream));
+    UCMC_HIP(hipStreamSynchronize(s.stream));
+    if (loss) *loss = h[0] / n;
+}
+
+void slmc_run(System& s, const double* Jeff, double beta, std::uint64_t run_seed, std::uint64_t flip0,
+              std::uint64_t nflips, ucmc_observables* series, std::uint8_t* accepted)
+{
+    if (!(beta == beta)) fail(UCMC_E_INVALID_ARGUMENT, "beta is NaN");
+    if (nflips == 0) return;
+    if (!s.stamp.p) {
+        s.stamp.alloc(s.sites * 4);
+        UCMC_HIP(hipMemsetAsync(s.stamp.p, 0, s.sites * 4, s.stream));
+        s.queue.alloc(s.sites * 4);
+        s.wolff_mark = 0;
+    }
+    if (static_cast<std::uint64_t>(s.wolff_mark) + nflips >= 0xFFFFFFFFull) {
+        UCMC_HIP(hipMemsetAsync(s.stamp.p, 0, s.sites * 4, s.stream));
+        s.wolff_mark = 0;
+    }
+    ucmc_observables start{};
+    if (series) fpi_observe(s, start);
+    DevBuf out;
+    out.alloc(nflips * 6 * 8);
+    SlmcArgs a{};
+    a.s = s.field[s.cur].as<std::int8_t>();
+    a.L = static_cast<std::uint32_t>(s.L);
+    a.stamp = s.stamp.as<std::uint32_t>();
+    a.queue = s.queue.as<std::uint32_t>();
+    a.out = out.as<double>();
+    a.run_seed = run_seed;
+    a.flip0 = flip0;
+    a.nflips = static_cast<std::uint32_t>(nflips);
+    a.mark0 = s.wolff_mark + 1;
+    a.beta = beta;
+    a.J1 = s.J;
+    a.K = s.K;
+    for (int k = 0; k < 3; ++k) {
+        a.Jeff[k] = Jeff[k];
+        // driver.hpp:195 with the bond dE 2 J_k s_i s_j
+        a.thr[k][1] = wolff_threshold(beta, 2.0 * Jeff[k]);
+        a.thr[k][0] = wolff_threshold(beta, -2.0 * Jeff[k]);
+    }
+    hipLaunchKernelGGL(slmc_flips, dim3(1), dim3(1024), 0, s.stream, a);
+    UCMC_HIP(hipGetLastError());
+    s.wolff_mark += static_cast<std::uint32_t>(nflips);
+    std::vector<double> h(nflips * 6);
+    UCMC_HIP(hipMemcpyAsync(h.data(), out.p, h.size() * 8, hipMemcpyDeviceToHost, s.stream));
+    UCMC_HIP(hipStreamSynchronize(s.stream));
+    double e = start.energy, m = start.order[0];
+    for (std::uint64_t f = 0; f < nflips; ++f) {
+        const bool acc = h[6 * f + 1] != 0.0;
+        if (acc) {
+            e += h[6 * f + 2];
+            m += h[6 * f + 4];
+        }
+        if (accepted) accepted[f] = acc ? 1 : 0;
+        if (series) {
+            ucmc_observables o{};
+            o.energy = e;
+            o.order[0] = m;
+            o.cluster = static_cast<std::uint64_t>(h[6 * f]);
+            o.accepted = acc ? 1 : 0;
+            series[f] = o;
+        }
+    }
+}
+
+}  // namespace ucmc

@@ -88,6 +88,16 @@ struct System {
     // stamp == mark_f, so no per-flip clear), the BFS queue, per-flip outputs
     DevBuf stamp, queue, flip_out;
     std::uint32_t wolff_mark = 0;
+    // parallel tempering batch (XY replicas)
+    struct Pt {
+        std::vector<std::uint32_t> L, ladder, tile_rep_h, ladder_start_h;
+        std::vector<std::uint64_t> offset;
+        std::vector<double> betas;
+        std::uint32_t tile = 0;       // chosen tile edge
+        std::uint32_t ntiles = 0;
+        DevBuf reps, tile_rep, betas_d, temp_of, rep_at, ladder_start, energy, accepts, swap_acc,
+            swap_try, done, sweep_ctr;
+    } pt;
 
     ~System();
     std::size_t device_bytes() const;
@@ -108,6 +118,37 @@ void vector_metropolis(System& s, const ucmc_metropolis_params& p, ucmc_observab
                        std::uint8_t* trace_acc, float* trace_dE);
 void vector_observe(System& s, ucmc_observables& out);
 std::uint64_t vector_bench(System& s, const ucmc_metropolis_params& p, int repeats, float& ms);
+void gauge_setup(System& s);
+void gauge_metropolis(System& s, const ucmc_metropolis_params& p, ucmc_observables* series,
+                      std::uint8_t* trace_acc, float* trace_dE);
+void gauge_observe(System& s, ucmc_observables& o);
+std::uint64_t gauge_bench(System& s, const ucmc_metropolis_params& p, int repeats, float& ms);
+void gauge_reunitarize(System& s);
+void gauge_upload(System& s, const double* host);
+void gauge_download(System& s, double* host);
+void xy_setup(System& s);
+void xy_metropolis(System& s, const ucmc_metropolis_params& p, ucmc_observables* series,
+                   std::uint8_t* trace_acc, float* trace_dE);
+void xy_observe(System& s, ucmc_observables& o);
+std::uint64_t xy_bench(System& s, const ucmc_metropolis_params& p, int repeats, float& ms);
+void pt_setup(System& s, std::size_t R, const std::size_t* L, const std::uint32_t* ladder,
+              const double* betas);
+void pt_run(System& s, const ucmc_pt_params& p, double* series, std::uint32_t* temp_of);
+void pt_stats(const System& s, std::uint64_t* acc, std::uint64_t* sw_acc, std::uint64_t* sw_try);
+void pt_autotune(System& s, double width, std::uint32_t* tile, double* best_ms);
+void fpi_setup(System& s);
+void fpi_metropolis(System& s, const ucmc_metropolis_params& p, ucmc_observables* series,
+                    std::uint8_t* trace_acc, float* trace_dE);
+void fpi_observe(System& s, ucmc_observables& o);
+std::uint64_t fpi_bench(System& s, const ucmc_metropolis_params& p, int repeats, float& ms);
+void slmc_collect(System& s, double beta, std::uint64_t burn, std::uint64_t nconfigs,
+                  std::uint64_t every, std::uint64_t run_seed, double* corr_out);
+void slmc_gram(System& s, float* G_out, double* theta_out);
+void slmc_loss_grad_host(System& s, const double* theta, double* out5);
+void slmc_adam_fit(System& s, int iters, double lr, double* theta, double* loss);
+void slmc_run(System& s, const double* Jeff, double beta, std::uint64_t run_seed,
+              std::uint64_t flip0, std::uint64_t nflips, ucmc_observables* series,
+              std::uint8_t* accepted);
 void square_csr(System& s);  // materialise square_lattice(Lx, Ly) for a dims-created system
 void wolff_run(System& s, const ucmc_wolff_params& p, std::uint64_t* sizes,
                ucmc_observables* series);

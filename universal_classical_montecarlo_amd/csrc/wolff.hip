@@ -47,6 +47,7 @@ struct WolffArgs {
     std::uint64_t flip0;
     std::uint32_t nflips;
     std::uint32_t mark0;           // mark of flip f is mark0 + f
+    std::uint64_t touch_target;    // > 0: stop once the cluster sizes add up to this
     double beta;
     double J;
     std::uint64_t thr_aligned;     // Ising: accepted draws when s_i s_j = +1
@@ -104,6 +105,9 @@ __global__ __launch_bounds__(kThreads) void wolff_flips(WolffArgs a)
     __shared__ std::uint32_t s_head, s_tail, s_seed;
     __shared__ double s_r[4];
     __shared__ double s_red[1 + 4];
+    __shared__ unsigned long long s_touched;
+    if (threadIdx.x == 0) s_touched = 0;
+    __syncthreads();
 
     const std::uint32_t tid = threadIdx.x, lane = tid & 63u;
     Field<N> fld;
@@ -114,6 +118,7 @@ __global__ __launch_bounds__(kThreads) void wolff_flips(WolffArgs a)
     }
 
     for (std::uint32_t fi = 0; fi < a.nflips; ++fi) {
+        if (a.touch_target && s_touched >= a.touch_target) break;  // uniform: read after a barrier
         const std::uint32_t mark = a.mark0 + fi;
         const std::uint64_t key = key_of(a.run_seed, kWolffCounterBase + a.flip0 + fi);
         if (tid == 0) {
@@ -246,6 +251,7 @@ __global__ __launch_bounds__(kThreads) void wolff_flips(WolffArgs a)
             }
         }
         if (tid == 0) {
+            s_touched += size;
             double* o = a.out + 6 * static_cast<std::size_t>(fi);
             o[0] = static_cast<double>(size);
             o[1] = s_red[0];
@@ -283,6 +289,7 @@ void wolff_run(System& s, const ucmc_wolff_params& p, std::uint64_t* sizes, ucmc
     }
     DevBuf out;
     out.alloc(p.nflips * 6 * sizeof(double));
+    UCMC_HIP(hipMemsetAsync(out.p, 0, out.bytes, s.stream));
     WolffArgs a{};
     a.field = s.field[s.cur].p;
     a.n = static_cast<std::uint32_t>(s.sites);
@@ -296,6 +303,7 @@ void wolff_run(System& s, const ucmc_wolff_params& p, std::uint64_t* sizes, ucmc
     a.flip0 = p.flip0;
     a.nflips = static_cast<std::uint32_t>(p.nflips);
     a.mark0 = s.wolff_mark + 1;
+    a.touch_target = p.touch_target;
     a.beta = p.beta;
     if (s.kind == UCMC_ISING) {
         a.J = static_cast<double>(s.Ji);

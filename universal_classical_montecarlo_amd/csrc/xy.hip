@@ -1,0 +1,657 @@
+// xy.hip -- XY spins stored as angles, E = -J sum_<ij> cos(t_i - t_j), updated
+// with the U1Shift proposal (dof/u1.hpp:35-50) under Metropolis
+// (driver.hpp:73-87); the reference stores XY as a 2-vector
+// (example/vector_spin.hpp:82), so the energy difference follows its form,
+// -J sum_j [cos(p - t_j) - cos(t_i - t_j)] in row order.
+//
+// Two paths:
+//  * a single system on any PairGraph: one launch per colour class;
+//  * a batch of replicas (parallel tempering): every replica is a periodic
+//    L x L square lattice; ONE launch per sweep updates both checkerboard
+//    colours of every replica tile by tile in LDS (tile + 3-site halo, colour 0
+//    on the tile and two rings, colour 1 on the tile and one ring, so the final
+//    values of the tile and its first ring give the tile's bond energies), and
+//    the last workgroup to finish performs the replica-exchange step for all
+//    ladders (no grid barrier).  Fields ping-pong between two buffers, so halos
+//    are always read unmodified; the per-site Pcg32 stream makes the
+//    redundant halo updates bit-identical to the owner's.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "ucmc_core.hpp"
+#include "ucmc_device.hpp"
+
+namespace ucmc {
+
+namespace {
+
+constexpr float kTwoPiF = 6.283185307179586f;
+
+__device__ __forceinline__ float uf(std::uint32_t k) { return static_cast<float>(k) * 0x1.0p-32f; }
+
+// dof/u1.hpp:40-46 in fp32
+__device__ __forceinline__ float u1_shift(float width, float cur, std::uint32_t k)
+{
+    return remainderf(cur + width * (2.0f * uf(k) - 1.0f), kTwoPiF);
+}
+
+// ---------------------------------------------------- single-system path --
+struct XyArgs {
+    float* t;
+    const std::uint32_t* offsets;
+    const std::uint32_t* bonds;
+    const std::uint32_t* sites;
+    std::uint32_t count;
+    std::uint64_t key;
+    float J, width;
+    double beta;
+    unsigned long long* accepts;
+    float* trace_dE;
+    std::uint8_t* trace_acc;
+};
+
+template <bool TRACE>
+__global__ __launch_bounds__(256) void xy_colour(XyArgs a)
+{
+    const std::uint32_t tt = blockIdx.x * blockDim.x + threadIdx.x;
+    bool acc = false;
+    if (tt < a.count) {
+        const std::uint32_t i = a.sites[tt];
+        const float ti = a.t[i];
+        Pcg r;
+        r.seed(a.key, i);
+        const float tp = u1_shift(a.width, ti, r.next());
+        float d = 0.f;
+        for (std::uint32_t q = a.offsets[i]; q < a.offsets[i + 1]; ++q) {
+            const float tj = a.t[a.bonds[q]];
+            d += cosf(tp - tj) - cosf(ti - tj);
+        }
+        const float dE = -a.J * d;
+        const double log_p = -a.beta * static_cast<double>(dE) + 0.0;
+        acc = log_p >= 0.0 || log(r.uni()) < log_p;
+        if (acc) a.t[i] = tp;
+        if constexpr (TRACE) {
+            a.trace_dE[i] = dE;
+            a.trace_acc[i] = acc ? 1 : 0;
+        }
+    }
+    if (a.accepts) {
+        const unsigned long long c = __popcll(__ballot(acc));
+        if ((threadIdx.x & 63) == 0 && c) atomicAdd(a.accepts, c);
+    }
+}
+
+__global__ __launch_bounds__(256) void xy_reduce(const float* t, std::uint32_t n, const std::uint32_t* off,
+                                                 const std::uint32_t* bonds, double* out)
+{
+    double e = 0.0, mx = 0.0, my = 0.0;
+    for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double d = 0.0;
+        for (std::uint32_t q = off[i]; q < off[i + 1]; ++q) d += cos(static_cast<double>(t[i]) - t[bonds[q]]);
+        e += 0.5 * d;
+        mx += cos(static_cast<double>(t[i]));
+        my += sin(static_cast<double>(t[i]));
+    }
+    e = wave_sum(e);
+    mx = wave_sum(mx);
+    my = wave_sum(my);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(out, e);
+        atomicAdd(out + 1, mx);
+        atomicAdd(out + 2, my);
+    }
+}
+
+// ------------------------------------------------------------ PT batch ---
+constexpr int kHalo = 3;
+constexpr int kMaxTile = 64;
+
+struct PtReplica {
+    std::uint64_t offset;    // first site in the batch buffers
+    std::uint32_t L;
+    std::uint32_t tile;      // tile edge (divides L)
+    std::uint32_t first_tile;
+    std::uint32_t ladder;    // ladder id (replicas that may exchange)
+    std::uint64_t run_seed;  // per-replica stream key base
+};
+
+struct PtArgs {
+    const float* in;
+    float* out;
+    const PtReplica* reps;
+    std::uint32_t nreps;
+    const std::uint32_t* tile_rep;   // tile -> replica
+    std::uint32_t ntiles;
+    const double* betas;             // temperature slots
+    std::uint32_t* temp_of;          // replica -> temperature slot
+    std::uint32_t* rep_at;           // ladder-major: slot -> replica
+    const std::uint32_t* ladder_start; // ladders + 1 offsets into rep_at (slot ranges)
+    std::uint32_t nladders;
+    double* energy;                  // per replica, reset by the finisher
+    unsigned long long* accepts;     // per replica, cumulative
+    unsigned long long* swap_acc;    // per slot pair (ladder-major), cumulative
+    unsigned long long* swap_try;
+    std::uint32_t* done;             // finished-block counter
+    unsigned long long* sweep_ctr;   // device sweep counter (graph-replayable)
+    double* energy_series;           // [sweep][replica] or null
+    std::uint32_t series_cap;
+    std::uint64_t sweep_base;        // series index origin
+    float J, width;
+    std::uint64_t swap_seed;
+};
+
+__device__ __forceinline__ float xy_dE(float J, float tp, float ti, const float* nb)
+{
+    float d = 0.f;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) d += cosf(tp - nb[q]) - cosf(ti - nb[q]);
+    return -J * d;
+}
+
+__global__ __launch_bounds__(256) void pt_sweep(PtArgs a)
+{
+    __shared__ float tl[(kMaxTile + 2 * kHalo) * (kMaxTile + 2 * kHalo)];
+    __shared__ double s_e;
+    __shared__ unsigned long long s_acc;
+    __shared__ bool s_last;
+    const std::uint32_t tile = blockIdx.x;
+    const std::uint32_t r = a.tile_rep[tile];
+    const PtReplica rp = a.reps[r];
+    const std::uint32_t L = rp.L, T = rp.tile, W = T + 2 * kHalo;
+    const std::uint32_t per_row = L / T;
+    const std::uint32_t local = tile - rp.first_tile;
+    const std::uint32_t ty0 = (local / per_row) * T, tx0 = (local % per_row) * T;
+    const std::uint64_t sweep = *a.sweep_ctr;
+    const std::uint64_t key = key_of(rp.run_seed, sweep);
+    const double beta = a.betas[a.temp_of[r]];
+    const float* in = a.in + rp.offset;
+    float* out = a.out + rp.offset;
+    if (threadIdx.x == 0) {
+        s_e = 0.0;
+        s_acc = 0;
+    }
+    // load tile + halo (periodic)
+    for (std::uint32_t q = threadIdx.x; q < W * W; q += blockDim.x) {
+        const std::uint32_t ly = q / W, lx = q % W;
+        const std::uint32_t y = (ty0 + L - kHalo + ly) % L, x = (tx0 + L - kHalo + lx) % L;
+        tl[q] = in[static_cast<std::size_t>(y) * L + x];
+    }
+    __syncthreads();
+    // colour c over the window [lo, hi) of local coordinates
+    auto pass = [&](std::uint32_t colour, std::uint32_t lo, std::uint32_t hi) {
+        const std::uint32_t w = hi - lo;
+        for (std::uint32_t q = threadIdx.x; q < w * w; q += blockDim.x) {
+            const std::uint32_t ly = lo + q / w, lx = lo + q % w;
+            const std::uint32_t y = (ty0 + L - kHalo + ly) % L, x = (tx0 + L - kHalo + lx) % L;
+            if (((x + y) & 1u) != colour) continue;
+            const float ti = tl[ly * W + lx];
+            const float nb[4] = {tl[ly * W + lx + 1], tl[ly * W + lx - 1], tl[(ly + 1) * W + lx],
+                                 tl[(ly - 1) * W + lx]};
+            Pcg rg;
+            rg.seed(key, static_cast<std::uint64_t>(y) * L + x);
+            const float tp = u1_shift(a.width, ti, rg.next());
+            const float dE = xy_dE(a.J, tp, ti, nb);
+            const double log_p = -beta * static_cast<double>(dE) + 0.0;
+            if (log_p >= 0.0 || log(rg.uni()) < log_p) tl[ly * W + lx] = tp;
+        }
+    };
+    pass(0, 1, W - 1);  // colour 0 on tile + 2 rings
+    __syncthreads();
+    pass(1, 2, W - 2);  // colour 1 on tile + 1 ring
+    __syncthreads();
+    // write the tile; count its changed sites; bonds to the right and down
+    // neighbours (both final within tile + 1 ring)
+    unsigned long long nacc = 0;
+    double e = 0.0;
+    for (std::uint32_t q = threadIdx.x; q < T * T; q += blockDim.x) {
+        const std::uint32_t ly = kHalo + q / T, lx = kHalo + q % T;
+        const std::uint32_t y = ty0 + q / T, x = tx0 + q % T;
+        const float v = tl[ly * W + lx];
+        out[static_cast<std::size_t>(y) * L + x] = v;
+        nacc += (v != in[static_cast<std::size_t>(y) * L + x]) ? 1u : 0u;
+        e += cos(static_cast<double>(v) - tl[ly * W + lx + 1]) + cos(static_cast<double>(v) - tl[(ly + 1) * W + lx]);
+    }
+    e = wave_sum(e);
+    nacc = wave_sum_u64(nacc);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&s_e, e);
+        atomicAdd(&s_acc, nacc);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAdd(&a.energy[r], -static_cast<double>(a.J) * s_e);
+        atomicAdd(&a.accepts[r], s_acc);
+        __threadfence();
+        s_last = atomicAdd(a.done, 1u) == a.ntiles - 1;
+    }
+    __syncthreads();
+    if (!s_last) return;
+
+    // ---- the last block: record energies, then replica exchange on every
+    // ladder, pairs (k, k+1) with k of the sweep's parity ----
+    __threadfence();
+    for (std::uint32_t q = threadIdx.x; q < a.nreps; q += blockDim.x) {
+        const double ev = __hip_atomic_load(&a.energy[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (a.energy_series && sweep >= a.sweep_base && sweep - a.sweep_base < a.series_cap)
+            a.energy_series[(sweep - a.sweep_base) * a.nreps + q] = ev;
+    }
+    __syncthreads();
+    const std::uint64_t pkey = key_of(a.swap_seed, sweep);
+    for (std::uint32_t ld = threadIdx.x; ld < a.nladders; ld += blockDim.x) {
+        const std::uint32_t s0 = a.ladder_start[ld], s1 = a.ladder_start[ld + 1];
+        for (std::uint32_t k = s0 + static_cast<std::uint32_t>(sweep & 1u); k + 1 < s1; k += 2) {
+            const std::uint32_t ra = a.rep_at[k], rb = a.rep_at[k + 1];
+            const double ea = __hip_atomic_load(&a.energy[ra], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const double eb = __hip_atomic_load(&a.energy[rb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // min(1, exp((beta_k - beta_k+1)(E_a - E_b)))
+            const double log_p = (a.betas[k] - a.betas[k + 1]) * (ea - eb);
+            Pcg sw;
+            sw.seed(pkey, k);
+            const bool swap = log_p >= 0.0 || log(sw.uni()) < log_p;
+            a.swap_try[k] += 1;
+            if (swap) {
+                a.rep_at[k] = rb;
+                a.rep_at[k + 1] = ra;
+                a.temp_of[ra] = k + 1;
+                a.temp_of[rb] = k;
+                a.swap_acc[k] += 1;
+            }
+        }
+    }
+    __syncthreads();
+    for (std::uint32_t q = threadIdx.x; q < a.nreps; q += blockDim.x) a.energy[q] = 0.0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        *a.done = 0;
+        *a.sweep_ctr = sweep + 1;
+    }
+    __threadfence();
+}
+
+// the ring / halo width needs tile + 2 kHalo <= kMaxTile + 2 kHalo
+bool tile_ok(std::uint32_t L, std::uint32_t T) { return T >= 2 && T <= kMaxTile && L % T == 0 && T % 2 == 0; }
+
+}  // namespace
+}  // namespace ucmc
+
+namespace ucmc {
+
+namespace {
+
+void xy_sweep(System& s, const ucmc_metropolis_params& p, std::uint64_t key, unsigned long long* dacc,
+              float* tdE, std::uint8_t* tacc)
+{
+    for (int c = 0; c < s.ncolours; ++c) {
+        XyArgs a{};
+        a.t = s.field[s.cur].as<float>();
+        a.offsets = s.offsets.as<std::uint32_t>();
+        a.bonds = s.bonds.as<std::uint32_t>();
+        a.sites = s.colour_sites.as<std::uint32_t>() + s.colour_start[c];
+        a.count = static_cast<std::uint32_t>(s.colour_start[c + 1] - s.colour_start[c]);
+        a.key = key;
+        a.J = static_cast<float>(s.J);
+        a.width = static_cast<float>(p.width);
+        a.beta = p.beta;
+        a.accepts = dacc;
+        a.trace_dE = tdE;
+        a.trace_acc = tacc;
+        if (!a.count) continue;
+        const dim3 grid((a.count + 255) / 256);
+        if (tdE) hipLaunchKernelGGL(xy_colour<true>, grid, dim3(256), 0, s.stream, a);
+        else hipLaunchKernelGGL(xy_colour<false>, grid, dim3(256), 0, s.stream, a);
+        UCMC_HIP(hipGetLastError());
+    }
+}
+
+void xy_reduce_launch(System& s, double* out)
+{
+    const unsigned blocks = static_cast<unsigned>(std::min<std::size_t>((s.sites + 255) / 256, 1024));
+    hipLaunchKernelGGL(xy_reduce, dim3(blocks), dim3(256), 0, s.stream, s.field[s.cur].as<float>(),
+                       static_cast<std::uint32_t>(s.sites), s.offsets.as<std::uint32_t>(),
+                       s.bonds.as<std::uint32_t>(), out);
+    UCMC_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void xy_setup(System& s)
+{
+    std::size_t Lx = 0, Ly = 0;
+    if (detect_square(s.h_offsets.data(), s.h_bonds.data(), s.sites, Lx, Ly) && Lx % 2 == 0 && Ly % 2 == 0) {
+        s.geom = Geom::Square;
+        s.Lx = Lx;
+        s.Ly = Ly;
+        s.ncolours = 2;
+        s.colour.resize(s.sites);
+        for (std::size_t i = 0; i < s.sites; ++i) s.colour[i] = static_cast<std::uint8_t>(((i % Lx) + (i / Lx)) & 1u);
+        s.schedule = UCMC_SCHED_CHECKERBOARD;
+    } else {
+        s.geom = Geom::General;
+        s.ncolours = greedy_colouring(s.h_offsets.data(), s.h_bonds.data(), s.sites, s.colour);
+        s.schedule = UCMC_SCHED_CSR;
+    }
+    upload_colour_lists(s);
+    s.field[0].alloc(s.sites * sizeof(float));
+    UCMC_HIP(hipMemset(s.field[0].p, 0, s.sites * sizeof(float)));
+    s.scratch.alloc(64);
+}
+
+void xy_metropolis(System& s, const ucmc_metropolis_params& p, ucmc_observables* series,
+                   std::uint8_t* trace_acc, float* trace_dE)
+{
+    if (p.proposal != UCMC_PROP_U1_SHIFT) fail(UCMC_E_UNSUPPORTED, "XY angles take U1Shift");
+    if (p.hits != 1) fail(UCMC_E_UNSUPPORTED, "one hit per site");
+    const bool rec = series && (p.flags & (UCMC_RECORD_ENERGY | UCMC_RECORD_ORDER | UCMC_RECORD_ACCEPTS));
+    DevBuf rb;
+    if (rec) {
+        rb.alloc(p.nsweeps * 4 * 8);
+        UCMC_HIP(hipMemsetAsync(rb.p, 0, rb.bytes, s.stream));
+    }
+    DevBuf tb;
+    float* d_dE = nullptr;
+    std::uint8_t* d_acc = nullptr;
+    if (trace_acc) {
+        tb.alloc(s.sites * 5);
+        d_dE = tb.as<float>();
+        d_acc = reinterpret_cast<std::uint8_t*>(d_dE + s.sites);
+    }
+    for (std::uint64_t k = 0; k < p.nsweeps; ++k) {
+        double* slot = rec ? rb.as<double>() + 4 * k : nullptr;
+        xy_sweep(s, p, key_of(p.run_seed, p.sweep0 + k),
+                 (rec && (p.flags & UCMC_RECORD_ACCEPTS)) ? reinterpret_cast<unsigned long long*>(slot) : nullptr,
+                 d_dE, d_acc);
+        if (rec && (p.flags & (UCMC_RECORD_ENERGY | UCMC_RECORD_ORDER))) xy_reduce_launch(s, slot + 1);
+    }
+    if (rec) {
+        std::vector<double> h(p.nsweeps * 4);
+        UCMC_HIP(hipMemcpyAsync(h.data(), rb.p, h.size() * 8, hipMemcpyDeviceToHost, s.stream));
+        UCMC_HIP(hipStreamSynchronize(s.stream));
+        for (std::uint64_t k = 0; k < p.nsweeps; ++k) {
+            ucmc_observables o{};
+            std::memcpy(&o.accepted, &h[4 * k], 8);
+            o.energy = -s.J * h[4 * k + 1];
+            o.order[0] = h[4 * k + 2];
+            o.order[1] = h[4 * k + 3];
+            series[k] = o;
+        }
+    }
+    if (trace_acc) {
+        UCMC_HIP(hipMemcpyAsync(trace_dE, d_dE, s.sites * 4, hipMemcpyDeviceToHost, s.stream));
+        UCMC_HIP(hipMemcpyAsync(trace_acc, d_acc, s.sites, hipMemcpyDeviceToHost, s.stream));
+    }
+    UCMC_HIP(hipStreamSynchronize(s.stream));
+}
+
+void xy_observe(System& s, ucmc_observables& o)
+{
+    UCMC_HIP(hipMemsetAsync(s.scratch.p, 0, 24, s.stream));
+    xy_reduce_launch(s, s.scratch.as<double>());
+    double h[3];
+    UCMC_HIP(hipMemcpyAsync(h, s.scratch.p, 24, hipMemcpyDeviceToHost, s.stream));
+    UCMC_HIP(hipStreamSynchronize(s.stream));
+    o.energy = -s.J * h[0];
+    o.order[0] = h[1];
+    o.order[1] = h[2];
+}
+
+std::uint64_t xy_bench(System& s, const ucmc_metropolis_params& p, int repeats, float& ms)
+{
+    if (p.proposal != UCMC_PROP_U1_SHIFT) fail(UCMC_E_UNSUPPORTED, "XY angles take U1Shift");
+    hipEvent_t e0, e1;
+    UCMC_HIP(hipEventCreate(&e0));
+    UCMC_HIP(hipEventCreate(&e1));
+    UCMC_HIP(hipStreamSynchronize(s.stream));
+    UCMC_HIP(hipEventRecord(e0, s.stream));
+    std::uint64_t n = 0;
+    for (int r = 0; r < repeats; ++r)
+        for (std::uint64_t k = 0; k < p.nsweeps; ++k) {
+            xy_sweep(s, p, key_of(p.run_seed, p.sweep0 + r * p.nsweeps + k), nullptr, nullptr, nullptr);
+            n += s.ncolours;
+        }
+    UCMC_HIP(hipEventRecord(e1, s.stream));
+    UCMC_HIP(hipEventSynchronize(e1));
+    UCMC_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return n;
+}
+
+// ------------------------------------------------------------------ PT --
+
+namespace {
+
+constexpr std::uint32_t kDefaultTile = 32;
+
+void pt_layout(System& s, std::uint32_t tile, std::uint64_t run_seed)
+{
+    auto& pt = s.pt;
+    const std::size_t R = pt.L.size();
+    std::vector<PtReplica> reps(R);
+    pt.tile_rep_h.clear();
+    for (std::size_t r = 0; r < R; ++r) {
+        const std::uint32_t L = pt.L[r];
+        std::uint32_t T = std::min(tile, L);
+        while (!tile_ok(L, T)) T /= 2; // L is even, so T = 2 always fits
+        reps[r].offset = pt.offset[r];
+        reps[r].L = L;
+        reps[r].tile = T;
+        reps[r].first_tile = static_cast<std::uint32_t>(pt.tile_rep_h.size());
+        reps[r].ladder = pt.ladder[r];
+        reps[r].run_seed = key_of(run_seed, (1ull << 61) + r);
+        const std::uint32_t nt = (L / T) * (L / T);
+        for (std::uint32_t q = 0; q < nt; ++q) pt.tile_rep_h.push_back(static_cast<std::uint32_t>(r));
+    }
+    pt.ntiles = static_cast<std::uint32_t>(pt.tile_rep_h.size());
+    pt.tile = tile;
+    if (pt.reps.bytes < R * sizeof(PtReplica)) pt.reps.alloc(R * sizeof(PtReplica));
+    UCMC_HIP(hipMemcpy(pt.reps.p, reps.data(), R * sizeof(PtReplica), hipMemcpyHostToDevice));
+    if (pt.tile_rep.bytes < pt.tile_rep_h.size() * 4) pt.tile_rep.alloc(pt.tile_rep_h.size() * 4);
+    UCMC_HIP(hipMemcpy(pt.tile_rep.p, pt.tile_rep_h.data(), pt.tile_rep_h.size() * 4, hipMemcpyHostToDevice));
+}
+
+PtArgs pt_args(System& s, const ucmc_pt_params& p, double* series_d, std::uint32_t cap)
+{
+    auto& pt = s.pt;
+    PtArgs a{};
+    a.reps = pt.reps.as<PtReplica>();
+    a.nreps = static_cast<std::uint32_t>(pt.L.size());
+    a.tile_rep = pt.tile_rep.as<std::uint32_t>();
+    a.ntiles = pt.ntiles;
+    a.betas = pt.betas_d.as<double>();
+    a.temp_of = pt.temp_of.as<std::uint32_t>();
+    a.rep_at = pt.rep_at.as<std::uint32_t>();
+    a.ladder_start = pt.ladder_start.as<std::uint32_t>();
+    a.nladders = static_cast<std::uint32_t>(pt.ladder_start_h.size() - 1);
+    a.energy = pt.energy.as<double>();
+    a.accepts = pt.accepts.as<unsigned long long>();
+    a.swap_acc = pt.swap_acc.as<unsigned long long>();
+    a.swap_try = pt.swap_try.as<unsigned long long>();
+    a.done = pt.done.as<std::uint32_t>();
+    a.sweep_ctr = pt.sweep_ctr.as<unsigned long long>();
+    a.energy_series = series_d;
+    a.series_cap = cap;
+    a.sweep_base = p.sweep0;
+    a.J = static_cast<float>(s.J);
+    a.width = static_cast<float>(p.width);
+    a.swap_seed = key_of(p.run_seed, 1ull << 60);
+    return a;
+}
+
+// Launches one sweep; the ping-pong buffer index is a host-side property, so
+// a captured graph holds an even number of sweeps.
+void pt_launch(System& s, PtArgs a)
+{
+    a.in = s.field[s.cur].as<float>();
+    a.out = s.field[s.cur ^ 1].as<float>();
+    hipLaunchKernelGGL(pt_sweep, dim3(s.pt.ntiles), dim3(256), 0, s.stream, a);
+    UCMC_HIP(hipGetLastError());
+    s.cur ^= 1;
+}
+
+}  // namespace
+
+void pt_setup(System& s, std::size_t R, const std::size_t* L, const std::uint32_t* ladder, const double* betas)
+{
+    if (R == 0 || !L || !ladder || !betas) fail(UCMC_E_INVALID_ARGUMENT, "empty replica batch");
+    auto& pt = s.pt;
+    pt.L.resize(R);
+    pt.ladder.assign(ladder, ladder + R);
+    pt.offset.resize(R);
+    pt.betas.assign(betas, betas + R);
+    std::uint64_t off = 0;
+    for (std::size_t r = 0; r < R; ++r) {
+        if (L[r] < 2 || L[r] % 2 || L[r] > 65536) fail(UCMC_E_INVALID_ARGUMENT, "replica L must be even, 2..65536");
+        if (r && ladder[r] < ladder[r - 1]) fail(UCMC_E_INVALID_ARGUMENT, "ladder ids must be non-decreasing");
+        if (!(betas[r] == betas[r])) fail(UCMC_E_INVALID_ARGUMENT, "beta is NaN");
+        pt.L[r] = static_cast<std::uint32_t>(L[r]);
+        pt.offset[r] = off;
+        off += static_cast<std::uint64_t>(L[r]) * L[r];
+    }
+    s.sites = off;
+    pt.ladder_start_h.clear();
+    for (std::size_t r = 0; r < R; ++r)
+        if (r == 0 || ladder[r] != ladder[r - 1]) pt.ladder_start_h.push_back(static_cast<std::uint32_t>(r));
+    pt.ladder_start_h.push_back(static_cast<std::uint32_t>(R));
+    s.ncolours = 2;
+    s.schedule = UCMC_SCHED_CHECKERBOARD;
+    s.colour.resize(s.sites);
+    for (std::size_t r = 0; r < R; ++r)
+        for (std::uint64_t i = 0; i < static_cast<std::uint64_t>(L[r]) * L[r]; ++i)
+            s.colour[pt.offset[r] + i] = static_cast<std::uint8_t>(((i % L[r]) + (i / L[r])) & 1u);
+    s.field[0].alloc(s.sites * 4);
+    s.field[1].alloc(s.sites * 4);
+    UCMC_HIP(hipMemset(s.field[0].p, 0, s.sites * 4));
+    pt.betas_d.alloc(R * 8);
+    UCMC_HIP(hipMemcpy(pt.betas_d.p, betas, R * 8, hipMemcpyHostToDevice));
+    std::vector<std::uint32_t> ident(R);
+    for (std::size_t r = 0; r < R; ++r) ident[r] = static_cast<std::uint32_t>(r);
+    pt.temp_of.alloc(R * 4);
+    pt.rep_at.alloc(R * 4);
+    UCMC_HIP(hipMemcpy(pt.temp_of.p, ident.data(), R * 4, hipMemcpyHostToDevice));
+    UCMC_HIP(hipMemcpy(pt.rep_at.p, ident.data(), R * 4, hipMemcpyHostToDevice));
+    pt.ladder_start.alloc(pt.ladder_start_h.size() * 4);
+    UCMC_HIP(hipMemcpy(pt.ladder_start.p, pt.ladder_start_h.data(), pt.ladder_start_h.size() * 4, hipMemcpyHostToDevice));
+    pt.energy.alloc(R * 8);
+    pt.accepts.alloc(R * 8);
+    pt.swap_acc.alloc(R * 8);
+    pt.swap_try.alloc(R * 8);
+    for (DevBuf* b : {&pt.energy, &pt.accepts, &pt.swap_acc, &pt.swap_try}) UCMC_HIP(hipMemset(b->p, 0, b->bytes));
+    pt.done.alloc(4);
+    UCMC_HIP(hipMemset(pt.done.p, 0, 4));
+    pt.sweep_ctr.alloc(8);
+    pt_layout(s, kDefaultTile, 0);
+}
+
+void pt_run(System& s, const ucmc_pt_params& p, double* series, std::uint32_t* temp_of)
+{
+    if (!(p.width == p.width)) fail(UCMC_E_INVALID_ARGUMENT, "width is NaN");
+    const std::uint32_t tile = p.tile ? p.tile : (s.pt.tile ? s.pt.tile : kDefaultTile);
+    if (tile < 2 || tile > static_cast<std::uint32_t>(kMaxTile) || (tile & (tile - 1)))
+        fail(UCMC_E_INVALID_ARGUMENT, "tile must be a power of two in [2, 64]");
+    pt_layout(s, tile, p.run_seed);
+    const unsigned long long s0 = p.sweep0;
+    UCMC_HIP(hipMemcpyAsync(s.pt.sweep_ctr.p, &s0, 8, hipMemcpyHostToDevice, s.stream));
+    const std::size_t R = s.pt.L.size();
+    DevBuf sd;
+    std::uint32_t cap = 0;
+    if (series) {
+        cap = static_cast<std::uint32_t>(p.nsweeps);
+        sd.alloc(p.nsweeps * R * 8);
+    }
+    const PtArgs a = pt_args(s, p, series ? sd.as<double>() : nullptr, cap);
+    std::uint64_t left = p.nsweeps;
+    if (p.use_graph && left >= 2) {
+        // capture an even number of sweeps (the ping-pong returns to the same buffer)
+        const std::uint64_t G = std::min<std::uint64_t>(left & ~1ull, 64);
+        const int cur0 = s.cur;
+        hipGraph_t g;
+        hipGraphExec_t ge;
+        UCMC_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
+        for (std::uint64_t k = 0; k < G; ++k) pt_launch(s, a);
+        UCMC_HIP(hipStreamEndCapture(s.stream, &g));
+        UCMC_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+        s.cur = cur0;
+        while (left >= G) {
+            UCMC_HIP(hipGraphLaunch(ge, s.stream));
+            left -= G; // G is even: s.cur is back to cur0
+        }
+        (void)hipGraphExecDestroy(ge);
+        (void)hipGraphDestroy(g);
+    }
+    while (left > 0) {
+        pt_launch(s, a);
+        --left;
+    }
+    if (series) UCMC_HIP(hipMemcpyAsync(series, sd.p, p.nsweeps * R * 8, hipMemcpyDeviceToHost, s.stream));
+    if (temp_of) UCMC_HIP(hipMemcpyAsync(temp_of, s.pt.temp_of.p, R * 4, hipMemcpyDeviceToHost, s.stream));
+    UCMC_HIP(hipStreamSynchronize(s.stream));
+}
+
+void pt_stats(const System& s, std::uint64_t* acc, std::uint64_t* sw_acc, std::uint64_t* sw_try)
+{
+    const std::size_t R = s.pt.L.size();
+    if (acc) UCMC_HIP(hipMemcpy(acc, s.pt.accepts.p, R * 8, hipMemcpyDeviceToHost));
+    if (sw_acc) UCMC_HIP(hipMemcpy(sw_acc, s.pt.swap_acc.p, R * 8, hipMemcpyDeviceToHost));
+    if (sw_try) UCMC_HIP(hipMemcpy(sw_try, s.pt.swap_try.p, R * 8, hipMemcpyDeviceToHost));
+}
+
+void pt_autotune(System& s, double width, std::uint32_t* tile, double* best_ms)
+{
+    auto& pt = s.pt;
+    const std::size_t R = pt.L.size();
+    // save the state the trial sweeps modify
+    std::vector<float> f(s.sites);
+    std::vector<std::uint32_t> to(R), ra(R);
+    std::vector<unsigned long long> acc(R), sa(R), st(R);
+    UCMC_HIP(hipStreamSynchronize(s.stream));
+    UCMC_HIP(hipMemcpy(f.data(), s.field[s.cur].p, s.sites * 4, hipMemcpyDeviceToHost));
+    UCMC_HIP(hipMemcpy(to.data(), pt.temp_of.p, R * 4, hipMemcpyDeviceToHost));
+    UCMC_HIP(hipMemcpy(ra.data(), pt.rep_at.p, R * 4, hipMemcpyDeviceToHost));
+    UCMC_HIP(hipMemcpy(acc.data(), pt.accepts.p, R * 8, hipMemcpyDeviceToHost));
+    UCMC_HIP(hipMemcpy(sa.data(), pt.swap_acc.p, R * 8, hipMemcpyDeviceToHost));
+    UCMC_HIP(hipMemcpy(st.data(), pt.swap_try.p, R * 8, hipMemcpyDeviceToHost));
+    const int cur0 = s.cur;
+    double best = 1e300;
+    std::uint32_t best_t = kDefaultTile;
+    for (std::uint32_t t : {16u, 32u, 64u}) {
+        ucmc_pt_params p{};
+        p.width = width;
+        p.run_seed = 0xA11CE;
+        p.nsweeps = 4;
+        p.tile = t;
+        pt_run(s, p, nullptr, nullptr); // warm-up
+        hipEvent_t e0, e1;
+        UCMC_HIP(hipEventCreate(&e0));
+        UCMC_HIP(hipEventCreate(&e1));
+        p.nsweeps = 20;
+        UCMC_HIP(hipEventRecord(e0, s.stream));
+        pt_run(s, p, nullptr, nullptr);
+        UCMC_HIP(hipEventRecord(e1, s.stream));
+        UCMC_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        UCMC_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (ms / 20.0 < best) {
+            best = ms / 20.0;
+            best_t = t;
+        }
+    }
+    s.cur = cur0;
+    UCMC_HIP(hipMemcpy(s.field[s.cur].p, f.data(), s.sites * 4, hipMemcpyHostToDevice));
+    UCMC_HIP(hipMemcpy(pt.temp_of.p, to.data(), R * 4, hipMemcpyHostToDevice));
+    UCMC_HIP(hipMemcpy(pt.rep_at.p, ra.data(), R * 4, hipMemcpyHostToDevice));
+    UCMC_HIP(hipMemcpy(pt.accepts.p, acc.data(), R * 8, hipMemcpyHostToDevice));
+    UCMC_HIP(hipMemcpy(pt.swap_acc.p, sa.data(), R * 8, hipMemcpyHostToDevice));
+    UCMC_HIP(hipMemcpy(pt.swap_try.p, st.data(), R * 8, hipMemcpyHostToDevice));
+    pt.tile = best_t;
+    if (tile) *tile = best_t;
+    if (best_ms) *best_ms = best;
+}
+
+}  // namespace ucmc

@@ -1,0 +1,508 @@
+// ucmc/gpu.hpp -- C++23 host layer: device drop-ins for the reference's
+// drivers (driver.hpp:61-254).
+//
+//   ucmc::gpu::Metropolis metro(model, Flip{}, Pcg32(11, 1), 0.3);   // driver.hpp:66-69
+//   metro.run(2000);                                                  // driver.hpp:90-93
+//   metro.sample(4000, [&](const Ising& m) { ... });                  // driver.hpp:98-105
+//   ucmc::gpu::Wolff wolf(model, FixedAxis{}, Pcg32(12, 1), 0.3);     // driver.hpp:140-145
+//   wolf.flip(); wolf.sample(20000, observe); wolf.mean_cluster_size();
+//
+// Same constructor arguments, same members, same ownership: the driver holds
+// the model by reference (M&) and copies the proposal / axis and the RNG, so
+// the caller's RNG is not advanced (driver.hpp:107-111, 242-246).  The model
+// stays the owner of record: its field is uploaded at construction, the device
+// copy is authoritative while sweeps run, and the host model is written back
+// (through ModelLike::update, model.hpp:70-86) before every observe() call and
+// on sync().  sample_device() is the fast path that hands device-reduced
+// observables to the callback instead of the model.
+//
+// The kernel family is chosen from the types by their structure, so the
+// reference's own types (and user types shaped like them) work unchanged:
+//   Ising-like:      value_type int8_t, size 1, degree()/bonds()/energy_diff_bond
+//   VectorSpin-like: value_type double, size N in {2,3,4}, degree()/bonds()
+//   U1Gauge-like:    value_type with a double `theta`, length()
+//   SUGauge-like:    value_type with `dim` and an `m` array of complex<double>, length()
+// with the proposal Flip / UniformSphereN / LocalRotation (has `delta`) /
+// U1Shift / SUShift (have `width`).  Couplings the models keep private (J,
+// beta) are read back through the models' own energy functions.
+//
+// Random numbers differ from the reference's single sequential stream: site i
+// in sweep s draws from Pcg32(key(run_seed, s), i) and sites are updated colour
+// class by colour class (include/ucmc.h).  run_seed is taken from two draws of
+// the driver's copy of the RNG.
+#pragma once
+
+#include <ucmc.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <complex>
+#include <concepts>
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+namespace ucmc::gpu {
+
+// ---------------------------------------------------------------- errors --
+[[noreturn]] inline void throw_status(int code)
+{
+    const std::string m = ucmc_last_error();
+    switch (code) {
+    case UCMC_E_INVALID_ARGUMENT: throw std::invalid_argument(m);
+    case UCMC_E_OUT_OF_RANGE: throw std::out_of_range(m);
+    case UCMC_E_OVERFLOW: throw std::overflow_error(m);
+    default: throw std::runtime_error("ucmc: " + m);
+    }
+}
+inline void check(int code)
+{
+    if (code != UCMC_OK) throw_status(code);
+}
+
+// -------------------------------------------------------- system handle --
+class System
+{
+public:
+    System() = default;
+    explicit System(ucmc_system* h) : h_(h) {}
+    System(const System&) = delete;
+    System& operator=(const System&) = delete;
+    System(System&& o) noexcept : h_(std::exchange(o.h_, nullptr)) {}
+    System& operator=(System&& o) noexcept
+    {
+        if (this != &o) {
+            reset();
+            h_ = std::exchange(o.h_, nullptr);
+        }
+        return *this;
+    }
+    ~System() { reset(); }
+    ucmc_system* get() const noexcept { return h_; }
+    void reset() noexcept
+    {
+        if (h_) (void)ucmc_system_destroy(h_);
+        h_ = nullptr;
+    }
+    ucmc_info info() const
+    {
+        ucmc_info i{};
+        check(ucmc_system_info(h_, &i));
+        return i;
+    }
+
+private:
+    ucmc_system* h_ = nullptr;
+};
+
+// ---------------------------------------------------- structural traits --
+template <class M>
+concept PairRows = requires(const M cm, std::size_t i) {
+    { cm.degree(i) } -> std::convertible_to<std::size_t>;
+    { cm.bonds(i) };
+};
+
+template <class M>
+concept IsingLike = PairRows<M> && std::same_as<typename M::value_type, std::int8_t> &&
+                    (M::size == 1);
+
+template <class M>
+concept VectorLike = PairRows<M> && std::same_as<typename M::value_type, double> &&
+                     (M::size >= 2) && (M::size <= 4);
+
+template <class M>
+concept U1GaugeLike = requires(const M cm, const typename M::value_type v) {
+    { v.theta } -> std::convertible_to<double>;
+    { cm.length() } -> std::convertible_to<std::size_t>;
+} && (sizeof(typename M::value_type) == sizeof(double));
+
+template <class M>
+concept SUGaugeLike = requires(const M cm, const typename M::value_type v) {
+    M::value_type::dim;
+    v.m;
+    { cm.length() } -> std::convertible_to<std::size_t>;
+} && (sizeof(typename M::value_type) ==
+      2 * sizeof(double) * M::value_type::dim * M::value_type::dim);
+
+template <class Q>
+concept HasDelta = requires(const Q q) { { q.delta } -> std::convertible_to<double>; };
+template <class Q>
+concept HasWidth = requires(const Q q) { { q.width } -> std::convertible_to<double>; };
+
+// 64-bit run seed from two uniforms of the driver's own copy of the RNG
+// (uni() is exactly k * 2^-32, rng.hpp:24-27).
+template <class R>
+std::uint64_t seed_from(R rng)
+{
+    const auto hi = static_cast<std::uint64_t>(rng.uni() * 4294967296.0);
+    const auto lo = static_cast<std::uint64_t>(rng.uni() * 4294967296.0);
+    return (hi << 32) | lo;
+}
+
+// CSR of the model's rows (pair_graph.hpp:96-107 layout).
+template <class M>
+void rows_of(const M& m, std::vector<std::uint32_t>& off, std::vector<std::uint32_t>& bonds)
+{
+    const std::size_t n = m.sites();
+    off.assign(n + 1, 0);
+    bonds.clear();
+    for (std::size_t i = 0; i < n; ++i) {
+        const auto* b = m.bonds(i);
+        for (std::size_t a = 0; a < m.degree(i); ++a) bonds.push_back(static_cast<std::uint32_t>(b[a].j));
+        off[i + 1] = static_cast<std::uint32_t>(bonds.size());
+    }
+}
+
+// J from energy_diff_bond(i, bond, p) = -J (p - s_j).s_i (example/ising.hpp:59-65,
+// example/vector_spin.hpp:63-69), probed on the first bond of the model.
+template <class M>
+double probe_pair_coupling(const M& m)
+{
+    using T = typename M::value_type;
+    constexpr std::size_t D = M::size;
+    for (std::size_t i = 0; i < m.sites(); ++i) {
+        if (m.degree(i) == 0) continue;
+        const auto& bond = m.bonds(i)[0];
+        const T* si = m.data() + i * D;
+        const T* sj = m.data() + static_cast<std::size_t>(bond.j) * D;
+        std::array<T, D> p{};
+        double denom = 0.0;
+        if constexpr (std::is_same_v<T, std::int8_t>) {
+            p[0] = static_cast<T>(-sj[0]);
+            denom = -(static_cast<double>(p[0]) - sj[0]) * si[0];
+        } else {
+            // p = s_i unless that makes the denominator vanish, then p = -s_j
+            double ss = 0.0, sd = 0.0;
+            for (std::size_t k = 0; k < D; ++k) {
+                ss += si[k] * si[k];
+                sd += sj[k] * si[k];
+            }
+            if (std::abs(ss - sd) > 1e-3) {
+                for (std::size_t k = 0; k < D; ++k) p[k] = si[k];
+                denom = -(ss - sd);
+            } else {
+                for (std::size_t k = 0; k < D; ++k) p[k] = -sj[k];
+                denom = 2.0 * sd;
+            }
+        }
+        if (denom == 0.0) continue;
+        return static_cast<double>(m.energy_diff_bond(i, bond, p.data())) / denom;
+    }
+    return 1.0; // no bonds: J does not enter any energy
+}
+
+// --------------------------------------------------------- device model --
+template <class M>
+struct Device
+{
+    static_assert(IsingLike<M> || VectorLike<M> || U1GaugeLike<M> || SUGaugeLike<M>,
+                  "ucmc::gpu: no device kernel family for this model type");
+    using T = typename M::value_type;
+
+    static constexpr std::size_t doubles_per_site()
+    {
+        if constexpr (IsingLike<M>) return 0;
+        else if constexpr (VectorLike<M>) return M::size;
+        else if constexpr (U1GaugeLike<M>) return 1;
+        else return 2 * T::dim * T::dim;
+    }
+
+    static System create(const M& m)
+    {
+        ucmc_system* h = nullptr;
+        if constexpr (IsingLike<M>) {
+            std::vector<std::uint32_t> off, b;
+            rows_of(m, off, b);
+            const ucmc_pair_graph g{m.sites(), off.data(), b.data()};
+            const double J = probe_pair_coupling(m);
+            check(ucmc_ising_create(&g, static_cast<std::int32_t>(J), 0, &h));
+        } else if constexpr (VectorLike<M>) {
+            std::vector<std::uint32_t> off, b;
+            rows_of(m, off, b);
+            const ucmc_pair_graph g{m.sites(), off.data(), b.data()};
+            check(ucmc_vector_create(static_cast<int>(M::size), &g, probe_pair_coupling(m), 0, &h));
+        } else if constexpr (U1GaugeLike<M>) {
+            check(ucmc_u1_gauge_create(m.length(), gauge_beta(m), 0, &h));
+        } else {
+            check(ucmc_sun_gauge_create(static_cast<int>(T::dim), m.length(), gauge_beta(m), 0, &h));
+        }
+        System s(h);
+        upload(s, m);
+        return s;
+    }
+
+    static void upload(System& s, const M& m)
+    {
+        if constexpr (IsingLike<M>)
+            check(ucmc_field_upload(s.get(), m.data(), m.sites()));
+        else
+            check(ucmc_field_upload(s.get(), m.data(), m.sites() * doubles_per_site() * sizeof(double)));
+    }
+
+    // Writes the device field into the model through ModelLike::update.
+    static void download(System& s, M& m)
+    {
+        const std::size_t n = m.sites();
+        if constexpr (IsingLike<M>) {
+            std::vector<std::int8_t> v(n);
+            check(ucmc_field_download(s.get(), v.data(), n));
+            for (std::size_t i = 0; i < n; ++i) m.update(i, &v[i]);
+        } else {
+            const std::size_t d = doubles_per_site();
+            std::vector<double> v(n * d);
+            check(ucmc_field_download(s.get(), v.data(), v.size() * sizeof(double)));
+            if constexpr (VectorLike<M>) {
+                for (std::size_t i = 0; i < n; ++i) m.update(i, v.data() + i * d);
+            } else {
+                for (std::size_t i = 0; i < n; ++i) {
+                    T t;
+                    static_assert(std::is_trivially_copyable_v<T>);
+                    std::memcpy(static_cast<void*>(&t), v.data() + i * d, sizeof(T));
+                    m.update(i, &t);
+                }
+            }
+        }
+    }
+
+    // The gauge models keep beta private; energy_diff_at of one link is
+    // beta * sum_p (before - after), and the same sum at beta = 1 comes from a
+    // throw-away device system holding the same field.
+    static double gauge_beta(const M& m)
+    {
+        ucmc_system* h = nullptr;
+        if constexpr (U1GaugeLike<M>) check(ucmc_u1_gauge_create(m.length(), 1.0, 0, &h));
+        else check(ucmc_sun_gauge_create(static_cast<int>(T::dim), m.length(), 1.0, 0, &h));
+        System unit(h);
+        upload(unit, m);
+        ucmc_observables before{}, after{};
+        check(ucmc_observe(unit.get(), &before));
+        // try a few candidate changes of link 0 until the plaquette sum moves
+        for (int attempt = 1; attempt <= 8; ++attempt) {
+            T p = m.data()[0];
+            const double a = 0.37 * attempt;
+            if constexpr (U1GaugeLike<M>) p.theta = p.theta + a;
+            else {
+                constexpr std::size_t N = T::dim;
+                const std::complex<double> e(std::cos(a), std::sin(a));
+                for (std::size_t r = 0; r < N; ++r) {
+                    p.m[r * N + 0] *= e;
+                    p.m[r * N + 1] *= std::conj(e);
+                }
+            }
+            std::vector<T> f(m.data(), m.data() + m.sites());
+            f[0] = p;
+            check(ucmc_field_upload(unit.get(), f.data(), f.size() * sizeof(T)));
+            check(ucmc_observe(unit.get(), &after));
+            const double unit_dE = after.energy - before.energy; // beta = 1 action change
+            if (std::abs(unit_dE) > 1e-3) return m.energy_diff_at(0, &p) / unit_dE;
+        }
+        throw std::invalid_argument("ucmc::gpu: cannot recover the gauge coupling");
+    }
+};
+
+// Proposal -> (UCMC_PROP_*, width).
+template <class M, class Q>
+constexpr int proposal_kind()
+{
+    if constexpr (IsingLike<M>) return UCMC_PROP_FLIP;
+    else if constexpr (VectorLike<M>) return HasDelta<Q> ? UCMC_PROP_ROTATION : UCMC_PROP_SPHERE;
+    else if constexpr (U1GaugeLike<M>) return UCMC_PROP_U1_SHIFT;
+    else return UCMC_PROP_SU_SHIFT;
+}
+template <class Q>
+double proposal_width(const Q& q)
+{
+    if constexpr (HasDelta<Q>) return static_cast<double>(q.delta);
+    else if constexpr (HasWidth<Q>) return static_cast<double>(q.width);
+    else return 0.0;
+}
+
+// ----------------------------------------------------------- Metropolis --
+template <class M, class Q, class R>
+class Metropolis
+{
+public:
+    Metropolis(M& model, Q proposal, R rng, double beta)
+        : model_(model), proposal_(std::move(proposal)), run_seed_(seed_from(rng)), beta_(beta),
+          sys_(Device<M>::create(model))
+    {}
+
+    /// One pass over every site (driver.hpp:73-87), in colour order on the device.
+    void sweep() { run(1); }
+
+    /// n passes over every site.
+    void run(std::size_t sweeps)
+    {
+        auto p = params(sweeps, 0);
+        check(ucmc_metropolis_run(sys_.get(), &p, nullptr));
+        sweep_ += sweeps;
+    }
+
+    /// Runs n sweeps and calls observe(model) after each one (driver.hpp:98-105);
+    /// the model is brought up to date before each call.
+    template <class F>
+    void sample(std::size_t sweeps, F&& observe)
+    {
+        for (std::size_t k = 0; k < sweeps; ++k) {
+            run(1);
+            Device<M>::download(sys_, model_);
+            observe(std::as_const(model_));
+        }
+    }
+
+    /// Fast path: n sweeps with the energy, order parameter and accept count of
+    /// every sweep reduced on the device (FP64) and handed to on_sweep.
+    template <class F>
+    void sample_device(std::size_t sweeps, F&& on_sweep)
+    {
+        std::vector<ucmc_observables> series(sweeps);
+        auto p = params(sweeps, UCMC_RECORD_ENERGY | UCMC_RECORD_ORDER | UCMC_RECORD_ACCEPTS);
+        check(ucmc_metropolis_run(sys_.get(), &p, series.data()));
+        sweep_ += sweeps;
+        for (const auto& o : series) on_sweep(o);
+    }
+
+    /// Writes the device field back into the model.
+    void sync() { Device<M>::download(sys_, model_); }
+
+    /// Re-reads the model's field (after the caller changed it).
+    void reload() { Device<M>::upload(sys_, model_); }
+
+    ucmc_info info() const { return sys_.info(); }
+    std::uint64_t sweeps_done() const noexcept { return sweep_; }
+    ucmc_system* system() const noexcept { return sys_.get(); }
+
+private:
+    ucmc_metropolis_params params(std::size_t n, std::uint32_t flags) const
+    {
+        ucmc_metropolis_params p{};
+        p.beta = beta_;
+        p.proposal = proposal_kind<M, Q>();
+        p.width = proposal_width(proposal_);
+        p.hits = 1;
+        p.run_seed = run_seed_;
+        p.sweep0 = sweep_;
+        p.nsweeps = n;
+        p.flags = flags;
+        return p;
+    }
+
+    M& model_;
+    Q proposal_;
+    std::uint64_t run_seed_;
+    double beta_;
+    System sys_;
+    std::uint64_t sweep_ = 0;
+};
+
+// ---------------------------------------------------------------- Wolff --
+template <class M, class A, class R>
+    requires(IsingLike<M> || VectorLike<M>)
+class Wolff
+{
+public:
+    Wolff(M& model, A axis, R rng, double beta)
+        : model_(model), axis_(std::move(axis)), run_seed_(seed_from(rng)), beta_(beta),
+          sys_(Device<M>::create(model))
+    {}
+
+    /// One cluster flip; returns how many sites it flipped (driver.hpp:148-206).
+    std::size_t flip()
+    {
+        std::uint64_t size = 0;
+        auto p = params(1, 0);
+        check(ucmc_wolff_run(sys_.get(), &p, &size, nullptr));
+        ++flip_;
+        ++flips_;
+        size_sum_ += static_cast<double>(size);
+        return static_cast<std::size_t>(size);
+    }
+
+    /// Flips clusters until the flipped sites add up to one lattice (driver.hpp:213-217).
+    void sweep()
+    {
+        const std::size_t n = model_.sites();
+        std::uint64_t touched = 0;
+        while (touched < n) {
+            const std::uint64_t batch = std::max<std::uint64_t>(1, std::min<std::uint64_t>(n, 4096));
+            std::vector<std::uint64_t> sizes(batch, 0);
+            auto p = params(batch, 0);
+            p.touch_target = n - touched;
+            check(ucmc_wolff_run(sys_.get(), &p, sizes.data(), nullptr));
+            for (auto s : sizes) {
+                if (s == 0) break;
+                touched += s;
+                ++flip_;
+                ++flips_;
+                size_sum_ += static_cast<double>(s);
+            }
+        }
+    }
+
+    void run(std::size_t sweeps)
+    {
+        for (std::size_t k = 0; k < sweeps; ++k) sweep();
+    }
+
+    /// Flips n clusters and calls observe(model) after each flip (driver.hpp:227-234).
+    template <class F>
+    void sample(std::size_t flips, F&& observe)
+    {
+        for (std::size_t k = 0; k < flips; ++k) {
+            flip();
+            Device<M>::download(sys_, model_);
+            observe(std::as_const(model_));
+        }
+    }
+
+    /// Fast path: n flips with device-tracked energy / order parameter per flip.
+    template <class F>
+    void sample_device(std::size_t flips, F&& on_flip)
+    {
+        std::vector<ucmc_observables> series(flips);
+        std::vector<std::uint64_t> sizes(flips);
+        auto p = params(flips, UCMC_RECORD_ENERGY | UCMC_RECORD_ORDER);
+        check(ucmc_wolff_run(sys_.get(), &p, sizes.data(), series.data()));
+        for (std::size_t k = 0; k < flips; ++k) {
+            ++flip_;
+            ++flips_;
+            size_sum_ += static_cast<double>(sizes[k]);
+            on_flip(series[k]);
+        }
+    }
+
+    /// Mean cluster size over every flip so far (driver.hpp:236-240).
+    double mean_cluster_size() const { return flips_ == 0 ? 0.0 : size_sum_ / double(flips_); }
+
+    void sync() { Device<M>::download(sys_, model_); }
+    void reload() { Device<M>::upload(sys_, model_); }
+
+private:
+    ucmc_wolff_params params(std::uint64_t n, std::uint32_t flags) const
+    {
+        ucmc_wolff_params p{};
+        p.beta = beta_;
+        p.run_seed = run_seed_;
+        p.flip0 = flip_;
+        p.nflips = n;
+        p.flags = flags;
+        return p;
+    }
+
+    M& model_;
+    A axis_;
+    std::uint64_t run_seed_;
+    double beta_;
+    System sys_;
+    std::uint64_t flip_ = 0;
+    std::size_t flips_ = 0;
+    double size_sum_ = 0.0;
+};
+
+}  // namespace ucmc::gpu
