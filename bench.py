@@ -92,6 +92,71 @@ def cpu_baseline(L, seconds_target=12.0):
             "seconds": dt}
 
 
+def suite(quick=False):
+    """Throughput of the other BASELINE configurations (one GPU), each bounded
+    to a few seconds; reported beside the headline, not as `value`."""
+    from universal_classical_montecarlo_amd import abi
+    from oracle_lib import csr_cubic
+    out = {}
+    # C2b: 3D Heisenberg 128^3, LocalRotation(0.5), beta = 0.693, 2-colour
+    L = 64 if quick else 128
+    off, bonds = csr_cubic(L)
+    v = abi.System.vector(3, off, bonds, 1.0)
+    rng = np.random.default_rng(5)
+    x = rng.normal(size=(L ** 3, 3))
+    x /= np.linalg.norm(x, axis=1, keepdims=True)
+    v.upload(x.reshape(-1))
+    v.metropolis(0.693, 5, 1, proposal=abi.PROP_ROTATION, width=0.5)
+    ms, n = v.bench(0.693, 50, 2, proposal=abi.PROP_ROTATION, width=0.5)
+    rate = L ** 3 * 50 / (ms * 1e-3)
+    out["heisenberg_128_metropolis"] = {"site_updates_per_s": rate,
+                                        "hbm_frac_24B": 24 * rate / 1e9 / HBM_PEAK_GBS}
+    t0 = time.perf_counter()
+    sizes, _ = v.wolff(0.693, 200, 3)
+    dt = time.perf_counter() - t0
+    out["heisenberg_128_wolff"] = {"flipped_sites_per_s": float(sizes.sum()) / dt,
+                                   "mean_cluster": float(sizes.mean())}
+    v.close()
+    # C3: SU(3) 24^4 (16^4 quick), beta = 5.7 set once (model), SUShift 0.2, 4 hits
+    for L in ((16,) if quick else (16, 24)):
+        g = abi.System.sun_gauge(3, L, 5.7)
+        g.metropolis(1.0, 2, 1, proposal=abi.PROP_SU_SHIFT, width=0.2, hits=4)
+        ms, n = g.bench(1.0, 10, 2, proposal=abi.PROP_SU_SHIFT, width=0.2, hits=4,
+                        reunitarize_every=10)
+        links = 4 * L ** 4
+        rate = links * 10 / (ms * 1e-3)
+        o = g.observe()
+        out[f"su3_{L}^4_4hit"] = {"link_updates_per_s": rate, "plaquette": o.plaquette,
+                                 "staple_and_hit_flop_per_link": 2466 + 4 * (198 + 36 + 198 * 10),
+                                 "tflops_est": rate * (2466 + 4 * (198 + 36 + 198 * 10)) / 1e12}
+        g.close()
+    # C4: 256 XY replicas under parallel tempering
+    Ls, ladders, betas = [], [], []
+    grid = 0.8 + 0.6 * np.arange(64) / 63
+    for ld, (Lr, count) in enumerate([(32, 64), (64, 64), (128, 64), (256, 32), (512, 32)]):
+        bs = grid if count == 64 else grid[::2]
+        Ls += [Lr] * count
+        ladders += [ld] * count
+        betas += list(bs)
+    pt = abi.System.pt(Ls, ladders, betas)
+    tile, _ = pt.pt_autotune(1.0)
+    pt.pt_run(1.0, 10, 1, use_graph=True)
+    t0 = time.perf_counter()
+    pt.pt_run(1.0, 100, 2, sweep0=10, use_graph=True)
+    dt = time.perf_counter() - t0
+    acc, sacc, stry = pt.pt_stats()
+    out["xy_pt_256_replicas"] = {"site_updates_per_s": pt.sites * 100 / dt, "tile": tile,
+                                 "hbm_frac_8B": 8 * pt.sites * 100 / dt / 1e9 / HBM_PEAK_GBS,
+                                 "swap_acceptance": float(sacc.sum() / max(stry.sum(), 1))}
+    pt.close()
+    # C5: SLMC vs plain GPU Metropolis, independent samples / s
+    from universal_classical_montecarlo_amd import slmc
+    out["slmc"] = slmc.compare(L=64 if quick else 256, nconfigs=4000 if quick else 20000,
+                               plain_sweeps=20000 if quick else 200000,
+                               slmc_steps=2000 if quick else 20000)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +164,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--L", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-suite", action="store_true")
+    ap.add_argument("--quick-suite", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one MI355X, and the lattice "
@@ -135,6 +202,7 @@ def main():
         s2.close()
 
     cpu = None if args.no_cpu else cpu_baseline(L)
+    extra = None if args.no_suite else suite(quick=args.quick_suite)
     line = {
         "metric": METRIC,
         "value": value,
@@ -158,6 +226,7 @@ def main():
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in
                                                   ("value", "unit", "cores", "kind", "sample")},
         "out_of_cache": ooc,
+        "other_configs": extra,
     }
     print(json.dumps(line), flush=True)
 

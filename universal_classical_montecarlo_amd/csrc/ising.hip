@@ -89,10 +89,15 @@ struct Row {
     std::uint32_t E, O; // 4b-form (0 for +1, 4 for -1 per byte) of even / odd x
 };
 
-__device__ __forceinline__ Row load_row(const std::uint32_t* in, std::uint32_t row_words,
-                                        std::uint32_t Y, std::uint32_t col)
+__device__ __forceinline__ uint2 load_raw(const std::uint32_t* in, std::uint32_t row_words, std::uint32_t Y,
+                                          std::uint32_t col)
 {
-    const uint2 w = *reinterpret_cast<const uint2*>(in + static_cast<std::size_t>(Y) * row_words + 2 * col);
+    return *reinterpret_cast<const uint2*>(in + static_cast<std::size_t>(Y) * row_words + 2 * col);
+}
+
+// Split 8 bytes into even-x and odd-x words and map +1 -> 0, -1 -> 4 per byte.
+__device__ __forceinline__ Row convert_row(uint2 w)
+{
     const std::uint32_t e = __builtin_amdgcn_perm(w.y, w.x, 0x06040200u);
     const std::uint32_t o = __builtin_amdgcn_perm(w.y, w.x, 0x07050301u);
     return Row{(e << 1) & 0x04040404u, (o << 1) & 0x04040404u};
@@ -109,9 +114,25 @@ __device__ __forceinline__ void store_row(std::uint32_t* out, std::uint32_t row_
     *reinterpret_cast<uint2*>(out + static_cast<std::size_t>(Y) * row_words + 2 * col) = w;
 }
 
+// Byte k of C takes byte k of Cf when u <= t: one compare plus one
+// sub-dword conditional move (SDWA dst_sel:BYTE_k, the other bytes preserved).
+template <int K>
+__device__ __forceinline__ void select_byte(std::uint32_t& C, std::uint32_t Cf, std::uint32_t u, std::uint32_t t)
+{
+#define UCMC_SEL(B)                                                                                   \
+    asm("v_cmp_le_u32 vcc, %1, %2\n\tv_cndmask_b32_sdwa %0, %0, %3, vcc dst_sel:BYTE_" #B          \
+        " dst_unused:UNUSED_PRESERVE src0_sel:BYTE_" #B " src1_sel:BYTE_" #B                          \
+        : "+v"(C) : "v"(u), "v"(t), "v"(Cf) : "vcc")
+    if constexpr (K == 0) UCMC_SEL(0);
+    else if constexpr (K == 1) UCMC_SEL(1);
+    else if constexpr (K == 2) UCMC_SEL(2);
+    else UCMC_SEL(3);
+#undef UCMC_SEL
+}
+
 // Decide the four sites of one 4b-form word C with neighbour counts S.
-// xb: per-lane stream offset of byte 0 relative to the row; rowk[k]: the
-// wave-uniform part of site k's state.  Returns the updated word.
+// lane_base: per-lane part of the stream state; rowk[k]: the wave-uniform part
+// of site k's state.  Returns the updated word.
 template <bool COUNT>
 __device__ __forceinline__ std::uint32_t decide(std::uint32_t C, std::uint32_t S,
                                                 std::uint64_t lane_base,
@@ -119,17 +140,21 @@ __device__ __forceinline__ std::uint32_t decide(std::uint32_t C, std::uint32_t S
                                                 const std::uint32_t* tbl, std::uint32_t& acc)
 {
     const std::uint32_t key = (C << 3) | S; // byte k = 4 * (8 C_k + S_k)
-    std::uint32_t F = 0;
+    const std::uint32_t Cf = C ^ 0x04040404u;
+    std::uint32_t out = C;
+    std::uint32_t t[4], u[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const std::uint32_t addr = (key >> (8 * k)) & 0xFFu;
-        const std::uint32_t t = *reinterpret_cast<const std::uint32_t*>(
-            reinterpret_cast<const char*>(tbl) + addr);
-        const std::uint32_t u = pcg_out(lane_base + rowk[k]);
-        F |= (u <= t) ? (4u << (8 * k)) : 0u;
+        t[k] = *reinterpret_cast<const std::uint32_t*>(reinterpret_cast<const char*>(tbl) + addr);
+        u[k] = pcg_out(lane_base + rowk[k]);
     }
-    if (COUNT) acc += __builtin_popcount(F);
-    return C ^ F;
+    select_byte<0>(out, Cf, u[0], t[0]);
+    select_byte<1>(out, Cf, u[1], t[1]);
+    select_byte<2>(out, Cf, u[2], t[2]);
+    select_byte<3>(out, Cf, u[3], t[3]);
+    if (COUNT) acc += __builtin_popcount(out ^ C);
+    return out;
 }
 
 template <bool COUNT>
@@ -140,7 +165,9 @@ __global__ __launch_bounds__(256) void ising_fused(FusedArgs a)
     __shared__ std::uint32_t haloR[4][66];
 
     const std::uint32_t lane = threadIdx.x & 63u;
-    const std::uint32_t wid = threadIdx.x >> 6;
+    // wave-uniform by construction; readfirstlane lets the compiler keep every
+    // row / stream quantity derived from it in scalar registers
+    const std::uint32_t wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const std::uint32_t gw = blockIdx.x * 4u + wid;          // global wave index
     const std::uint32_t nblocks = a.Ly / a.H;
     if (threadIdx.x < 16) tbl[threadIdx.x] = a.tm1[threadIdx.x];
@@ -212,35 +239,41 @@ __global__ __launch_bounds__(256) void ising_fused(FusedArgs a)
         cur.O = decide<COUNT>(cur.O, S, lane_base, rk, tbl, acc);
     };
 
-    // rows y0-2 .. y0+1
+    // rows y0-2 .. y0+1; each row is fetched one step before it is used
     Row A_, B_, C_, D_;
-    D_ = load_row(a.in, row_words, wrap(y0 + a.Ly - 2u), col);
-    A_ = load_row(a.in, row_words, wrap(y0 + a.Ly - 1u), col);
-    B_ = load_row(a.in, row_words, y0, col);
-    updO(A_, D_, B_, wrap(y0 + a.Ly - 1u), hidx(0u) - 1u);  // colour 0 of row y0-1 (odd)
-    C_ = load_row(a.in, row_words, wrap(y0 + 1u), col);
-    updE(B_, A_, C_, y0, hidx(0u));                      // colour 0 of row y0 (even)
+    D_ = convert_row(load_raw(a.in, row_words, wrap(y0 + a.Ly - 2u), col));
+    A_ = convert_row(load_raw(a.in, row_words, wrap(y0 + a.Ly - 1u), col));
+    B_ = convert_row(load_raw(a.in, row_words, y0, col));
+    uint2 nxt = load_raw(a.in, row_words, wrap(y0 + 1u), col);
+    updO(A_, D_, B_, wrap(y0 + a.Ly - 1u), 0u);  // colour 0 of row y0-1 (odd)
+    C_ = convert_row(nxt);
+    nxt = load_raw(a.in, row_words, wrap(y0 + 2u), col);
+    updE(B_, A_, C_, y0, hidx(0u));            // colour 0 of row y0 (even)
 
     // main loop: step for absolute row r = y0 + 2 + k
     for (std::uint32_t k = 0; k < a.H; k += 4) {
         const std::uint32_t r = y0 + 2u + k; // even
         // k % 4 == 0: window (A,B,C,D) = rows r-3 .. r
-        D_ = load_row(a.in, row_words, wrap(r), col);
+        D_ = convert_row(nxt);
+        nxt = load_raw(a.in, row_words, wrap(r + 1u), col);
         updO(C_, B_, D_, wrap(r - 1u), hidx(k + 1u));      // colour 0, row r-1 (odd)
         updO(B_, A_, C_, r - 2u, hidx(k));                // colour 1, row r-2 (even)
         store_row(a.out, row_words, r - 2u, col, B_);
         // k % 4 == 1: window (B,C,D,A), r+1 odd
-        A_ = load_row(a.in, row_words, wrap(r + 1u), col);
+        A_ = convert_row(nxt);
+        nxt = load_raw(a.in, row_words, wrap(r + 2u), col);
         updE(D_, C_, A_, wrap(r), hidx(k + 2u));           // colour 0, row r (even)
         updE(C_, B_, D_, wrap(r - 1u), hidx(k + 1u));      // colour 1, row r-1 (odd)
         store_row(a.out, row_words, wrap(r - 1u), col, C_);
         // k % 4 == 2: window (C,D,A,B), r+2 even
-        B_ = load_row(a.in, row_words, wrap(r + 2u), col);
+        B_ = convert_row(nxt);
+        nxt = load_raw(a.in, row_words, wrap(r + 3u), col);
         updO(A_, D_, B_, wrap(r + 1u), hidx(k + 3u));      // colour 0, row r+1 (odd)
         updO(D_, C_, A_, wrap(r), hidx(k + 2u));           // colour 1, row r (even)
         store_row(a.out, row_words, wrap(r), col, D_);
         // k % 4 == 3: window (D,A,B,C), r+3 odd
-        C_ = load_row(a.in, row_words, wrap(r + 3u), col);
+        C_ = convert_row(nxt);
+        if (k + 4 < a.H) nxt = load_raw(a.in, row_words, wrap(r + 4u), col);
         updE(B_, A_, C_, wrap(r + 2u), hidx(k + 4u));      // colour 0, row r+2 (even)
         updE(A_, D_, B_, wrap(r + 1u), hidx(k + 3u));      // colour 1, row r+1 (odd)
         store_row(a.out, row_words, wrap(r + 1u), col, A_);

@@ -1,0 +1,100 @@
+"""Self-learning Monte Carlo orchestration over the device kernels of
+csrc/slmc.hip: collect plain-Metropolis training configurations of the
+frustrated plaquette Ising model, fit the three-shell effective pair model on
+the device, and compare independent samples per second of plain Metropolis
+and of SLMC steps (Wolff clusters on the effective couplings with the exact
+Metropolis-Hastings correction).
+
+Independent samples / s = measurements / s / (2 tau_int), tau_int from
+Sokal's automatic windowing (stats.tau_int)."""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import abi, stats
+
+
+def fit_effective_model(sysm, beta, burn=2000, nconfigs=20000, every=1, run_seed=1,
+                        method="normal", adam_iters=2000, adam_lr=0.01):
+    """Returns (theta, J_eff[3], rms residual per site) of E_eff/N = theta0 -
+    sum_k J_k C_k / N fitted on `nconfigs` configurations."""
+    corr = sysm.slmc_collect(beta, burn, nconfigs, every, run_seed)
+    if method == "normal":
+        theta, _ = sysm.slmc_fit_normal()
+    else:
+        theta, _ = sysm.slmc_fit_adam(adam_iters, adam_lr, np.zeros(4))
+    loss, _ = sysm.slmc_loss_grad(theta)
+    return theta, theta[1:4].copy(), float(np.sqrt(loss / nconfigs)), corr
+
+
+def plain_series(sysm, beta, nsweeps, run_seed, sweep0=0, chunk=5000):
+    """Plain 4-colour Metropolis: per-sweep energy and |M|, and wall seconds."""
+    e, m = [], []
+    t0 = time.perf_counter()
+    done = 0
+    while done < nsweeps:
+        k = min(chunk, nsweeps - done)
+        ser = sysm.metropolis(beta, k, run_seed, sweep0=sweep0 + done,
+                              flags=abi.RECORD_ENERGY | abi.RECORD_ORDER, record=True)
+        e += [o.energy for o in ser]
+        m += [abs(o.order[0]) for o in ser]
+        done += k
+    return np.array(e), np.array(m), time.perf_counter() - t0
+
+
+def slmc_series(sysm, J_eff, beta, nsteps, run_seed, flip0=0, chunk=2000):
+    """SLMC steps: per-step energy, |M|, acceptance, cluster sizes, wall seconds."""
+    e, m, acc, cl = [], [], [], []
+    t0 = time.perf_counter()
+    done = 0
+    while done < nsteps:
+        k = min(chunk, nsteps - done)
+        ser, a = sysm.slmc_run(J_eff, beta, k, run_seed, flip0=flip0 + done)
+        e += [o.energy for o in ser]
+        m += [abs(o.order[0]) for o in ser]
+        cl += [o.cluster for o in ser]
+        acc += a.tolist()
+        done += k
+    return np.array(e), np.array(m), np.array(acc), np.array(cl), time.perf_counter() - t0
+
+
+def compare(L=256, J1=1.0, K=0.2, beta=0.40, nconfigs=20000, plain_sweeps=200000,
+            slmc_steps=20000, run_seed=7, baseline_J=None):
+    """Independent samples / s of plain GPU Metropolis and of SLMC on the same
+    model (and optionally of the untrained nearest-neighbour proposal)."""
+    out = {"L": L, "J1": J1, "K": K, "beta": beta}
+    sysm = abi.System.plaquette_ising(L, J1, K)
+    theta, J_eff, rms, _ = fit_effective_model(sysm, beta, nconfigs=nconfigs, run_seed=run_seed)
+    out["J_eff"] = J_eff.tolist()
+    out["fit_rms_per_site"] = rms
+    # plain Metropolis continues from the training chain's state
+    e, m, secs = plain_series(sysm, beta, plain_sweeps, run_seed + 1)
+    out["plain"] = _rates(e, m, secs, "sweeps")
+    e, m, acc, cl, secs = slmc_series(sysm, J_eff, beta, slmc_steps, run_seed + 2)
+    out["slmc"] = _rates(e, m, secs, "steps")
+    out["slmc"]["acceptance"] = float(acc.mean())
+    out["slmc"]["mean_cluster"] = float(cl.mean())
+    if baseline_J is not None:
+        e, m, acc, cl, secs = slmc_series(sysm, np.asarray(baseline_J, float), beta, slmc_steps,
+                                          run_seed + 3)
+        out["untrained"] = _rates(e, m, secs, "steps")
+        out["untrained"]["acceptance"] = float(acc.mean())
+    for obs in ("E", "absM"):
+        out[f"speedup_{obs}"] = out["slmc"][f"indep_per_s_{obs}"] / out["plain"][f"indep_per_s_{obs}"]
+    sysm.close()
+    return out
+
+
+def _rates(e, m, secs, unit):
+    r = {"n": len(e), "unit": unit, "seconds": secs, "per_s": len(e) / secs}
+    for name, x in (("E", e), ("absM", m)):
+        mean, err, tau = stats.mean_error(x)
+        r[f"mean_{name}"] = mean
+        r[f"err_{name}"] = err
+        r[f"tau_{name}"] = tau
+        r[f"indep_per_s_{name}"] = len(x) / secs / (2.0 * tau)
+        # a run shorter than ~50 tau under-estimates tau; say so
+        r[f"tau_reliable_{name}"] = bool(len(x) > 50 * tau)
+    return r
