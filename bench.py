@@ -166,6 +166,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-suite", action="store_true")
     ap.add_argument("--quick-suite", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="only the headline kernel and a same-size copy kernel, for rocprofv3 --pmc")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one MI355X, and the lattice "
@@ -174,6 +176,15 @@ def main():
 
     from universal_classical_montecarlo_amd import abi
     L = args.L
+    if args.profile:
+        # 50 fused sweeps, then 50 copies of the same byte count (read L^2 + write L^2)
+        s = abi.System.ising_square(L, L, 1)
+        s.upload(random_spins(L * L, 7))
+        s.metropolis(0.44, 50, 2026)
+        s.close()
+        print(json.dumps({"profile": "ising_fused", "L": L,
+                          "copy_GBps": abi.measure_copy(L * L, 50)}), flush=True)
+        return
     sysm = abi.System.ising_square(L, L, 1)
     assert sysm.info.schedule == abi.SCHED_FUSED
     sysm.upload(random_spins(L * L, 7))

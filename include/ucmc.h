@@ -267,6 +267,10 @@ int ucmc_slmc_run(ucmc_system* sys, const double* J_eff, double beta, uint64_t r
  * sweep-kernel launches in it. */
 int ucmc_metropolis_bench(ucmc_system* sys, const ucmc_metropolis_params* p, int repeats,
                           double* elapsed_ms, uint64_t* launches);
+/* A plain device copy of `bytes` bytes (read + write), `repeats` times on
+ * device 0: the calibration kernel for HBM byte counters.  gbps = read +
+ * write bytes per second / 1e9. */
+int ucmc_measure_copy(size_t bytes, int repeats, double* gbps);
 
 #ifdef __cplusplus
 }

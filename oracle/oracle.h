@@ -150,6 +150,12 @@ void   orc_vec_wolff_keyed(int N, size_t n, const uint32_t* off, const uint32_t*
                            const uint32_t* edge_id, double* s, double J, double beta,
                            uint64_t run_seed, uint64_t flip0, size_t nflips,
                            uint64_t* size_out, uint8_t* last_cluster);
+/* The same chain with the field rounded to fp32 after every flip, as the
+ * device stores it. */
+void   orc_vec_wolff_keyed_f32(int N, size_t n, const uint32_t* off, const uint32_t* bonds,
+                               const uint32_t* edge_id, double* s, double J, double beta,
+                               uint64_t run_seed, uint64_t flip0, size_t nflips,
+                               uint64_t* size_out, uint8_t* last_cluster);
 
 /* ------------------------------------------------------------------ U(1) -- */
 /* dof/u1.hpp:19-50, example/u1_gauge.hpp:16-85, term/plaquette.hpp:13-48 */

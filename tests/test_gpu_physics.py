@@ -1,0 +1,96 @@
+"""Chain-level checks on the GPU: observables agree with closed forms and
+between independent algorithms within autocorrelation-corrected error bars
+(Sokal windowing), as the reference's own chain tests do
+(test/model_check.cpp:171-222, test/graph_check.cpp:153-192,
+test/su_check.cpp:102-133)."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle_lib import csr_cubic, csr_square
+from universal_classical_montecarlo_amd import abi, slmc, stats
+
+pytestmark = pytest.mark.gpu
+
+# Kaufman's exact finite-torus energies (checked against enumeration at 4x4)
+EXACT = {(64, 0.44): -1.41612127883043, (256, 0.44): -1.40629196919122,
+         (16, 0.3): -0.704532670858768}
+
+
+def metro_energy(L, beta, burn, sweeps, seed):
+    s = abi.System.ising_square(L, L, 1)
+    s.metropolis(beta, burn, seed)
+    ser = s.metropolis(beta, sweeps, seed, sweep0=burn,
+                       flags=abi.RECORD_ENERGY | abi.RECORD_ORDER | abi.RECORD_ACCEPTS, record=True)
+    e = np.array([o.energy for o in ser]) / (L * L)
+    m = np.array([abs(o.order[0]) for o in ser]) / (L * L)
+    acc = np.array([o.accepted for o in ser]) / (L * L)
+    s.close()
+    return e, m, acc
+
+
+@pytest.mark.parametrize("L,beta,sweeps", [(64, 0.44, 60000), (16, 0.3, 40000), (256, 0.44, 20000)])
+def test_ising_metropolis_exact_energy(gpu, L, beta, sweeps):
+    e, m, acc = metro_energy(L, beta, 2000, sweeps, 2026)
+    mean, err, tau = stats.mean_error(e)
+    print(f"L={L} beta={beta}: E/site {mean:.5f} +- {err:.5f} (tau {tau:.1f}), exact "
+          f"{EXACT[(L, beta)]:.5f}, |M| {m.mean():.4f}, acceptance {acc.mean():.4f}", flush=True)
+    assert abs(mean - EXACT[(L, beta)]) < 4 * err + 2e-4
+
+
+def test_ising_wolff_exact_energy(gpu):
+    L, beta = 64, 0.44
+    s = abi.System.ising_square(L, L, 1)
+    s.wolff(beta, 2000, 5)
+    sizes, ser = s.wolff(beta, 40000, 5, flip0=2000, record=True)
+    e = np.array([o.energy for o in ser]) / (L * L)
+    mean, err, tau = stats.mean_error(e)
+    print(f"Wolff L=64: E/site {mean:.5f} +- {err:.5f}, mean cluster {sizes.mean():.1f}", flush=True)
+    assert abs(mean - EXACT[(L, beta)]) < 4 * err + 2e-4
+    s.close()
+
+
+def test_heisenberg_metropolis_vs_wolff(gpu):
+    """model_check.cpp:198-222 on an 8^3 cubic lattice at beta = 0.5."""
+    off, bonds = csr_cubic(8)
+    a = abi.System.vector(3, off, bonds, 1.0)
+    a.metropolis(0.5, 2000, 1, proposal=abi.PROP_ROTATION, width=0.8)
+    ser = a.metropolis(0.5, 20000, 1, sweep0=2000, proposal=abi.PROP_ROTATION, width=0.8,
+                       flags=abi.RECORD_ENERGY, record=True)
+    em = stats.mean_error(np.array([o.energy for o in ser]) / 512)
+    b = abi.System.vector(3, off, bonds, 1.0)
+    b.wolff(0.5, 2000, 2)
+    _, ws = b.wolff(0.5, 40000, 2, flip0=2000, record=True)
+    ew = stats.mean_error(np.array([o.energy for o in ws]) / 512)
+    print(f"Heisenberg 8^3 beta=0.5: Metropolis {em[0]:.5f}+-{em[1]:.5f} Wolff {ew[0]:.5f}+-{ew[1]:.5f}")
+    assert abs(em[0] - ew[0]) < 4 * math.hypot(em[1], ew[1]) + 1e-3
+    a.close()
+    b.close()
+
+
+def test_su3_plaquette_beta_5p7(gpu):
+    """8^4 SU(3) at beta = 5.7 (set once, in the model), 4 hits: the mean
+    plaquette is near the literature's 0.549 (small-volume effects included)."""
+    g = abi.System.sun_gauge(3, 8, 5.7)
+    g.metropolis(1.0, 200, 1, proposal=abi.PROP_SU_SHIFT, width=0.2, hits=4, reunitarize_every=10)
+    ser = g.metropolis(1.0, 200, 1, sweep0=200, proposal=abi.PROP_SU_SHIFT, width=0.2, hits=4,
+                       flags=abi.RECORD_ENERGY | abi.RECORD_ACCEPTS, record=True,
+                       reunitarize_every=10)
+    p = np.array([o.plaquette for o in ser])
+    print(f"SU(3) 8^4 beta=5.7 plaquette {p.mean():.4f}", flush=True)
+    assert abs(p.mean() - 0.549) < 0.02
+    g.close()
+
+
+def test_slmc_detailed_balance_and_gain(gpu):
+    """SLMC samples the exact plaquette model (its energy agrees with plain
+    Metropolis), and the trained effective couplings raise the acceptance over
+    the untrained nearest-neighbour proposal (J_eff = (J1, 0, 0))."""
+    out = slmc.compare(L=32, beta=0.40, nconfigs=4000, plain_sweeps=40000, slmc_steps=6000,
+                       baseline_J=[1.0, 0.0, 0.0])
+    p, s, u = out["plain"], out["slmc"], out["untrained"]
+    print(out, flush=True)
+    assert abs(p["mean_E"] - s["mean_E"]) < 4 * math.hypot(p["err_E"], s["err_E"]) + 1.0
+    assert s["acceptance"] > u["acceptance"]
+    assert out["J_eff"][0] > 0.5

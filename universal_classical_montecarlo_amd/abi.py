@@ -38,6 +38,7 @@ EXPORTED = [
     "ucmc_observe", "ucmc_reunitarize", "ucmc_metropolis_bench", "ucmc_pt_create",
     "ucmc_pt_run", "ucmc_pt_stats", "ucmc_pt_autotune", "ucmc_slmc_collect",
     "ucmc_slmc_fit_normal", "ucmc_slmc_loss_grad", "ucmc_slmc_fit_adam", "ucmc_slmc_run",
+    "ucmc_measure_copy",
 ]
 
 
@@ -128,6 +129,7 @@ def lib():
     L.ucmc_slmc_fit_adam.argtypes = [vp, C.c_int, C.c_double, C.c_void_p, C.POINTER(C.c_double)]
     L.ucmc_slmc_run.argtypes = [vp, C.c_void_p, C.c_double, C.c_uint64, C.c_uint64, C.c_uint64,
                                 C.c_void_p, C.c_void_p]
+    L.ucmc_measure_copy.argtypes = [C.c_size_t, C.c_int, C.POINTER(C.c_double)]
     L.ucmc_pt_autotune.argtypes = [vp, C.c_double, C.POINTER(C.c_uint32), C.POINTER(C.c_double)]
     L.ucmc_metropolis_bench.argtypes = [vp, C.POINTER(MetropolisParams), C.c_int,
                                         C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
@@ -144,6 +146,13 @@ def device_count():
     n = C.c_int(0)
     check(lib().ucmc_device_count(C.byref(n)))
     return n.value
+
+
+def measure_copy(nbytes, repeats=20):
+    """GB/s (read + write) of a plain device copy of nbytes."""
+    g = C.c_double()
+    check(lib().ucmc_measure_copy(int(nbytes), int(repeats), C.byref(g)))
+    return g.value
 
 
 def _ptr(a):

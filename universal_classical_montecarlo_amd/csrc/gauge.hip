@@ -317,7 +317,7 @@ __global__ __launch_bounds__(256) void sun_colour(SunArgs a)
             }
             const float dE = a.beta_over_n * retr_mul<N>(D, A);
             const double log_p = -a.driver_beta * static_cast<double>(dE) + 0.0;
-            const bool acc = log_p >= 0.0 || log(r.uni()) < log_p;
+            const bool acc = log_p >= 0.0 || accept_draw(r.next(), log_p);
             if (acc) {
                 U = Up;
                 ++nacc;
@@ -396,7 +396,7 @@ __global__ __launch_bounds__(256) void u1_colour(U1Args a)
             // beta sum_p [cos(t + phi) - cos(t' + phi)] = beta Re((e^{it} - e^{it'}) W)
             const float dE = a.beta * ((c0 - c1) * wr - (s0 - s1) * wi);
             const double log_p = -a.driver_beta * static_cast<double>(dE) + 0.0;
-            const bool acc = log_p >= 0.0 || log(r.uni()) < log_p;
+            const bool acc = log_p >= 0.0 || accept_draw(r.next(), log_p);
             if (acc) {
                 t = tp;
                 ++nacc;

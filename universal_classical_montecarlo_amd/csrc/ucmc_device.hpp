@@ -64,6 +64,20 @@ struct Pcg {
     __device__ __forceinline__ double uni() { return static_cast<double>(next()) * 0x1.0p-32; }
 };
 
+// The accept test of driver.hpp:84, log(u) < log_p with u = k 2^-32: decided
+// in fp32 when the two sides are more than 1e-3 apart (the fp32 logarithm is
+// within ~3e-6 of the double one for every k, so the outcome is the same), and
+// with the double expression otherwise.  k = 0 gives log 0 = -inf (accept);
+// a NaN log_p rejects, as in the reference.
+__device__ __forceinline__ bool accept_draw(std::uint32_t k, double log_p)
+{
+    const float lf = __logf(static_cast<float>(k) * 0x1.0p-32f);
+    const float lp = static_cast<float>(log_p);
+    if (lf < lp - 1e-3f) return true;
+    if (lf > lp + 1e-3f) return false;
+    return log(static_cast<double>(k) * 0x1.0p-32) < log_p;
+}
+
 // Wave-wide shifts by one lane (DPP wave_shr:1 / wave_shl:1).  Lane 0
 // (resp. 63) receives `edge`.
 __device__ __forceinline__ std::uint32_t from_prev_lane(std::uint32_t v, std::uint32_t edge)

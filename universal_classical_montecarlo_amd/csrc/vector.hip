@@ -159,7 +159,7 @@ __global__ __launch_bounds__(256) void vector_colour(VecArgs a)
         for (int k = 0; k < N; ++k) d += (p[k] - s[k]) * h[k];
         const float dE = -a.J * d;
         const double log_p = -a.beta * static_cast<double>(dE) + 0.0;
-        acc = log_p >= 0.0 || log(r.uni()) < log_p;
+        acc = log_p >= 0.0 || accept_draw(r.next(), log_p);
         if (acc) {
             // keep |S| = 1 against fp32 drift
             float nn = 0.f;

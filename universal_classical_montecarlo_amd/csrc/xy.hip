@@ -73,7 +73,7 @@ __global__ __launch_bounds__(256) void xy_colour(XyArgs a)
         }
         const float dE = -a.J * d;
         const double log_p = -a.beta * static_cast<double>(dE) + 0.0;
-        acc = log_p >= 0.0 || log(r.uni()) < log_p;
+        acc = log_p >= 0.0 || accept_draw(r.next(), log_p);
         if (acc) a.t[i] = tp;
         if constexpr (TRACE) {
             a.trace_dE[i] = dE;
@@ -197,7 +197,7 @@ __global__ __launch_bounds__(256) void pt_sweep(PtArgs a)
             const float tp = u1_shift(a.width, ti, rg.next());
             const float dE = xy_dE(a.J, tp, ti, nb);
             const double log_p = -beta * static_cast<double>(dE) + 0.0;
-            if (log_p >= 0.0 || log(rg.uni()) < log_p) tl[ly * W + lx] = tp;
+            if (log_p >= 0.0 || accept_draw(rg.next(), log_p)) tl[ly * W + lx] = tp;
         }
     };
     pass(0, 1, W - 1);  // colour 0 on tile + 2 rings
