@@ -35,13 +35,22 @@
  *      plaquette Ising int8_t [L * L]
  *    The device stores int8 (Ising) or fp32 (everything else).
  *
- * Random numbers.  Site (or link) i in sweep s draws, in the reference's order
- * (proposal draws, then the accept draw only when log_p < 0, per hit), from
- *   Pcg32(ucmc_key(run_seed, sweep0 + s), i)           -- rng.hpp:14-40
- * and Wolff flip f draws its axis and seed site from Pcg32(k_f, 0) and the
- * activation uniform of undirected edge e from Pcg32(k_f, 1 + e), with
- * k_f = ucmc_key(run_seed, 2^62 + flip0 + f).  ucmc_key is the
- * (counter+1)-th output of SplitMix64 seeded with run_seed.
+ * Random numbers are counter-based: Philox4x32-10 (Salmon et al., SC'11) words
+ * philox(counter, key) with key = ucmc_key(run_seed, sweep0 + s) for sweep s;
+ * ucmc_key is the (counter+1)-th output of SplitMix64 seeded with run_seed.
+ *  - Ising (and plaquette Ising) site i = 512 r + 8 q + j: the 32-bit uniform
+ *    compared with the integer threshold has its high 16 bits from
+ *    philox((q, r, 0, 0)) and its low 16 bits from philox((q, r, 1, 0)), word
+ *    (j & 1) * 2 + (j >> 2), half (j >> 1) & 1.
+ *  - Other models: site (or link) i draws, in the reference's order (proposal
+ *    draws, then the accept draw only when log_p < 0, per hit), the words of
+ *    philox((i_lo, i_hi, 2, n), key), n = 0, 1, ..., with uni() = word * 2^-32
+ *    as rng.hpp:24-27.
+ *  - Wolff flip f, k_f = ucmc_key(run_seed, 2^62 + flip0 + f): axis and seed
+ *    site from the stream philox((0, 0, 3, n), k_f); undirected edge e
+ *    activates with word e & 3 of philox((e >> 2, 0, 4, 0), k_f).
+ *  - Parallel-tempering swap of ladder position k in sweep s: first word of
+ *    philox((k, 0, 5, 0), ucmc_key(ucmc_key(run_seed, 2^60), s)).
  */
 #ifndef UCMC_H
 #define UCMC_H
@@ -254,7 +263,7 @@ int ucmc_slmc_loss_grad(ucmc_system* sys, const double* theta, double* out);
 /* `iters` Adam steps (lr, 0.9, 0.999) on the mean loss, from theta (in/out). */
 int ucmc_slmc_fit_adam(ucmc_system* sys, int iters, double lr, double* theta, double* mean_loss);
 /* nflips SLMC steps: Wolff cluster on the effective couplings J_eff[3] (the
- * growth uses keyed edge uniforms, Pcg32(k_f, 1 + 6 owner + slot)), accepted
+ * growth uses the keyed edge uniforms of edge 6 owner + slot), accepted
  * with min(1, exp(-beta (dE_true - dE_eff))) from the control stream's second
  * draw (k_f = key(run_seed, 3 2^61 + flip0 + f)).  series (nullable): true
  * energy, magnetisation, cluster size and 0/1 acceptance per step. */

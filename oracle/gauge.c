@@ -6,6 +6,7 @@
  */
 #define _GNU_SOURCE
 #include "oracle.h"
+#include "philox.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -138,7 +139,7 @@ void orc_u1_metropolis_keyed(size_t L, double* theta, double model_beta, double 
             for (size_t i = 0; i < h.sites; ++i) {
                 if (link_colour(L, i) != c) continue;
                 orc_pcg32 rng;
-                orc_pcg32_init(&rng, key, i);
+                orc_stream_init(&rng, key, i, ORC_TAG_SITE);
                 const double cand = orc_u1_shift(width, theta[i], &rng);
                 const double dE = orc_u1_energy_diff_at(h.cells, h.inc_off, h.inc_cell,
                                                         h.inc_which, theta, model_beta, i, cand);
@@ -368,7 +369,7 @@ void orc_sun_metropolis_keyed(int N, size_t L, double* links, double model_beta,
             for (size_t i = 0; i < h.sites; ++i) {
                 if (link_colour(L, i) != c) continue;
                 orc_pcg32 rng;
-                orc_pcg32_init(&rng, key, i);
+                orc_stream_init(&rng, key, i, ORC_TAG_SITE);
                 for (int hit = 0; hit < hits; ++hit) {
                     orc_sun_shift(N, width, links + i * stride, &rng, cand);
                     const double dE = orc_sun_energy_diff_at(N, h.cells, h.inc_off, h.inc_cell,
@@ -449,7 +450,7 @@ void orc_xy_metropolis_keyed(size_t n, const uint32_t* off, const uint32_t* bond
             for (size_t i = 0; i < n; ++i) {
                 if (colour[i] != c) continue;
                 orc_pcg32 rng;
-                orc_pcg32_init(&rng, key, i);
+                orc_stream_init(&rng, key, i, ORC_TAG_SITE);
                 const double cand = orc_u1_shift(width, t[i], &rng);
                 const double dE = orc_xy_energy_diff_at(off, bonds, t, J, i, cand);
                 const double log_p = -beta * dE + 0.0;
@@ -511,11 +512,10 @@ void orc_fpi_metropolis_keyed(size_t L, int8_t* s, double J1, double K, double b
             for (size_t i = 0; i < n; ++i) {
                 const size_t x = i % L, y = i / L;
                 if ((int)((x & 1) + 2 * (y & 1)) != c) continue;
-                orc_pcg32 rng;
-                orc_pcg32_init(&rng, key, i);
                 const double dE = orc_fpi_energy_diff_flip(L, s, J1, K, i);
                 const double log_p = -beta * dE + 0.0;
-                if (log_p >= 0.0 || log(orc_pcg32_uni(&rng)) < log_p) {
+                if (log_p >= 0.0 ||
+                    log((double)orc_ising_u32_inline(key, (uint32_t)i) * 0x1.0p-32) < log_p) {
                     s[i] = (int8_t)(-s[i]);
                     ++acc;
                 }
@@ -567,7 +567,7 @@ void orc_slmc_keyed(size_t L, int8_t* s, double J1, double K, const double* Jeff
     for (size_t f = 0; f < nflips; ++f) {
         const uint64_t key = orc_key(run_seed, (UINT64_C(3) << 61) + flip0 + f);
         orc_pcg32 ctl;
-        orc_pcg32_init(&ctl, key, 0);
+        orc_stream_init(&ctl, key, 0, ORC_TAG_WOLFF_CTRL);
         const size_t seed = (size_t)(orc_pcg32_uni(&ctl) * (double)n);
         memset(in, 0, n);
         size_t head = 0, tail = 0;
@@ -580,10 +580,9 @@ void orc_slmc_keyed(size_t L, int8_t* s, double J1, double K, const double* Jeff
                 const uint32_t j = (uint32_t)fpi_id(L, x + 2 * L + (size_t)(long)dx[d], y + 2 * L + (size_t)(long)dy[d]);
                 if (in[j]) continue;
                 const uint64_t owner = fwd[d] ? i : j;
-                orc_pcg32 er;
-                orc_pcg32_init(&er, key, 1 + 6 * owner + (uint64_t)slot[d]);
                 const double dE = 2.0 * Jeff[shell[d]] * (double)(s[i] * s[j]);
-                if (orc_pcg32_uni(&er) < 1.0 - exp(-beta * dE)) {
+                const uint32_t e = (uint32_t)(6 * owner + (uint64_t)slot[d]);
+                if ((double)orc_edge_u32_inline(key, e) * 0x1.0p-32 < 1.0 - exp(-beta * dE)) {
                     in[j] = 1;
                     q[tail++] = j;
                 }

@@ -4,6 +4,7 @@
  * of driver.hpp:73-87 (Metropolis) and driver.hpp:148-206 (Wolff).
  */
 #include "oracle.h"
+#include "philox.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -121,11 +122,20 @@ void orc_ising_metropolis_seq(size_t n, const uint32_t* off, const uint32_t* bon
 
 /* The GPU schedule: colour classes in increasing colour, sites of a class in
  * increasing index (the order inside a class does not matter, no two sites of a
- * class are bonded), each site drawing from its own Pcg32(key, site). */
+ * class are bonded), site i drawing the uniform orc_ising_u32(key, i). */
 void orc_ising_metropolis_keyed(size_t n, const uint32_t* off, const uint32_t* bonds,
                                 int8_t* s, int32_t J, double beta, const uint8_t* colour,
                                 int ncolours, uint64_t run_seed, uint64_t sweep0,
                                 size_t nsweeps, uint64_t* accepts_out)
+{
+    orc_ising_metropolis_keyed_series(n, off, bonds, s, J, beta, colour, ncolours, run_seed,
+                                      sweep0, nsweeps, accepts_out, NULL);
+}
+
+void orc_ising_metropolis_keyed_series(size_t n, const uint32_t* off, const uint32_t* bonds,
+                                       int8_t* s, int32_t J, double beta, const uint8_t* colour,
+                                       int ncolours, uint64_t run_seed, uint64_t sweep0,
+                                       size_t nsweeps, uint64_t* accepts_out, double* e_out)
 {
     for (size_t k = 0; k < nsweeps; ++k) {
         const uint64_t key = orc_key(run_seed, sweep0 + k);
@@ -133,17 +143,17 @@ void orc_ising_metropolis_keyed(size_t n, const uint32_t* off, const uint32_t* b
         for (int c = 0; c < ncolours; ++c)
             for (size_t i = 0; i < n; ++i) {
                 if (colour[i] != c) continue;
-                orc_pcg32 rng;
-                orc_pcg32_init(&rng, key, i);
                 const int8_t cand = (int8_t)(-s[i]);
                 const double dE = orc_ising_energy_diff_at(off, bonds, s, J, i, cand);
                 const double log_p = -beta * dE + 0.0;
-                if (log_p >= 0.0 || log(orc_pcg32_uni(&rng)) < log_p) {
+                if (log_p >= 0.0 ||
+                    log((double)orc_ising_u32_inline(key, (uint32_t)i) * 0x1.0p-32) < log_p) {
                     s[i] = cand;
                     ++acc;
                 }
             }
         if (accepts_out) accepts_out[k] = acc;
+        if (e_out) e_out[k] = total_energy_ising(n, off, bonds, s, J);
     }
 }
 
@@ -200,8 +210,8 @@ void orc_ising_wolff_seq(size_t n, const uint32_t* off, const uint32_t* bonds, i
 }
 
 /* Keyed Wolff: the cluster is the connected component of the seed in the graph
- * of activated edges; edge e is activated iff the first draw of
- * Pcg32(k_f, 1 + e) is below 1 - exp(-beta dE), dE = energy_diff_bond on the
+ * of activated edges; edge e is activated iff orc_edge_u32(k_f, e) * 2^-32 is
+ * below 1 - exp(-beta dE), dE = energy_diff_bond on the
  * pre-flip spins (driver.hpp:186-198).  This is the distribution the reference's
  * depth-first growth samples, made independent of traversal order. */
 void orc_ising_wolff_keyed(size_t n, const uint32_t* off, const uint32_t* bonds,
@@ -214,7 +224,7 @@ void orc_ising_wolff_keyed(size_t n, const uint32_t* off, const uint32_t* bonds,
     for (size_t f = 0; f < nflips; ++f) {
         const uint64_t key = orc_key(run_seed, ORC_WOLFF_COUNTER_BASE + flip0 + f);
         orc_pcg32 ctl;
-        orc_pcg32_init(&ctl, key, 0);
+        orc_stream_init(&ctl, key, 0, ORC_TAG_WOLFF_CTRL);
         const size_t seed = (size_t)(orc_pcg32_uni(&ctl) * (double)n);
         memset(in, 0, n);
         size_t head = 0, tail = 0;
@@ -227,9 +237,7 @@ void orc_ising_wolff_keyed(size_t n, const uint32_t* off, const uint32_t* bonds,
                 if (in[j]) continue;
                 const int8_t nb = (int8_t)(-s[j]);
                 const double dE = orc_ising_energy_diff_bond(s, J, i, j, nb);
-                orc_pcg32 er;
-                orc_pcg32_init(&er, key, 1 + (uint64_t)edge_id[a]);
-                if (orc_pcg32_uni(&er) < 1.0 - exp(-beta * dE)) {
+                if ((double)orc_edge_u32_inline(key, edge_id[a]) * 0x1.0p-32 < 1.0 - exp(-beta * dE)) {
                     in[j] = 1;
                     queue[tail++] = j;
                 }

@@ -13,7 +13,8 @@
  * tests/golden/).  The "keyed" drivers replay the GPU schedule: the reference's
  * own per-site update (proposal -> energy_diff_at -> accept rule -> update),
  * applied colour class by colour class, with the uniforms of site i in sweep s
- * drawn from Pcg32(orc_key(run_seed, s), i).
+ * drawn from the counter-based Philox4x32-10 streams of philox.h under the key
+ * orc_key(run_seed, s).
  *
  * Build: gcc -std=c11 -O2 -ffp-contract=off -fPIC -shared (see oracle/Makefile).
  * -ffp-contract=off keeps a*b+c as two roundings, as in the reference's x86-64
@@ -30,11 +31,25 @@ extern "C" {
 #endif
 
 /* ------------------------------------------------------------------ RNG -- */
-/* rng.hpp:8-44 */
-typedef struct { uint64_t state, inc; } orc_pcg32;
+/* rng.hpp:8-44.  The same handle also serves as a keyed Philox stream
+ * (orc_stream_init), so the proposal restatements below draw from either. */
+typedef struct {
+    uint64_t state, inc;     /* Pcg32 */
+    uint64_t key;            /* keyed stream: philox((id_lo, id_hi, tag, n), key) */
+    uint32_t ctr[4], buf[4];
+    uint32_t pos, philox;    /* philox != 0: keyed stream */
+} orc_pcg32;
 void     orc_pcg32_init(orc_pcg32* r, uint64_t seed, uint64_t stream);
 uint32_t orc_pcg32_next(orc_pcg32* r);
 double   orc_pcg32_uni(orc_pcg32* r);
+size_t   orc_rng_size(void);
+/* Keyed stream of the GPU chains: words of philox((id_lo, id_hi, tag, n), key),
+ * n = 0, 1, ... (philox.h). */
+void     orc_stream_init(orc_pcg32* r, uint64_t key, uint64_t id, uint32_t tag);
+/* Philox4x32-10 of one counter, and the Ising site / Wolff edge uniforms. */
+void     orc_philox(const uint32_t ctr[4], uint64_t key, uint32_t out[4]);
+uint32_t orc_ising_u32(uint64_t key, uint32_t site);
+uint32_t orc_edge_u32(uint64_t key, uint32_t edge);
 /* First n outputs of Pcg32(seed, stream).next(). */
 void     orc_pcg32_block(uint64_t seed, uint64_t stream, size_t n, uint32_t* out);
 /* Counter-based key: the (counter+1)-th output of SplitMix64 seeded with
@@ -106,14 +121,21 @@ void orc_ising_metropolis_keyed(size_t n, const uint32_t* off, const uint32_t* b
                                 int8_t* s, int32_t J, double beta, const uint8_t* colour,
                                 int ncolours, uint64_t run_seed, uint64_t sweep0,
                                 size_t nsweeps, uint64_t* accepts_out);
+/* The same, also recording the total energy after every sweep (e_out may be
+ * NULL). */
+void orc_ising_metropolis_keyed_series(size_t n, const uint32_t* off, const uint32_t* bonds,
+                                       int8_t* s, int32_t J, double beta, const uint8_t* colour,
+                                       int ncolours, uint64_t run_seed, uint64_t sweep0,
+                                       size_t nsweeps, uint64_t* accepts_out, double* e_out);
 /* Reference chain: Wolff<Ising, FixedAxis, Pcg32>; burn flips then nflips flips
  * with the total energy recorded after each (driver.hpp:130-240). */
 void orc_ising_wolff_seq(size_t n, const uint32_t* off, const uint32_t* bonds, int8_t* s,
                          int32_t J, double beta, uint64_t seed, uint64_t stream, size_t burn,
                          size_t nflips, double* e_out, uint64_t* size_out);
-/* Keyed Wolff replay: flip f uses control stream Pcg32(k_f, 0) for the seed
- * site and Pcg32(k_f, 1 + e) for undirected edge e, k_f =
- * orc_key(run_seed, ORC_WOLFF_COUNTER_BASE + f).  size_out[k] = cluster size. */
+/* Keyed Wolff replay: flip f draws the seed site from the keyed stream
+ * (k_f, 0, ORC_TAG_WOLFF_CTRL) and activates undirected edge e with
+ * orc_edge_u32(k_f, e), k_f = orc_key(run_seed, ORC_WOLFF_COUNTER_BASE + f).
+ * size_out[k] = cluster size. */
 void orc_ising_wolff_keyed(size_t n, const uint32_t* off, const uint32_t* bonds,
                            const uint32_t* edge_id, int8_t* s, int32_t J, double beta,
                            uint64_t run_seed, uint64_t flip0, size_t nflips,

@@ -1,0 +1,108 @@
+/*
+ * philox.h -- the counter-based uniforms of the GPU chains, restated for the
+ * CPU replays (test infrastructure only; see oracle.h).  Header-only so that
+ * the C restatement and the shim over the compiled reference share one copy.
+ *
+ * Philox4x32-10: Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy
+ * as 1, 2, 3", SC'11.  Round: (hi0, lo0) = M0 * c0, (hi1, lo1) = M1 * c2,
+ * c' = (hi1 ^ c1 ^ k0, lo1, hi0 ^ c3 ^ k1, lo0); the key is bumped by
+ * (W0, W1) before every round but the first.
+ *
+ * Uses (the device side is ucmc_device.hpp):
+ *   Ising site uniform  orc_ising_u32(key, i): i = 512 r + 8 q + j, high 16
+ *     bits from philox((q, r, TAG_ISING_HI, 0)), low 16 bits from
+ *     philox((q, r, TAG_ISING_LO, 0)); word (j & 1) * 2 + (j >> 2), half
+ *     (j >> 1) & 1.
+ *   Per-site streams    philox((id_lo, id_hi, tag, n)) for n = 0, 1, ..., words
+ *     in order x, y, z, w.
+ *   Wolff edge uniform  word e & 3 of philox((e >> 2, 0, TAG_WOLFF_EDGE, 0)).
+ */
+#ifndef UCMC_ORACLE_PHILOX_H
+#define UCMC_ORACLE_PHILOX_H
+
+#include <stdint.h>
+
+#define ORC_TAG_ISING_HI   0u
+#define ORC_TAG_ISING_LO   1u
+#define ORC_TAG_SITE       2u
+#define ORC_TAG_WOLFF_CTRL 3u
+#define ORC_TAG_WOLFF_EDGE 4u
+#define ORC_TAG_SWAP       5u
+
+static inline void orc_philox4x32_10(const uint32_t ctr[4], uint64_t key, uint32_t out[4])
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = (uint32_t)key, k1 = (uint32_t)(key >> 32);
+    for (int r = 0; r < 10; ++r) {
+        if (r) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+static inline uint32_t orc_ising_half(const uint32_t w[4], uint32_t j)
+{
+    const uint32_t v = w[(j & 1u) * 2u + (j >> 2)];
+    return ((j >> 1) & 1u) ? (v >> 16) : (v & 0xFFFFu);
+}
+
+static inline uint32_t orc_ising_u32_inline(uint64_t key, uint32_t i)
+{
+    const uint32_t q = (i >> 3) & 63u, r = i >> 9, j = i & 7u;
+    uint32_t c[4] = {q, r, ORC_TAG_ISING_HI, 0u}, hi[4], lo[4];
+    orc_philox4x32_10(c, key, hi);
+    c[2] = ORC_TAG_ISING_LO;
+    orc_philox4x32_10(c, key, lo);
+    return (orc_ising_half(hi, j) << 16) | orc_ising_half(lo, j);
+}
+
+static inline uint32_t orc_edge_u32_inline(uint64_t key, uint32_t e)
+{
+    const uint32_t c[4] = {e >> 2, 0u, ORC_TAG_WOLFF_EDGE, 0u};
+    uint32_t w[4];
+    orc_philox4x32_10(c, key, w);
+    return w[e & 3u];
+}
+
+/* A sequential keyed stream (words in counter order). */
+typedef struct {
+    uint64_t key;
+    uint32_t ctr[4];
+    uint32_t buf[4];
+    uint32_t pos;
+} orc_philox_stream;
+
+static inline void orc_philox_stream_init(orc_philox_stream* s, uint64_t key, uint64_t id, uint32_t tag)
+{
+    s->key = key;
+    s->ctr[0] = (uint32_t)id;
+    s->ctr[1] = (uint32_t)(id >> 32);
+    s->ctr[2] = tag;
+    s->ctr[3] = 0u;
+    s->pos = 4u;
+}
+
+static inline uint32_t orc_philox_stream_next(orc_philox_stream* s)
+{
+    if (s->pos == 4u) {
+        orc_philox4x32_10(s->ctr, s->key, s->buf);
+        s->ctr[3] += 1u;
+        s->pos = 0u;
+    }
+    return s->buf[s->pos++];
+}
+
+#endif

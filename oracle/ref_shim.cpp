@@ -10,7 +10,9 @@
 //    (driver.hpp Metropolis / Wolff) on its own models;
 //  * the keyed colour-order replay, which applies the reference's per-site
 //    update (proposal.sample -> model.energy_diff_at -> accept rule ->
-//    model.update) colour class by colour class with Pcg32(key, site) streams.
+//    model.update) colour class by colour class, with the counter-based
+//    uniforms of the GPU chains (philox.h: the Ising site uniform, per-site
+//    Philox streams for the other models).
 
 #include <array>
 #include <cmath>
@@ -33,6 +35,8 @@
 #include <lattice/pair_graph.hpp>
 #include <lattice/square.hpp>
 #include <rng.hpp>
+
+#include "philox.h"
 
 namespace {
 
@@ -100,8 +104,26 @@ double half_energy(const M& m)
     return 0.5 * sum;
 }
 
-/// One keyed colour-order sweep over any ModelLike with proposal Q.
-template <class M, class Q, class ColourFn>
+/// The keyed per-site stream of the GPU chains, with the uni() the reference's
+/// proposals and accept rule call (rng.hpp:24-27 semantics).
+struct KeyedStream
+{
+    orc_philox_stream s;
+    KeyedStream(std::uint64_t key, std::uint64_t id) { orc_philox_stream_init(&s, key, id, ORC_TAG_SITE); }
+    double uni() { return static_cast<double>(orc_philox_stream_next(&s)) * 0x1.0p-32; }
+};
+
+/// The single Metropolis uniform of an Ising site (Flip draws nothing).
+struct IsingUniform
+{
+    std::uint32_t u;
+    IsingUniform(std::uint64_t key, std::uint64_t i) : u(orc_ising_u32_inline(key, static_cast<std::uint32_t>(i))) {}
+    double uni() { return static_cast<double>(u) * 0x1.0p-32; }
+};
+
+/// One keyed colour-order sweep over any ModelLike with proposal Q, site i
+/// drawing from Rng(key, i).
+template <class Rng = KeyedStream, class M, class Q, class ColourFn>
 std::uint64_t keyed_sweep(M& model, const Q& q, double beta, std::uint64_t key, int ncolours,
                           ColourFn colour_of, double* trace_dE)
 {
@@ -111,7 +133,7 @@ std::uint64_t keyed_sweep(M& model, const Q& q, double beta, std::uint64_t key, 
     for (int c = 0; c < ncolours; ++c)
         for (std::size_t i = 0; i < model.sites(); ++i) {
             if (colour_of(i) != c) continue;
-            Pcg32 rng(key, i);
+            Rng rng(key, i);
             const T* cur = model.data() + i * M::size;
             q.sample(cur, rng, cand.data());
             const double dE = model.energy_diff_at(i, cand.data());
@@ -272,7 +294,7 @@ void ref_ising_keyed(std::size_t n, std::size_t ne, const std::uint32_t* ei,
     Ising m(graph_from(n, ne, ei, ej), J);
     for (std::size_t i = 0; i < n; ++i) m.update(i, spins + i);
     for (std::size_t k = 0; k < nsweeps; ++k) {
-        const auto acc = keyed_sweep(m, Flip{}, beta, key_of(run_seed, sweep0 + k), ncolours,
+        const auto acc = keyed_sweep<IsingUniform>(m, Flip{}, beta, key_of(run_seed, sweep0 + k), ncolours,
                                      [&](std::size_t i) { return int(colour[i]); }, nullptr);
         if (acc_out) acc_out[k] = acc;
     }
@@ -580,7 +602,7 @@ void ref_sun_metropolis_seq(int N, std::size_t L, double* links, double model_be
     if (N == 3) run(std::integral_constant<std::size_t, 3>{});
 }
 
-// Keyed replay with `hits` hits per link: the same per-link Pcg32 stream feeds
+// Keyed replay with `hits` hits per link: the same per-link keyed stream feeds
 // every hit of the link.
 void ref_sun_keyed(int N, std::size_t L, double* links, double model_beta, double driver_beta,
                    double width, int hits, std::uint64_t run_seed, std::uint64_t sweep0,
@@ -597,7 +619,7 @@ void ref_sun_keyed(int N, std::size_t L, double* links, double model_beta, doubl
             for (int c = 0; c < 8; ++c)
                 for (std::size_t i = 0; i < g.sites(); ++i) {
                     if (link_colour(L, i) != c) continue;
-                    Pcg32 rng(key, i);
+                    KeyedStream rng(key, i);
                     for (int h = 0; h < hits; ++h) {
                         SULink<NN> cand;
                         q.sample(g.data() + i, rng, &cand);

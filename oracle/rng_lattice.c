@@ -3,6 +3,7 @@
  * oracle.h).
  */
 #include "oracle.h"
+#include "philox.h"
 
 #include <stdlib.h>
 #include <string.h>
@@ -10,6 +11,7 @@
 /* rng.hpp:14-20: state = 0, inc = (stream << 1) | 1, step, state += seed, step. */
 void orc_pcg32_init(orc_pcg32* r, uint64_t seed, uint64_t stream)
 {
+    memset(r, 0, sizeof *r);
     r->state = 0;
     r->inc   = (stream << 1u) | 1u;
     (void)orc_pcg32_next(r);
@@ -17,9 +19,30 @@ void orc_pcg32_init(orc_pcg32* r, uint64_t seed, uint64_t stream)
     (void)orc_pcg32_next(r);
 }
 
-/* rng.hpp:31-40: LCG step, XSH-RR output of the old state. */
+void orc_stream_init(orc_pcg32* r, uint64_t key, uint64_t id, uint32_t tag)
+{
+    memset(r, 0, sizeof *r);
+    r->philox = 1;
+    r->key = key;
+    r->ctr[0] = (uint32_t)id;
+    r->ctr[1] = (uint32_t)(id >> 32);
+    r->ctr[2] = tag;
+    r->ctr[3] = 0;
+    r->pos = 4;
+}
+
+/* rng.hpp:31-40: LCG step, XSH-RR output of the old state (keyed streams:
+ * the next Philox word). */
 uint32_t orc_pcg32_next(orc_pcg32* r)
 {
+    if (r->philox) {
+        if (r->pos == 4) {
+            orc_philox4x32_10(r->ctr, r->key, r->buf);
+            r->ctr[3] += 1;
+            r->pos = 0;
+        }
+        return r->buf[r->pos++];
+    }
     const uint64_t old = r->state;
     r->state = old * UINT64_C(6364136223846793005) + r->inc;
     const uint32_t xs  = (uint32_t)(((old >> 18u) ^ old) >> 27u);
@@ -32,6 +55,16 @@ double orc_pcg32_uni(orc_pcg32* r)
 {
     return (double)orc_pcg32_next(r) * 0x1.0p-32;
 }
+
+size_t orc_rng_size(void) { return sizeof(orc_pcg32); }
+
+void orc_philox(const uint32_t ctr[4], uint64_t key, uint32_t out[4])
+{
+    orc_philox4x32_10(ctr, key, out);
+}
+
+uint32_t orc_ising_u32(uint64_t key, uint32_t site) { return orc_ising_u32_inline(key, site); }
+uint32_t orc_edge_u32(uint64_t key, uint32_t edge) { return orc_edge_u32_inline(key, edge); }
 
 void orc_pcg32_block(uint64_t seed, uint64_t stream, size_t n, uint32_t* out)
 {

@@ -5,6 +5,7 @@
  */
 #define _GNU_SOURCE
 #include "oracle.h"
+#include "philox.h"
 
 #include <float.h>
 #include <math.h>
@@ -160,7 +161,7 @@ void orc_vec_metropolis_keyed(int N, size_t n, const uint32_t* off, const uint32
             for (size_t i = 0; i < n; ++i) {
                 if (colour[i] != c) continue;
                 orc_pcg32 rng;
-                orc_pcg32_init(&rng, key, i);
+                orc_stream_init(&rng, key, i, ORC_TAG_SITE);
                 double* cur = s + i * (size_t)N;
                 propose(N, prop_kind, delta, cur, &rng, cand);
                 const double dE = orc_vec_energy_diff_at(N, off, bonds, s, J, i, cand);
@@ -253,7 +254,7 @@ static void vec_wolff_keyed_impl(int N, size_t n, const uint32_t* off, const uin
     for (size_t f = 0; f < nflips; ++f) {
         const uint64_t key = orc_key(run_seed, ORC_WOLFF_COUNTER_BASE + flip0 + f);
         orc_pcg32 ctl;
-        orc_pcg32_init(&ctl, key, 0);
+        orc_stream_init(&ctl, key, 0, ORC_TAG_WOLFF_CTRL);
         orc_sphere_sample(N, &ctl, r);
         const size_t seed = (size_t)(orc_pcg32_uni(&ctl) * (double)n);
         memset(in, 0, n);
@@ -268,9 +269,7 @@ static void vec_wolff_keyed_impl(int N, size_t n, const uint32_t* off, const uin
                 if (in[j]) continue;
                 const double aj = vdot(N, s + (size_t)j * N, r);
                 const double dE = (2.0 * J) * (ai * aj);
-                orc_pcg32 er;
-                orc_pcg32_init(&er, key, 1 + (uint64_t)edge_id[a]);
-                if (orc_pcg32_uni(&er) < 1.0 - exp(-beta * dE)) {
+                if ((double)orc_edge_u32_inline(key, edge_id[a]) * 0x1.0p-32 < 1.0 - exp(-beta * dE)) {
                     in[j] = 1;
                     queue[tail++] = j;
                 }

@@ -57,6 +57,14 @@ def oracle():
         raise RuntimeError(f"oracle not built: {ORACLE_SO} (run make -C oracle)")
     L = C.CDLL(ORACLE_SO)
     _sig(L, "orc_pcg32_block", None, u64, u64, sz, u32p)
+    _sig(L, "orc_rng_size", sz)
+    _sig(L, "orc_pcg32_init", None, C.c_void_p, u64, u64)
+    _sig(L, "orc_pcg32_next", C.c_uint32, C.c_void_p)
+    _sig(L, "orc_pcg32_uni", f64, C.c_void_p)
+    _sig(L, "orc_stream_init", None, C.c_void_p, u64, u64, C.c_uint32)
+    _sig(L, "orc_philox", None, C.c_void_p, u64, C.c_void_p)
+    _sig(L, "orc_ising_u32", C.c_uint32, u64, C.c_uint32)
+    _sig(L, "orc_edge_u32", C.c_uint32, u64, C.c_uint32)
     _sig(L, "orc_key", u64, u64, u64)
     _sig(L, "orc_pair_graph", i64, sz, sz, u32p, u32p, cint, ou32, ou32)
     _sig(L, "orc_square_lattice", i64, sz, sz, ou32, ou32)
@@ -71,6 +79,8 @@ def oracle():
     _sig(L, "orc_wolff_threshold", u64, f64, f64)
     _sig(L, "orc_ising_metropolis_seq", None, sz, u32p, u32p, i8p, i32, f64, u64, u64, sz, sz,
          of64, of64, ou64)
+    _sig(L, "orc_ising_metropolis_keyed_series", None, sz, u32p, u32p, i8p, i32, f64, u8p, cint,
+         u64, u64, sz, ou64, of64)
     _sig(L, "orc_ising_metropolis_keyed", None, sz, u32p, u32p, i8p, i32, f64, u8p, cint, u64,
          u64, sz, ou64)
     _sig(L, "orc_ising_wolff_seq", None, sz, u32p, u32p, i8p, i32, f64, u64, u64, sz, sz, of64,
@@ -132,7 +142,7 @@ class Pcg32:
     """Oracle Pcg32 state usable with the oracle's proposal functions."""
 
     def __init__(self, seed, stream=0):
-        self.buf = (C.c_uint64 * 2)()
+        self.buf = (C.c_uint64 * (oracle().orc_rng_size() // 8))()
         oracle().orc_pcg32_init(self.buf, seed, stream)
 
     @property
@@ -141,6 +151,27 @@ class Pcg32:
 
     def uni(self):
         return oracle().orc_pcg32_uni(self.ptr)
+
+    def next(self):
+        return oracle().orc_pcg32_next(self.ptr)
+
+
+TAG_ISING_HI, TAG_ISING_LO, TAG_SITE, TAG_WOLFF_CTRL, TAG_WOLFF_EDGE, TAG_SWAP = range(6)
+
+
+class Stream(Pcg32):
+    """The keyed Philox stream of the GPU chains (philox((id_lo, id_hi, tag, n), key))."""
+
+    def __init__(self, key, ident, tag):
+        self.buf = (C.c_uint64 * (oracle().orc_rng_size() // 8))()
+        oracle().orc_stream_init(self.buf, key, ident, tag)
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    out = (C.c_uint32 * 4)()
+    oracle().orc_philox(c, key, out)
+    return list(out)
 
 
 _ref = None

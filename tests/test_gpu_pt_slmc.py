@@ -5,7 +5,7 @@ Wolff step) against the oracle and against direct host computations."""
 import numpy as np
 import pytest
 
-from oracle_lib import checkerboard, csr_square, oracle, random_spins
+from oracle_lib import TAG_SWAP, Stream, checkerboard, csr_square, oracle, random_spins
 from universal_classical_montecarlo_amd import abi
 
 pytestmark = pytest.mark.gpu
@@ -67,9 +67,8 @@ def pt_replay(Ls, ladders, betas, fields, width, run_seed, sweep0, nsweeps):
             while kk + 1 < s1:
                 ra, rb = rep_at[kk], rep_at[kk + 1]
                 log_p = (betas[kk] - betas[kk + 1]) * (E[k, ra] - E[k, rb])
-                u = np.zeros(1, np.uint32)
-                o.orc_pcg32_block(pkey, kk, 1, u)
-                if log_p >= 0.0 or np.log(u[0] * 2.0 ** -32) < log_p:
+                u = Stream(pkey, kk, TAG_SWAP).next()
+                if log_p >= 0.0 or np.log(u * 2.0 ** -32) < log_p:
                     rep_at[kk], rep_at[kk + 1] = rb, ra
                     temp_of[ra], temp_of[rb] = kk + 1, kk
                 kk += 2

@@ -1,6 +1,6 @@
 """GPU parity of Wolff cluster flips (driver.hpp:136-254) against the oracle's
 keyed replay: the cluster is the connected component of the seed in the graph
-of bonds activated by their own Pcg32(k_f, 1 + e) draw.  Ising: bit-exact
+of bonds activated by their own keyed uniform edge_u32(k_f, e).  Ising: bit-exact
 spins and cluster sizes.  O(N): identical cluster sizes, spins within 2e-6."""
 import numpy as np
 import pytest

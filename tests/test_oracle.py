@@ -11,7 +11,8 @@ import os
 import numpy as np
 import pytest
 
-from oracle_lib import (Pcg32, checkerboard, csr_cubic, csr_square, hypercubic, oracle,
+from oracle_lib import (TAG_ISING_HI, TAG_ISING_LO, TAG_SITE, TAG_WOLFF_EDGE, Pcg32, Stream,
+                        checkerboard, csr_cubic, csr_square, hypercubic, oracle, philox,
                         random_spins, ref, ref_available, square_edges)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -39,6 +40,80 @@ def test_key_is_splitmix64():
     # SplitMix64 (Steele, Lea, Flood 2014), seed 0: 0xe220a8397b1dcdaf
     assert o.orc_key(0, 0) == 0xE220A8397B1DCDAF
     assert o.orc_key(0, 1) == 0x6E789E6AA1B965F4
+
+
+def test_philox_published_vectors():
+    """Philox4x32-10 known answers of Random123 (Salmon et al. SC'11, kat_vectors)."""
+    assert philox([0, 0, 0, 0], 0) == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    assert philox([0xFFFFFFFF] * 4, 0xFFFFFFFFFFFFFFFF) == [0x408F276D, 0x41C83B0E, 0xA20BC7C6,
+                                                            0x6D5451FD]
+    assert philox([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344],
+                  (0x299F31D0 << 32) | 0xA4093822) == [0xD16CFE09, 0x94FDCCEB, 0x5001E420,
+                                                       0x24126EA1]
+
+
+def test_keyed_uniform_layouts():
+    """The site / stream / edge uniforms are the documented Philox words."""
+    key = o.orc_key(5, 17)
+    for i in [0, 1, 2, 7, 8, 511, 512, 4097, 123457]:
+        q, r, j = (i >> 3) & 63, i >> 9, i & 7
+        w = (j & 1) * 2 + (j >> 2)
+        sh = 16 * ((j >> 1) & 1)
+        hi = (philox([q, r, TAG_ISING_HI, 0], key)[w] >> sh) & 0xFFFF
+        lo = (philox([q, r, TAG_ISING_LO, 0], key)[w] >> sh) & 0xFFFF
+        assert o.orc_ising_u32(key, i) == (hi << 16) | lo
+    st = Stream(key, 2 ** 33 + 9, TAG_SITE)
+    want = philox([9, 2, TAG_SITE, 0], key) + philox([9, 2, TAG_SITE, 1], key)
+    assert [st.next() for _ in range(8)] == want
+    for e in [0, 3, 4, 1001]:
+        assert o.orc_edge_u32(key, e) == philox([e >> 2, 0, TAG_WOLFF_EDGE, 0], key)[e & 3]
+
+
+def _pair_chi2(u, k, bins=16):
+    a = (u[:-k] >> 28).astype(np.int64)
+    b = (u[k:] >> 28).astype(np.int64)
+    h = np.bincount(a * bins + b, minlength=bins * bins)
+    e = len(a) / bins ** 2
+    return float(((h - e) ** 2 / e).sum())
+
+
+def test_site_uniforms_pairwise_independent():
+    """Joint histograms of (u_i, u_{i+k}) at the lattice separations a sweep uses
+    are flat (chi^2 with 255 dof; mean 255, sd 22.6).  The first outputs of
+    Pcg32(key, site), an earlier candidate, give chi^2 ~ 5e4 at k = 3 and 16."""
+    n = 1 << 16
+    for s in range(3):
+        key = o.orc_key(99, s)
+        u = np.array([o.orc_ising_u32(key, i) for i in range(n)], np.uint64)
+        for k in [1, 2, 3, 16, 64, 512, 4096]:
+            assert _pair_chi2(u, k) < 400, (s, k)
+
+
+def _exact_4x4(beta):
+    """<E>/site of the periodic 4x4 Ising model by enumerating all 2^16 states."""
+    idx = np.arange(1 << 16, dtype=np.int64)
+    spins = 1 - 2 * ((idx[:, None] >> np.arange(16)) & 1)
+    sp = spins.reshape(-1, 4, 4)
+    e = -(sp * np.roll(sp, 1, axis=1)).sum(axis=(1, 2)) - (sp * np.roll(sp, 1, axis=2)).sum(axis=(1, 2))
+    w = np.exp(-beta * (e - e.min()))
+    return float((e * w).sum() / w.sum()) / 16
+
+
+@pytest.mark.parametrize("beta", [0.3, 0.44])
+def test_keyed_chain_samples_exact_distribution(beta):
+    """The keyed colour-order chain (the GPU chain's definition) reproduces the
+    exact 4x4 energy within 4 sigma (batch means)."""
+    off, bonds = csr_square(4, 4)
+    s = np.ones(16, np.int8)
+    col = checkerboard((4, 4))
+    nsw = 400_000
+    e = np.zeros(nsw)
+    o.orc_ising_metropolis_keyed_series(16, off, bonds, s, 1, beta, col, 2, 77, 0, nsw, None, e)
+    e = e[1000:] / 16
+    blocks = e[: len(e) // 100 * 100].reshape(100, -1).mean(axis=1)
+    err = blocks.std(ddof=1) / 10
+    exact = _exact_4x4(beta)
+    assert abs(blocks.mean() - exact) < 4 * err, (blocks.mean(), exact, err)
 
 
 # ------------------------------------------------------------- tables --

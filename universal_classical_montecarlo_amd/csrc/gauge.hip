@@ -12,7 +12,7 @@
 //   dE = (beta / N) Re Tr((U - U') A).
 // Links of one colour (mu, parity of t+z+y+x) share no plaquette for even L,
 // so the 8 colour classes run as 8 launches, one thread per link, `hits`
-// Metropolis hits each drawing from the link's own Pcg32(key, link).
+// Metropolis hits each drawing from the link's own Stream(key, link, kTagSite).
 //
 // Storage: fp32, 2 N^2 planes (re, im of the row-major entries) for SU(N), one
 // angle plane for U(1); within a plane, links are ordered
@@ -207,7 +207,7 @@ __device__ __forceinline__ float uf(std::uint32_t k) { return static_cast<float>
 // dof/sun.hpp:245-274: X traceless Hermitian from N diagonal draws, then (a, b)
 // per i < j, scaled by width; U' = U expm(i X).
 template <int N>
-__device__ __forceinline__ void su_shift(Pcg& r, float width, const Mat<N>& u, Mat<N>& out)
+__device__ __forceinline__ void su_shift(Stream& r, float width, const Mat<N>& u, Mat<N>& out)
 {
     Mat<N> a;
     float tr = 0.f, d[N];
@@ -303,9 +303,9 @@ __global__ __launch_bounds__(256) void sun_colour(SunArgs a)
             }
         }
         load<N>(a.f, a.nl, self, U);
-        Pcg r;
+        Stream r;
         const std::uint32_t ref_id = a.mu * a.g.L4 + lin4(a.g, c);
-        r.seed(a.key, ref_id);
+        r.init(a.key, ref_id, kTagSite);
         for (int hit = 0; hit < a.hits; ++hit) {
             Mat<N> Up;
             su_shift<N>(r, a.width, U, Up);
@@ -383,9 +383,9 @@ __global__ __launch_bounds__(256) void u1_colour(U1Args a)
             wi += s;
         }
         float t = a.th[self];
-        Pcg r;
+        Stream r;
         const std::uint32_t ref_id = a.mu * a.g.L4 + lin4(a.g, c);
-        r.seed(a.key, ref_id);
+        r.init(a.key, ref_id, kTagSite);
         const float two_pi = static_cast<float>(2.0 * 3.14159265358979323846);
         for (int hit = 0; hit < a.hits; ++hit) {
             // dof/u1.hpp:40-46

@@ -7,14 +7,18 @@
 //    lattices, and as the simple reference-shaped path.
 //  * ising_fused: periodic square lattice with Lx % 512 == 0 and Ly % H == 0.
 //    Both checkerboard colours of a sweep in one pass: each wave owns a 512-byte
-//    column band and an H-row block, slides down the rows with a 4-row register
-//    window (colour 0 of row r-1, then colour 1 of row r-2), and reads every
-//    spin once and writes it once (2 B/site).  Rows are ping-ponged between two
-//    buffers so halo rows are always read unmodified.
+//    column band and an H-row block, loads the block and its two halo rows
+//    above and below into registers at once, then walks down the rows (colour 0
+//    of row j+1, then colour 1 of row j) and stores each row once (2 B/site).
+//    Rows are ping-ponged between two buffers so halo rows are always read
+//    unmodified.
 //
 // The accept rule log(u) < -beta dE (driver.hpp:83-84) is evaluated exactly as
 // u_32 <= k*(dE) - 1, with k* computed on the host with the same double
 // expression (accept_threshold), so decisions are bit-identical to the CPU.
+// u_32 of site i is ising_u32(key, i) (Philox4x32-10, ucmc_device.hpp); the
+// fused kernel compares its high 16 bits first and computes the low 16 bits
+// only for a wave in which some site ties.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -58,7 +62,7 @@ __global__ __launch_bounds__(256) void ising_csr_colour(CsrArgs a)
         int sum = 0;
         for (std::uint32_t q = a.offsets[i]; q < a.offsets[i + 1]; ++q) sum += a.field[a.bonds[q]];
         const int h = s * sum;
-        const std::uint32_t u = pcg_out(first_state(a.key, i));
+        const std::uint32_t u = ising_u32(a.key, i);
         acc = u <= a.tm1[h + static_cast<int>(a.max_degree)];
         if (acc) a.field[i] = static_cast<std::int8_t>(-s);
         if (a.trace_dE) {
@@ -76,11 +80,9 @@ __global__ __launch_bounds__(256) void ising_csr_colour(CsrArgs a)
 struct FusedArgs {
     const std::uint32_t* in;   // field as 32-bit words, Lx / 4 per row
     std::uint32_t* out;
-    std::uint32_t Lx, Ly, H;
+    std::uint32_t Lx, Ly;
     std::uint32_t bands;       // Lx / 512
-    std::uint64_t A;           // first_state(key, 0)
-    std::uint64_t LxD;         // Lx * stride (mod 2^64)
-    std::uint64_t D;           // stream stride
+    std::uint64_t key;         // sweep key
     std::uint32_t tm1[16];     // index C * 8 + S (C = site is -1, S = # of -1 neighbours)
     unsigned long long* accepts;
 };
@@ -95,12 +97,13 @@ __device__ __forceinline__ uint2 load_raw(const std::uint32_t* in, std::uint32_t
     return *reinterpret_cast<const uint2*>(in + static_cast<std::size_t>(Y) * row_words + 2 * col);
 }
 
-// Split 8 bytes into even-x and odd-x words and map +1 -> 0, -1 -> 4 per byte.
+// Split 8 bytes into even-x and odd-x words and map +1 -> 0, -1 -> 4 per byte
+// (0x01 & 4 == 0, 0xFF & 4 == 4).
 __device__ __forceinline__ Row convert_row(uint2 w)
 {
     const std::uint32_t e = __builtin_amdgcn_perm(w.y, w.x, 0x06040200u);
     const std::uint32_t o = __builtin_amdgcn_perm(w.y, w.x, 0x07050301u);
-    return Row{(e << 1) & 0x04040404u, (o << 1) & 0x04040404u};
+    return Row{e & 0x04040404u, o & 0x04040404u};
 }
 
 __device__ __forceinline__ void store_row(std::uint32_t* out, std::uint32_t row_words,
@@ -114,169 +117,243 @@ __device__ __forceinline__ void store_row(std::uint32_t* out, std::uint32_t row_
     *reinterpret_cast<uint2*>(out + static_cast<std::size_t>(Y) * row_words + 2 * col) = w;
 }
 
-// Byte k of C takes byte k of Cf when u <= t: one compare plus one
-// sub-dword conditional move (SDWA dst_sel:BYTE_k, the other bytes preserved).
-template <int K>
-__device__ __forceinline__ void select_byte(std::uint32_t& C, std::uint32_t Cf, std::uint32_t u, std::uint32_t t)
+// The fast half of the exact decision u <= t for the four sites of one
+// 4b-form word, u = (h << 16) | l: byte k of C takes byte k of Cf when
+// h_k < T_k (T = t >> 16), with h_0, h_1 the halves of wa and h_2, h_3 those of
+// wb.  Per byte: one SDWA compare on the 16-bit half and one sub-dword
+// conditional move (dst_sel:BYTE_k, the other bytes preserved).  One asm block
+// for the word, so the table reads are issued before it.
+__device__ __forceinline__ std::uint32_t select_lt(std::uint32_t C, std::uint32_t Cf, std::uint32_t wa,
+                                                   std::uint32_t wb, const std::uint32_t (&t)[4])
 {
-#define UCMC_SEL(B)                                                                                   \
-    asm("v_cmp_le_u32 vcc, %1, %2\n\tv_cndmask_b32_sdwa %0, %0, %3, vcc dst_sel:BYTE_" #B          \
-        " dst_unused:UNUSED_PRESERVE src0_sel:BYTE_" #B " src1_sel:BYTE_" #B                          \
-        : "+v"(C) : "v"(u), "v"(t), "v"(Cf) : "vcc")
-    if constexpr (K == 0) UCMC_SEL(0);
-    else if constexpr (K == 1) UCMC_SEL(1);
-    else if constexpr (K == 2) UCMC_SEL(2);
-    else UCMC_SEL(3);
+#define UCMC_SEL(B, W, H, T)                                                                     \
+    "v_cmp_lt_u32_sdwa vcc, " W ", " T " src0_sel:WORD_" #H " src1_sel:DWORD\n\t"                 \
+    "v_cndmask_b32_sdwa %0, %0, %7, vcc dst_sel:BYTE_" #B                                       \
+    " dst_unused:UNUSED_PRESERVE src0_sel:BYTE_" #B " src1_sel:BYTE_" #B "\n\t"
+    asm(UCMC_SEL(0, "%1", 0, "%3") UCMC_SEL(1, "%1", 1, "%4") UCMC_SEL(2, "%2", 0, "%5")
+            UCMC_SEL(3, "%2", 1, "%6")
+        : "+v"(C)
+        : "v"(wa), "v"(wb), "v"(t[0]), "v"(t[1]), "v"(t[2]), "v"(t[3]), "v"(Cf)
+        : "vcc");
 #undef UCMC_SEL
+    return C;
 }
 
-// Decide the four sites of one 4b-form word C with neighbour counts S.
-// lane_base: per-lane part of the stream state; rowk[k]: the wave-uniform part
-// of site k's state.  Returns the updated word.
-template <bool COUNT>
-__device__ __forceinline__ std::uint32_t decide(std::uint32_t C, std::uint32_t S,
-                                                std::uint64_t lane_base,
-                                                const std::uint64_t (&rowk)[4],
-                                                const std::uint32_t* tbl, std::uint32_t& acc)
+// Lanes (bit mask) where some byte's high half equals T_k: those bytes need
+// the low halves.
+__device__ __forceinline__ std::uint64_t ties(std::uint32_t wa, std::uint32_t wb, const std::uint32_t (&t)[4])
 {
-    const std::uint32_t key = (C << 3) | S; // byte k = 4 * (8 C_k + S_k)
+    std::uint64_t m0, m1, m2, m3;
+    asm("v_cmp_eq_u32_sdwa %0, %4, %6 src0_sel:WORD_0 src1_sel:DWORD\n\t"
+        "v_cmp_eq_u32_sdwa %1, %4, %7 src0_sel:WORD_1 src1_sel:DWORD\n\t"
+        "v_cmp_eq_u32_sdwa %2, %5, %8 src0_sel:WORD_0 src1_sel:DWORD\n\t"
+        "v_cmp_eq_u32_sdwa %3, %5, %9 src0_sel:WORD_1 src1_sel:DWORD"
+        : "=s"(m0), "=s"(m1), "=s"(m2), "=s"(m3)
+        : "v"(wa), "v"(wb), "v"(t[0]), "v"(t[1]), "v"(t[2]), "v"(t[3]));
+    return m0 | m1 | m2 | m3;
+}
+
+// Thresholds of the fused kernel in LDS: T (high 16 bits, 0x10000 when every
+// u is accepted) at word S + 64 C4 (S = 4 * # of -1 neighbours, C4 = 0 or 4 for
+// a +1 / -1 site), the low 16 bits at the same index of the second table.
+struct FusedTables {
+    std::uint32_t hi[261];
+    std::uint32_t lo[261];
+};
+
+// The ties of one word (rare: probability 2^-16 per site): bytes whose high
+// half equals T_k flip when the low half is <= the low threshold.  Kept out of
+// line so the sweep loop around its call site stays small.
+__device__ __noinline__ std::uint32_t resolve_ties(std::uint32_t out, std::uint32_t C, std::uint32_t S,
+                                                   std::uint32_t wa, std::uint32_t wb,
+                                                   const FusedTables* tb, std::uint64_t key,
+                                                   std::uint32_t q, std::uint32_t r, bool odd)
+{
+    const U4 lo = philox(U4{q, r, kTagIsingLo, 0u}, key);
+    const std::uint32_t la = odd ? lo.z : lo.x, lb = odd ? lo.w : lo.y;
+    for (int k = 0; k < 4; ++k) {
+        const std::uint32_t w = k < 2 ? wa : wb, l = k < 2 ? la : lb;
+        const std::uint32_t h16 = (k & 1) ? (w >> 16) : (w & 0xFFFFu);
+        const std::uint32_t l16 = (k & 1) ? (l >> 16) : (l & 0xFFFFu);
+        const std::uint32_t idx = (((S >> (8 * k)) & 0xFFu) >> 2) + 64u * ((C >> (8 * k)) & 0xFFu);
+        if (h16 == tb->hi[idx] && l16 <= tb->lo[idx]) out ^= 4u << (8 * k);
+    }
+    return out;
+}
+
+// Decide the four sites of one 4b-form word C with neighbour counts S.  wa, wb:
+// the high-half Philox words of these sites; (q, r, odd): the lane's block and
+// the half of the row, for the low halves on a tie.
+template <bool COUNT>
+__device__ __forceinline__ std::uint32_t decide(std::uint32_t C, std::uint32_t S, std::uint32_t wa,
+                                                std::uint32_t wb, const FusedTables& tb,
+                                                std::uint64_t key, std::uint32_t q, std::uint32_t r,
+                                                bool odd, std::uint32_t& acc)
+{
     const std::uint32_t Cf = C ^ 0x04040404u;
-    std::uint32_t out = C;
-    std::uint32_t t[4], u[4];
+    std::uint32_t t[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const std::uint32_t addr = (key >> (8 * k)) & 0xFFu;
-        t[k] = *reinterpret_cast<const std::uint32_t*>(reinterpret_cast<const char*>(tbl) + addr);
-        u[k] = pcg_out(lane_base + rowk[k]);
+        // table byte address S_k + 256 C_k: S_k in byte 0, C_k in byte 1, one v_perm
+        const std::uint32_t addr = __builtin_amdgcn_perm(C, S, 0x0C0C0000u | ((4u + k) << 8) | k);
+        t[k] = *reinterpret_cast<const std::uint32_t*>(reinterpret_cast<const char*>(tb.hi) + addr);
     }
-    select_byte<0>(out, Cf, u[0], t[0]);
-    select_byte<1>(out, Cf, u[1], t[1]);
-    select_byte<2>(out, Cf, u[2], t[2]);
-    select_byte<3>(out, Cf, u[3], t[3]);
+    std::uint32_t out = select_lt(C, Cf, wa, wb, t);
+    if (ties(wa, wb, t))
+        out = resolve_ties(out, C, S, wa, wb, &tb, key, q, r, odd);
     if (COUNT) acc += __builtin_popcount(out ^ C);
     return out;
 }
 
-template <bool COUNT>
-__global__ __launch_bounds__(256) void ising_fused(FusedArgs a)
+template <int H, bool COUNT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void ising_fused(FusedArgs a)
 {
-    __shared__ std::uint32_t tbl[16];
-    __shared__ std::uint32_t haloL[4][66];
-    __shared__ std::uint32_t haloR[4][66];
+    static_assert(H % 2 == 0 && H + 2 <= 64, "block height: even, halo rows fit one wave");
+    __shared__ FusedTables tb;
+    __shared__ std::uint32_t haloL[4][H + 2];
+    __shared__ std::uint32_t haloR[4][H + 2];
 
     const std::uint32_t lane = threadIdx.x & 63u;
     // wave-uniform by construction; readfirstlane lets the compiler keep every
     // row / stream quantity derived from it in scalar registers
     const std::uint32_t wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const std::uint32_t gw = blockIdx.x * 4u + wid;          // global wave index
-    const std::uint32_t nblocks = a.Ly / a.H;
-    if (threadIdx.x < 16) tbl[threadIdx.x] = a.tm1[threadIdx.x];
-    __syncthreads();
-
+    // Workgroup ids are dealt round-robin over the 8 XCDs; renumber them so
+    // that each XCD owns a contiguous run of row blocks and the halo rows that
+    // vertically adjacent blocks share are read through the same L2.
+    std::uint32_t wg = blockIdx.x;
+    if ((gridDim.x & 7u) == 0) wg = (wg & 7u) * (gridDim.x >> 3) + (wg >> 3);
+    const std::uint32_t gw = wg * 4u + wid;                  // global wave index
+    const std::uint32_t nblocks = a.Ly / H;
     const bool active = gw < a.bands * nblocks;
     const std::uint32_t band = active ? gw % a.bands : 0;
     const std::uint32_t rb = active ? gw / a.bands : 0;
-    const std::uint32_t y0 = rb * a.H;
+    const std::uint32_t y0 = rb * H;
     const std::uint32_t row_words = a.Lx >> 2;
     const std::uint32_t col = band * 64u + lane;              // 8-byte column
-    const std::uint32_t x0 = col * 8u;
-    const std::uint8_t* inb = reinterpret_cast<const std::uint8_t*>(a.in);
+    const std::int32_t Ly = static_cast<std::int32_t>(a.Ly);
+    // absolute row of window row j (j = 0 .. H + 3 <-> y0 - 2 .. y0 + H + 1)
+    auto rowY = [&](int j) -> std::uint32_t {
+        std::int32_t t = static_cast<std::int32_t>(y0) + j - 2;
+        if (t < 0) t += Ly;
+        else if (t >= Ly) t -= Ly;
+        return static_cast<std::uint32_t>(t);
+    };
 
-    // ---- halo pre-pass: one lane per row y0-1 .. y0+H (H <= 64) ----
-    if (active && lane < a.H + 2) {
-        const std::uint32_t Y = (y0 + a.Ly - 1u + lane) % a.Ly;
-        const std::uint32_t Yu = (Y + a.Ly - 1u) % a.Ly, Yd = (Y + 1u) % a.Ly;
-        const std::uint32_t xl = (band * 512u + a.Lx - 1u) % a.Lx;   // odd
-        const std::uint32_t xr = (band * 512u + 512u) % a.Lx;        // even
-        auto b4 = [&](std::uint32_t x, std::uint32_t y) -> std::uint32_t {
-            return (static_cast<std::uint32_t>(inb[static_cast<std::size_t>(y) * a.Lx + x]) << 1) & 4u;
-        };
-        auto update0 = [&](std::uint32_t x) -> std::uint32_t {
-            // colour-0 update of site (x, Y) from unmodified neighbours
-            const std::uint32_t c = b4(x, Y);
-            const std::uint32_t s = b4((x + 1u) % a.Lx, Y) + b4((x + a.Lx - 1u) % a.Lx, Y) +
-                                    b4(x, Yu) + b4(x, Yd);
-            const std::uint32_t t = tbl[(c << 1) + (s >> 2)];
-            const std::uint64_t st = a.A + (static_cast<std::uint64_t>(Y) * a.Lx + x) * a.D;
-            return (pcg_out(st) <= t) ? (c ^ 4u) : c;
+    // Every row of the window is requested before anything waits on memory:
+    // one memory latency per wave instead of one per row.
+    uint2 raw[H + 4];
+#pragma unroll
+    for (int j = 0; j < H + 4; ++j) raw[j] = load_raw(a.in, row_words, rowY(j), col);
+
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int n = 0; n < 5; ++n)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const std::uint32_t t = a.tm1[8 * c + n];
+                tb.hi[256 * c + n] = t == 0xFFFFFFFFu ? 0x10000u : t >> 16;
+                tb.lo[256 * c + n] = t == 0xFFFFFFFFu ? 0u : t & 0xFFFFu;
+            }
+    }
+    __syncthreads();   // the row loads stay in flight across it
+
+    // ---- halo pre-pass: one lane per window row 1 .. H + 2 (y0-1 .. y0+H) ----
+    // Lane l owns row Y = y0 - 1 + l and the two sites just outside the band,
+    // xl = x_band - 1 (odd) and xr = x_band + 512 (even).  All ten bytes it
+    // needs are requested at once; both candidate values are computed and the
+    // row parity picks one, so the pre-pass costs one memory latency.
+    const std::uint8_t* inb = reinterpret_cast<const std::uint8_t*>(a.in);
+    const bool halo_lane = active && lane < static_cast<std::uint32_t>(H + 2);
+    if (halo_lane) {
+        std::uint32_t Y = y0 + lane + a.Ly - 1u;               // < 3 Ly
+        if (Y >= a.Ly) Y -= a.Ly;
+        if (Y >= a.Ly) Y -= a.Ly;
+        const std::uint32_t Yu = Y == 0 ? a.Ly - 1u : Y - 1u;
+        const std::uint32_t Yd = Y + 1u == a.Ly ? 0u : Y + 1u;
+        const std::uint32_t xb = band * 512u;
+        const std::uint32_t xl = xb == 0 ? a.Lx - 1u : xb - 1u;   // odd, >= 1
+        const std::uint32_t xr = xb + 512u == a.Lx ? 0u : xb + 512u;
+        const std::uint32_t xlr = xl + 1u == a.Lx ? 0u : xl + 1u;
+        const std::uint32_t xrl = xr == 0 ? a.Lx - 1u : xr - 1u;
+        const std::size_t r0 = static_cast<std::size_t>(Y) * a.Lx;
+        const std::size_t ru = static_cast<std::size_t>(Yu) * a.Lx;
+        const std::size_t rd = static_cast<std::size_t>(Yd) * a.Lx;
+        const std::uint32_t l_c = inb[r0 + xl], l_l = inb[r0 + xl - 1u], l_r = inb[r0 + xlr];
+        const std::uint32_t l_u = inb[ru + xl], l_d = inb[rd + xl];
+        const std::uint32_t r_c = inb[r0 + xr], r_l = inb[r0 + xrl], r_r = inb[r0 + xr + 1u];
+        const std::uint32_t r_u = inb[ru + xr], r_d = inb[rd + xr];
+        auto b4 = [](std::uint32_t v) { return v & 4u; };   // +1 (0x01) -> 0, -1 (0xFF) -> 4
+        auto update0 = [&](std::uint32_t x, std::uint32_t c, std::uint32_t s) -> std::uint32_t {
+            // colour-0 update of site (x, Y) from its unmodified neighbours
+            const std::uint32_t idx = (c << 6) + (s >> 2);
+            const std::uint32_t th = tb.hi[idx];
+            const std::uint32_t i = Y * a.Lx + x, q = (i >> 3) & 63u, r = i >> 9, j = i & 7u;
+            const std::uint32_t h16 = ising_half(philox(U4{q, r, kTagIsingHi, 0u}, a.key), j);
+            bool acc = h16 < th;
+            if (h16 == th) acc = ising_half(philox(U4{q, r, kTagIsingLo, 0u}, a.key), j) <= tb.lo[idx];
+            return acc ? (c ^ 4u) : c;
         };
         const bool even = (Y & 1u) == 0;
-        // left: odd x; even rows need the old value, odd rows the colour-0 value
-        haloL[wid][lane] = (even ? b4(xl, Y) : update0(xl)) << 24;
-        // right: even x; even rows need the colour-0 value, odd rows the old one
-        haloR[wid][lane] = even ? update0(xr) : b4(xr, Y);
+        const std::uint32_t lc = b4(l_c), rc = b4(r_c);
+        const std::uint32_t lnew = update0(xl, lc, b4(l_l) + b4(l_r) + b4(l_u) + b4(l_d));
+        const std::uint32_t rnew = update0(xr, rc, b4(r_l) + b4(r_r) + b4(r_u) + b4(r_d));
+        // left (odd x): even rows need the old value, odd rows the colour-0 value;
+        // right (even x): even rows need the colour-0 value, odd rows the old one
+        haloL[wid][lane] = (even ? lc : lnew) << 24;
+        haloR[wid][lane] = even ? rnew : rc;
     }
     __syncthreads();
     if (!active) return;
 
     std::uint32_t acc = 0;
-    const std::uint64_t lane_base = static_cast<std::uint64_t>(x0) * a.D;
-
-    auto rowk = [&](std::uint32_t Y, std::uint32_t odd, std::uint64_t (&rk)[4]) {
-        const std::uint64_t base = a.A + static_cast<std::uint64_t>(Y) * a.LxD;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) rk[k] = base + static_cast<std::uint64_t>(2 * k + odd) * a.D;
+    // high-half Philox words of window row j: (x, y) even-x sites, (z, w) odd-x
+    U4 P[H + 4];
+    auto philox_row = [&](int j) {
+        P[j] = philox(U4{lane, rowY(j) * a.bands + band, kTagIsingHi, 0u}, a.key);
     };
-    auto wrap = [&](std::uint32_t y) { return y >= a.Ly ? y - a.Ly : y; };
-    // Row index relative to the block start, as used by the halo arrays.
-    auto hidx = [&](std::uint32_t r) { return r + 1u; };
-
-    // E update of row Y (site colour == Y & 1 ... handled by caller).
-    auto updE = [&](Row& cur, const Row& up, const Row& dn, std::uint32_t Y, std::uint32_t hr) {
-        const std::uint32_t Oprev = from_prev_lane(cur.O, haloL[wid][hr]);
+    // even-x half of window row j (neighbours: odd half of the row, with the
+    // left halo for lane 0, and the even halves of rows j - 1, j + 1)
+    auto updE = [&](Row& cur, const Row& up, const Row& dn, int j, bool count) {
+        const std::uint32_t Oprev = from_prev_lane(cur.O, haloL[wid][j - 1]);
         const std::uint32_t Lft = __builtin_amdgcn_alignbit(cur.O, Oprev, 24);
         const std::uint32_t S = Lft + cur.O + up.E + dn.E;
-        std::uint64_t rk[4];
-        rowk(Y, 0u, rk);
-        cur.E = decide<COUNT>(cur.E, S, lane_base, rk, tbl, acc);
+        std::uint32_t c = 0;
+        cur.E = decide<COUNT>(cur.E, S, P[j].x, P[j].y, tb, a.key, lane, rowY(j) * a.bands + band, false, c);
+        if (count) acc += c;
     };
-    auto updO = [&](Row& cur, const Row& up, const Row& dn, std::uint32_t Y, std::uint32_t hr) {
-        const std::uint32_t Enext = from_next_lane(cur.E, haloR[wid][hr]);
+    auto updO = [&](Row& cur, const Row& up, const Row& dn, int j, bool count) {
+        const std::uint32_t Enext = from_next_lane(cur.E, haloR[wid][j - 1]);
         const std::uint32_t Rgt = __builtin_amdgcn_alignbit(Enext, cur.E, 8);
         const std::uint32_t S = cur.E + Rgt + up.O + dn.O;
-        std::uint64_t rk[4];
-        rowk(Y, 1u, rk);
-        cur.O = decide<COUNT>(cur.O, S, lane_base, rk, tbl, acc);
+        std::uint32_t c = 0;
+        cur.O = decide<COUNT>(cur.O, S, P[j].z, P[j].w, tb, a.key, lane, rowY(j) * a.bands + band, true, c);
+        if (count) acc += c;
+    };
+    // y0 is even, so window row j has the parity of j: colour 0 is the even-x
+    // half on even j and the odd-x half on odd j.  Rows 1 and H + 2 (y0 - 1 and
+    // y0 + H) get their colour-0 update only as neighbours of the block's own
+    // rows; their owners update, store and count them.
+    auto colour0 = [&](Row (&R)[H + 4], int j) {
+        philox_row(j);
+        if (j & 1) updO(R[j], R[j - 1], R[j + 1], j, j != 1 && j != H + 2);
+        else updE(R[j], R[j - 1], R[j + 1], j, j != 1 && j != H + 2);
+    };
+    auto colour1 = [&](Row (&R)[H + 4], int j) {
+        if (j & 1) updE(R[j], R[j - 1], R[j + 1], j, true);
+        else updO(R[j], R[j - 1], R[j + 1], j, true);
     };
 
-    // rows y0-2 .. y0+1; each row is fetched one step before it is used
-    Row A_, B_, C_, D_;
-    D_ = convert_row(load_raw(a.in, row_words, wrap(y0 + a.Ly - 2u), col));
-    A_ = convert_row(load_raw(a.in, row_words, wrap(y0 + a.Ly - 1u), col));
-    B_ = convert_row(load_raw(a.in, row_words, y0, col));
-    uint2 nxt = load_raw(a.in, row_words, wrap(y0 + 1u), col);
-    updO(A_, D_, B_, wrap(y0 + a.Ly - 1u), 0u);  // colour 0 of row y0-1 (odd)
-    C_ = convert_row(nxt);
-    nxt = load_raw(a.in, row_words, wrap(y0 + 2u), col);
-    updE(B_, A_, C_, y0, hidx(0u));            // colour 0 of row y0 (even)
-
-    // main loop: step for absolute row r = y0 + 2 + k
-    for (std::uint32_t k = 0; k < a.H; k += 4) {
-        const std::uint32_t r = y0 + 2u + k; // even
-        // k % 4 == 0: window (A,B,C,D) = rows r-3 .. r
-        D_ = convert_row(nxt);
-        nxt = load_raw(a.in, row_words, wrap(r + 1u), col);
-        updO(C_, B_, D_, wrap(r - 1u), hidx(k + 1u));      // colour 0, row r-1 (odd)
-        updO(B_, A_, C_, r - 2u, hidx(k));                // colour 1, row r-2 (even)
-        store_row(a.out, row_words, r - 2u, col, B_);
-        // k % 4 == 1: window (B,C,D,A), r+1 odd
-        A_ = convert_row(nxt);
-        nxt = load_raw(a.in, row_words, wrap(r + 2u), col);
-        updE(D_, C_, A_, wrap(r), hidx(k + 2u));           // colour 0, row r (even)
-        updE(C_, B_, D_, wrap(r - 1u), hidx(k + 1u));      // colour 1, row r-1 (odd)
-        store_row(a.out, row_words, wrap(r - 1u), col, C_);
-        // k % 4 == 2: window (C,D,A,B), r+2 even
-        B_ = convert_row(nxt);
-        nxt = load_raw(a.in, row_words, wrap(r + 3u), col);
-        updO(A_, D_, B_, wrap(r + 1u), hidx(k + 3u));      // colour 0, row r+1 (odd)
-        updO(D_, C_, A_, wrap(r), hidx(k + 2u));           // colour 1, row r (even)
-        store_row(a.out, row_words, wrap(r), col, D_);
-        // k % 4 == 3: window (D,A,B,C), r+3 odd
-        C_ = convert_row(nxt);
-        if (k + 4 < a.H) nxt = load_raw(a.in, row_words, wrap(r + 4u), col);
-        updE(B_, A_, C_, wrap(r + 2u), hidx(k + 4u));      // colour 0, row r+2 (even)
-        updE(A_, D_, B_, wrap(r + 1u), hidx(k + 3u));      // colour 1, row r+1 (odd)
-        store_row(a.out, row_words, wrap(r + 1u), col, A_);
+    Row R[H + 4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) R[j] = convert_row(raw[j]);
+    colour0(R, 1);
+    colour0(R, 2);
+#pragma unroll
+    for (int j = 2; j <= H + 1; ++j) {
+        R[j + 2] = convert_row(raw[j + 2]);
+        colour0(R, j + 1);
+        colour1(R, j);
+        store_row(a.out, row_words, rowY(j), col, R[j]);
     }
 
     if (COUNT && a.accepts) {
@@ -333,11 +410,15 @@ __global__ __launch_bounds__(256) void ising_reduce_csr(const std::int8_t* f, co
     }
 }
 
-constexpr std::uint32_t kFusedH = 16;
+// Rows per wave block of the fused kernel (even, <= 62).
+#ifndef UCMC_FUSED_H
+#define UCMC_FUSED_H 16
+#endif
+constexpr int kFusedH = UCMC_FUSED_H;
 
 bool fused_ok(const System& s)
 {
-    return s.geom == Geom::Square && s.Lx % 512 == 0 && s.Ly % kFusedH == 0 && s.Ly >= kFusedH;
+    return s.geom == Geom::Square && s.Lx % 512 == 0 && s.Ly % kFusedH == 0 && s.Ly >= static_cast<std::size_t>(kFusedH);
 }
 
 struct IsingTables {
@@ -375,17 +456,14 @@ void launch_sweep(System& s, const IsingTables& t, std::uint64_t key, bool simpl
         a.out = s.field[s.cur ^ 1].as<std::uint32_t>();
         a.Lx = static_cast<std::uint32_t>(s.Lx);
         a.Ly = static_cast<std::uint32_t>(s.Ly);
-        a.H = kFusedH;
         a.bands = static_cast<std::uint32_t>(s.Lx / 512);
-        a.D = kStreamStride;
-        a.A = first_state(key, 0);
-        a.LxD = static_cast<std::uint64_t>(s.Lx) * kStreamStride;
+        a.key = key;
         std::memcpy(a.tm1, t.sq, sizeof(a.tm1));
         a.accepts = dacc;
-        const std::uint32_t waves = a.bands * (a.Ly / a.H);
+        const std::uint32_t waves = a.bands * (a.Ly / static_cast<std::uint32_t>(kFusedH));
         const dim3 grid((waves + 3) / 4);
-        if (dacc) hipLaunchKernelGGL(ising_fused<true>, grid, dim3(256), 0, s.stream, a);
-        else hipLaunchKernelGGL(ising_fused<false>, grid, dim3(256), 0, s.stream, a);
+        if (dacc) hipLaunchKernelGGL((ising_fused<kFusedH, true>), grid, dim3(256), 0, s.stream, a);
+        else hipLaunchKernelGGL((ising_fused<kFusedH, false>), grid, dim3(256), 0, s.stream, a);
         UCMC_HIP(hipGetLastError());
         s.cur ^= 1;
         return;

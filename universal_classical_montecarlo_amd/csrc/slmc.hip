@@ -71,7 +71,7 @@ __global__ __launch_bounds__(256) void fpi_colour(FpiArgs a)
         const int c_mp = at(a.s, L, xm, yp), c_0p = at(a.s, L, x, yp), c_pp = at(a.s, L, xp, yp);
         const int pq = si * (c_mm * c_0m * c_m0 + c_0m * c_pm * c_p0 + c_m0 * c_mp * c_0p + c_p0 * c_0p * c_pp);
         const int hn = si * nn;
-        const std::uint32_t u = pcg_out(first_state(a.key, i));
+        const std::uint32_t u = ising_u32(a.key, i);
         acc = u <= a.tm1[((hn + 4) / 2) * 5 + (pq + 4) / 2];
         if (acc) a.s[i] = static_cast<std::int8_t>(-si);
         if constexpr (TRACE) {
@@ -284,8 +284,8 @@ __global__ __launch_bounds__(1024) void slmc_flips(SlmcArgs a)
         const std::uint32_t mark = a.mark0 + fi;
         const std::uint64_t key = key_of(a.run_seed, kSlmcCounterBase + a.flip0 + fi);
         if (tid == 0) {
-            Pcg ctl;
-            ctl.seed(key, 0);
+            Stream ctl;
+            ctl.init(key, 0, kTagWolffCtrl);
             const std::uint32_t seed = static_cast<std::uint32_t>(ctl.uni() * static_cast<double>(n));
             a.stamp[seed] = mark;
             a.queue[0] = seed;
@@ -311,8 +311,8 @@ __global__ __launch_bounds__(1024) void slmc_flips(SlmcArgs a)
                         const std::uint32_t sj = a.stamp[j];
                         if (sj != mark) {
                             const std::uint32_t owner = kForward[d] ? i : j;
-                            const std::uint64_t e = 6ull * owner + kSlot[d];
-                            const std::uint32_t u = pcg_out(first_state(key, 1ull + e));
+                            const std::uint32_t e = 6u * owner + kSlot[d];
+                            const std::uint32_t u = edge_u32(key, e);
                             const bool aligned = si * a.s[j] > 0;
                             if (static_cast<std::uint64_t>(u) < a.thr[kShell[d]][aligned ? 1 : 0])
                                 won = atomicCAS(&a.stamp[j], sj, mark) == sj;
@@ -374,8 +374,8 @@ __global__ __launch_bounds__(1024) void slmc_flips(SlmcArgs a)
         __syncthreads();
         if (tid == 0) {
             // min(1, exp(-beta (dE_true - dE_eff))), the draw after the seed draw
-            Pcg ctl;
-            ctl.seed(key, 0);
+            Stream ctl;
+            ctl.init(key, 0, kTagWolffCtrl);
             (void)ctl.next();
             const double log_p = -a.beta * (s_red[0] - s_red[1]) + 0.0;
             s_accept = (log_p >= 0.0 || log(ctl.uni()) < log_p) ? 1 : 0;

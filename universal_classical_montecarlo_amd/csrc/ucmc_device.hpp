@@ -1,6 +1,6 @@
-// ucmc_device.hpp -- device-side building blocks shared by every kernel:
-// the per-site counter-based PCG32 stream, uniform conversions and wave
-// helpers.  gfx950 only.
+// ucmc_device.hpp -- device-side building blocks shared by every kernel: the
+// counter-based random numbers (sweep keys, Philox4x32-10 words, per-site
+// streams), the exact accept test and wave helpers.  gfx950 only.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -9,9 +9,9 @@
 
 namespace ucmc {
 
-constexpr std::uint64_t kPcgMult = 6364136223846793005ull;
-
-// SplitMix64 output number (counter + 1) for state run_seed.
+// SplitMix64 output number (counter + 1) for state run_seed (Steele, Lea, Flood
+// 2014): the key of Metropolis sweep `counter`, of Wolff flip
+// kWolffCounterBase + f, and so on.
 __host__ __device__ inline std::uint64_t key_of(std::uint64_t run_seed, std::uint64_t counter)
 {
     std::uint64_t z = run_seed + (counter + 1u) * 0x9E3779B97F4A7C15ull;
@@ -22,47 +22,127 @@ __host__ __device__ inline std::uint64_t key_of(std::uint64_t run_seed, std::uin
 
 constexpr std::uint64_t kWolffCounterBase = 1ull << 62;
 
-// The state whose XSH-RR output is the first draw of Pcg32(seed, stream)
-// (rng.hpp:14-20 seeding, rng.hpp:31-40 step): (seed + inc) M + inc with
-// inc = 2 stream + 1.  It is affine in the stream: A + stream * D with
-// A = (seed + 1) M + 1 and D = 2 M + 2.
-__host__ __device__ inline std::uint64_t first_state(std::uint64_t seed, std::uint64_t stream)
+// ---------------------------------------------------------------------------
+// Device random numbers: Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel
+// random numbers: as easy as 1, 2, 3", SC'11), a counter-based generator: four
+// 32-bit words from a 128-bit counter and a 64-bit key, no state carried
+// between calls.  Sweep s of a chain uses the key key_of(run_seed, s); the
+// counter names the site (or link, edge, replica pair), a tag that separates
+// the uses, and a block number for sites that draw more than four words.
+//
+// Every draw is a pure function of (key, counter), so a CPU replay sees the
+// same uniforms in any update order.  (The first outputs of Pcg32(key, site)
+// for consecutive sites come from states on an arithmetic progression, and
+// XSH-RR of such states is strongly correlated at fixed site separations;
+// Philox has no such structure.)
+struct U4 {
+    std::uint32_t x, y, z, w;
+};
+
+constexpr std::uint32_t kTagIsingHi = 0;   // Ising site uniforms, high 16 bits
+constexpr std::uint32_t kTagIsingLo = 1;   // Ising site uniforms, low 16 bits
+constexpr std::uint32_t kTagSite = 2;      // sequential per-site / per-link streams
+constexpr std::uint32_t kTagWolffCtrl = 3; // Wolff axis and seed site
+constexpr std::uint32_t kTagWolffEdge = 4; // Wolff per-edge uniforms
+constexpr std::uint32_t kTagSwap = 5;      // parallel-tempering swaps
+
+constexpr std::uint32_t kPhiloxM0 = 0xD2511F53u, kPhiloxM1 = 0xCD9E8D57u;
+constexpr std::uint32_t kPhiloxW0 = 0x9E3779B9u, kPhiloxW1 = 0xBB67AE85u;
+
+// a ^ b ^ c in one v_bitop3_b32 (truth table 0x96), a full-rate op on gfx950
+__device__ __forceinline__ std::uint32_t xor3(std::uint32_t a, std::uint32_t b, std::uint32_t c)
 {
-    const std::uint64_t inc = (stream << 1u) | 1u;
-    return (seed + inc) * kPcgMult + inc;
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
 }
-constexpr std::uint64_t kStreamStride = 2ull * kPcgMult + 2ull;
 
-// XSH-RR output of a 64-bit state: rotr32(((old >> 18) ^ old) >> 27, old >> 59).
-// ((old >> 18) ^ old) >> 27 == (old >> 27) ^ (old >> 45), whose low 32 bits are
-// alignbit(hi, lo, 27) ^ (hi >> 13); the rotate is alignbit(x, x, rot).
-__device__ __forceinline__ std::uint32_t pcg_out(std::uint64_t old)
+__device__ __forceinline__ U4 philox_round(U4 c, std::uint32_t k0, std::uint32_t k1)
 {
-    const std::uint32_t hi = static_cast<std::uint32_t>(old >> 32);
-    const std::uint32_t lo = static_cast<std::uint32_t>(old);
-    const std::uint32_t xs = __builtin_amdgcn_alignbit(hi, lo, 27) ^ (hi >> 13);
-    return __builtin_amdgcn_alignbit(xs, xs, hi >> 27);
+    const std::uint64_t p0 = static_cast<std::uint64_t>(kPhiloxM0) * c.x;
+    const std::uint64_t p1 = static_cast<std::uint64_t>(kPhiloxM1) * c.z;
+    return U4{xor3(static_cast<std::uint32_t>(p1 >> 32), c.y, k0), static_cast<std::uint32_t>(p1),
+              xor3(static_cast<std::uint32_t>(p0 >> 32), c.w, k1), static_cast<std::uint32_t>(p0)};
 }
 
-// A full Pcg32 for kernels that need several draws per site.
-struct Pcg {
-    std::uint64_t state;
-    std::uint64_t inc;
+__device__ __forceinline__ U4 philox(U4 c, std::uint64_t key)
+{
+    std::uint32_t k0 = static_cast<std::uint32_t>(key), k1 = static_cast<std::uint32_t>(key >> 32);
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) {
+            k0 += kPhiloxW0;
+            k1 += kPhiloxW1;
+        }
+        c = philox_round(c, k0, k1);
+    }
+    return c;
+}
 
-    __device__ __forceinline__ void seed(std::uint64_t s, std::uint64_t stream)
+__host__ __device__ __forceinline__ std::uint32_t word_of(const U4& v, std::uint32_t k)
+{
+    return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w;
+}
+
+// Sequential draws of one site (link, replica, ...): words x, y, z, w of
+// philox((id_lo, id_hi, tag, n), key) for n = 0, 1, 2, ...
+struct Stream {
+    std::uint64_t key;
+    std::uint32_t id0, id1, tag, n;
+    U4 buf;
+    std::uint32_t pos;
+
+    __device__ __forceinline__ void init(std::uint64_t k, std::uint64_t id, std::uint32_t t)
     {
-        inc = (stream << 1u) | 1u;
-        state = (s + inc) * kPcgMult + inc; // two seeding steps folded
+        key = k;
+        id0 = static_cast<std::uint32_t>(id);
+        id1 = static_cast<std::uint32_t>(id >> 32);
+        tag = t;
+        n = 0;
+        pos = 4;
     }
     __device__ __forceinline__ std::uint32_t next()
     {
-        const std::uint64_t old = state;
-        state = old * kPcgMult + inc;
-        return pcg_out(old);
+        if (pos == 4) {
+            buf = philox(U4{id0, id1, tag, n}, key);
+            ++n;
+            pos = 0;
+        }
+        const std::uint32_t v = buf.x; // shift the block down one word
+        buf.x = buf.y;
+        buf.y = buf.z;
+        buf.z = buf.w;
+        ++pos;
+        return v;
     }
-    // rng.hpp:24-27: exactly k * 2^-32, in double.
+    // rng.hpp:24-27 semantics: exactly k * 2^-32, in double.
     __device__ __forceinline__ double uni() { return static_cast<double>(next()) * 0x1.0p-32; }
 };
+
+// The Metropolis uniform of Ising site i (32 bits, compared with the integer
+// threshold).  Site i = 512 r + 8 q + j: the high 16 bits come from
+// philox((q, r, kTagIsingHi, 0)) and the low 16 bits from the same counter with
+// kTagIsingLo.  Each word carries two sites: even j in words 0-1, odd j in
+// words 2-3, half (j >> 1) & 1.  A row of 8-site blocks so needs one Philox
+// call per block for the high halves of both colours; the low halves matter
+// only when a high half ties the threshold's.
+__host__ __device__ __forceinline__ std::uint32_t ising_half(const U4& v, std::uint32_t j)
+{
+    const std::uint32_t w = word_of(v, (j & 1u) * 2u + (j >> 2));
+    return ((j >> 1) & 1u) ? (w >> 16) : (w & 0xFFFFu);
+}
+__device__ __forceinline__ std::uint32_t ising_u32(std::uint64_t key, std::uint32_t i)
+{
+    const std::uint32_t q = (i >> 3) & 63u, r = i >> 9, j = i & 7u;
+    const U4 hi = philox(U4{q, r, kTagIsingHi, 0u}, key);
+    const U4 lo = philox(U4{q, r, kTagIsingLo, 0u}, key);
+    return (ising_half(hi, j) << 16) | ising_half(lo, j);
+}
+
+// The Wolff activation uniform of edge e: word e & 3 of philox((e >> 2, 0, tag, 0)).
+__device__ __forceinline__ std::uint32_t edge_u32(std::uint64_t key, std::uint32_t e,
+                                                           std::uint32_t tag = kTagWolffEdge)
+{
+    return word_of(philox(U4{e >> 2, 0u, tag, 0u}, key), e & 3u);
+}
 
 // The accept test of driver.hpp:84, log(u) < log_p with u = k 2^-32: decided
 // in fp32 when the two sides are more than 1e-3 apart (the fp32 logarithm is

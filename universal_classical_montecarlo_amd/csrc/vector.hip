@@ -6,8 +6,8 @@
 // (coalesced per component).  One launch per colour class: (x + y + z) & 1 on
 // a periodic simple-cubic lattice with even L, (x + y) & 1 on an even square
 // lattice, a greedy colouring otherwise.  Each thread owns one site of the
-// class and draws its proposal and accept uniforms from its own
-// Pcg32(key, site) stream in the reference's order.
+// class and draws its proposal and accept uniforms, in the reference's order,
+// from its own counter-based stream (Stream(key, site, kTagSite)).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -49,7 +49,7 @@ __device__ __forceinline__ float uf(std::uint32_t k) { return static_cast<float>
 
 // dof/vector.hpp:35-61 in fp32: N == 2 angle, N >= 3 height then recursion.
 template <int N>
-__device__ __forceinline__ void sphere(Pcg& r, float* out)
+__device__ __forceinline__ void sphere(Stream& r, float* out)
 {
     if constexpr (N == 2) {
         const float phi = static_cast<float>(2.0 * kPi) * uf(r.next());
@@ -70,7 +70,7 @@ __device__ __forceinline__ void sphere(Pcg& r, float* out)
 // dof/vector.hpp:79-118 in fp32: uniform oriented 2-frame by projection and
 // rejection, rotation by delta (2u - 1) in that plane.
 template <int N>
-__device__ __forceinline__ void rotate(Pcg& r, float delta, const float* cur, float* out)
+__device__ __forceinline__ void rotate(Stream& r, float delta, const float* cur, float* out)
 {
     float u[N], v[N];
     sphere<N>(r, u);
@@ -90,6 +90,21 @@ __device__ __forceinline__ void rotate(Pcg& r, float delta, const float* cur, fl
             const float inv = 1.0f / sqrtf(r2);
 #pragma unroll
             for (int k = 0; k < N; ++k) v[k] *= inv;
+            // second Gram-Schmidt pass: when v was nearly parallel to u, the
+            // first pass leaves an fp32 component along u of ~1e-7 / |v_perp|;
+            // the double-precision frame of the reference has none
+            float p2 = 0.f;
+#pragma unroll
+            for (int k = 0; k < N; ++k) p2 += v[k] * u[k];
+            float q2 = 0.f;
+#pragma unroll
+            for (int k = 0; k < N; ++k) {
+                v[k] -= p2 * u[k];
+                q2 += v[k] * v[k];
+            }
+            const float inv2 = rsqrtf(q2);
+#pragma unroll
+            for (int k = 0; k < N; ++k) v[k] *= inv2;
             break;
         }
     }
@@ -149,8 +164,8 @@ __global__ __launch_bounds__(256) void vector_colour(VecArgs a)
             }
         }
         (void)deg;
-        Pcg r;
-        r.seed(a.key, i);
+        Stream r;
+        r.init(a.key, i, kTagSite);
         if (a.kind == 0) sphere<N>(r, p);
         else rotate<N>(r, a.delta, s, p);
         // example/vector_spin.hpp:50-60: -J sum_j (p.s_j - s.s_j) = -J (p - s).h

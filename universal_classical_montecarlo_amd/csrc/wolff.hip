@@ -6,7 +6,7 @@
 // cluster it samples is the connected component of the seed in the random
 // graph of bonds activated with p = 1 - exp(-beta dE_bond), dE_bond taken on
 // the pre-flip spins (driver.hpp:180-200).  Here every undirected edge e draws
-// its own uniform from Pcg32(k_f, 1 + e), so the component does not depend on
+// its own uniform edge_u32(k_f, e), so the component does not depend on
 // the traversal order and a parallel breadth-first growth reproduces the CPU
 // replay exactly.
 //
@@ -60,7 +60,7 @@ __device__ __forceinline__ double dmul(double a, double b) { return __dmul_rn(a,
 __device__ __forceinline__ double dadd(double a, double b) { return __dadd_rn(a, b); }
 
 template <int N>
-__device__ void sphere_d(Pcg& r, double* out)
+__device__ void sphere_d(Stream& r, double* out)
 {
     if constexpr (N == 1) {
         out[0] = r.uni() < 0.5 ? -1.0 : 1.0;
@@ -123,8 +123,8 @@ __global__ __launch_bounds__(kThreads) void wolff_flips(WolffArgs a)
         const std::uint64_t key = key_of(a.run_seed, kWolffCounterBase + a.flip0 + fi);
         if (tid == 0) {
             // driver.hpp:154-159: the axis first, then the seed site
-            Pcg ctl;
-            ctl.seed(key, 0);
+            Stream ctl;
+            ctl.init(key, 0, kTagWolffCtrl);
             double r[4] = {1.0, 0.0, 0.0, 0.0};
             if constexpr (N > 1) sphere_d<N>(ctl, r);
             for (int k = 0; k < 4; ++k) s_r[k] = r[k];
@@ -163,8 +163,7 @@ __global__ __launch_bounds__(kThreads) void wolff_flips(WolffArgs a)
                         j = a.bonds[q];
                         const std::uint32_t sj = a.stamp[j];
                         if (sj != mark) {
-                            const std::uint64_t es = first_state(key, 1ull + a.edge_id[q]);
-                            const std::uint32_t u = pcg_out(es);
+                            const std::uint32_t u = edge_u32(key, a.edge_id[q]);
                             bool act;
                             if constexpr (N == 1) {
                                 const bool aligned = fld.proj(i, r) * fld.proj(j, r) > 0.0;

@@ -13,7 +13,7 @@
 //    values of the tile and its first ring give the tile's bond energies), and
 //    the last workgroup to finish performs the replica-exchange step for all
 //    ladders (no grid barrier).  Fields ping-pong between two buffers, so halos
-//    are always read unmodified; the per-site Pcg32 stream makes the
+//    are always read unmodified; the counter-based per-site stream makes the
 //    redundant halo updates bit-identical to the owner's.
 #include <hip/hip_runtime.h>
 
@@ -63,8 +63,8 @@ __global__ __launch_bounds__(256) void xy_colour(XyArgs a)
     if (tt < a.count) {
         const std::uint32_t i = a.sites[tt];
         const float ti = a.t[i];
-        Pcg r;
-        r.seed(a.key, i);
+        Stream r;
+        r.init(a.key, i, kTagSite);
         const float tp = u1_shift(a.width, ti, r.next());
         float d = 0.f;
         for (std::uint32_t q = a.offsets[i]; q < a.offsets[i + 1]; ++q) {
@@ -192,8 +192,8 @@ __global__ __launch_bounds__(256) void pt_sweep(PtArgs a)
             const float ti = tl[ly * W + lx];
             const float nb[4] = {tl[ly * W + lx + 1], tl[ly * W + lx - 1], tl[(ly + 1) * W + lx],
                                  tl[(ly - 1) * W + lx]};
-            Pcg rg;
-            rg.seed(key, static_cast<std::uint64_t>(y) * L + x);
+            Stream rg;
+            rg.init(key, static_cast<std::uint64_t>(y) * L + x, kTagSite);
             const float tp = u1_shift(a.width, ti, rg.next());
             const float dE = xy_dE(a.J, tp, ti, nb);
             const double log_p = -beta * static_cast<double>(dE) + 0.0;
@@ -250,8 +250,8 @@ __global__ __launch_bounds__(256) void pt_sweep(PtArgs a)
             const double eb = __hip_atomic_load(&a.energy[rb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             // min(1, exp((beta_k - beta_k+1)(E_a - E_b)))
             const double log_p = (a.betas[k] - a.betas[k + 1]) * (ea - eb);
-            Pcg sw;
-            sw.seed(pkey, k);
+            Stream sw;
+            sw.init(pkey, k, kTagSwap);
             const bool swap = log_p >= 0.0 || log(sw.uni()) < log_p;
             a.swap_try[k] += 1;
             if (swap) {
