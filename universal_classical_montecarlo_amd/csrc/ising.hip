@@ -28,6 +28,15 @@
 #include "ucmc_core.hpp"
 #include "ucmc_device.hpp"
 
+// Rows per wave block of the fused kernel (even, <= 62), and how many rows
+// ahead of their use its Philox words are computed.
+#ifndef UCMC_FUSED_H
+#define UCMC_FUSED_H 16
+#endif
+#ifndef UCMC_PHILOX_AHEAD
+#define UCMC_PHILOX_AHEAD 2
+#endif
+
 namespace ucmc {
 
 void square_csr(System& s);
@@ -181,15 +190,18 @@ __device__ __noinline__ std::uint32_t resolve_ties(std::uint32_t out, std::uint3
     return out;
 }
 
-// Decide the four sites of one 4b-form word C with neighbour counts S.  wa, wb:
-// the high-half Philox words of these sites; (q, r, odd): the lane's block and
-// the half of the row, for the low halves on a tie.
-template <bool COUNT>
-__device__ __forceinline__ std::uint32_t decide(std::uint32_t C, std::uint32_t S, std::uint32_t wa,
-                                                std::uint32_t wb, const FusedTables& tb,
-                                                std::uint64_t key, std::uint32_t q, std::uint32_t r,
-                                                bool odd, std::uint32_t& acc)
+// The fast half of one word's decision, and what resolving its ties needs.
+struct WordDecision {
+    std::uint32_t C, S, wa, wb, out;
+    std::uint64_t tie;   // lanes in which some byte's high half ties
+};
+
+// Decide the four sites of one 4b-form word C with neighbour counts S; wa, wb
+// are the high-half Philox words of these sites.
+__device__ __forceinline__ WordDecision decide_fast(std::uint32_t C, std::uint32_t S, std::uint32_t wa,
+                                                    std::uint32_t wb, const FusedTables& tb)
 {
+    WordDecision d{C, S, wa, wb, 0u, 0u};
     const std::uint32_t Cf = C ^ 0x04040404u;
     std::uint32_t t[4];
 #pragma unroll
@@ -198,11 +210,9 @@ __device__ __forceinline__ std::uint32_t decide(std::uint32_t C, std::uint32_t S
         const std::uint32_t addr = __builtin_amdgcn_perm(C, S, 0x0C0C0000u | ((4u + k) << 8) | k);
         t[k] = *reinterpret_cast<const std::uint32_t*>(reinterpret_cast<const char*>(tb.hi) + addr);
     }
-    std::uint32_t out = select_lt(C, Cf, wa, wb, t);
-    if (ties(wa, wb, t))
-        out = resolve_ties(out, C, S, wa, wb, &tb, key, q, r, odd);
-    if (COUNT) acc += __builtin_popcount(out ^ C);
-    return out;
+    d.out = select_lt(C, Cf, wa, wb, t);
+    d.tie = ties(wa, wb, t);
+    return d;
 }
 
 template <int H, bool COUNT>
@@ -239,8 +249,43 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
         return static_cast<std::uint32_t>(t);
     };
 
-    // Every row of the window is requested before anything waits on memory:
-    // one memory latency per wave instead of one per row.
+    // ---- halo pre-pass: one lane per window row 1 .. H + 2 (y0-1 .. y0+H) ----
+    // Lane l owns row Y = y0 - 1 + l and the two sites just outside the band,
+    // xl = x_band - 1 (odd) and xr = x_band + 512 (even).  Its ten bytes are
+    // requested first, then every row of the window: loads complete in order,
+    // so the pre-pass waits for its own bytes only and the sweep below for one
+    // row at a time.
+    const std::uint8_t* inb = reinterpret_cast<const std::uint8_t*>(a.in);
+    const bool halo_lane = active && lane < static_cast<std::uint32_t>(H + 2);
+    std::uint32_t hY = 0, hv[10] = {};
+    const std::uint32_t xb = band * 512u;
+    const std::uint32_t xl = xb == 0 ? a.Lx - 1u : xb - 1u;   // odd, >= 1
+    const std::uint32_t xr = xb + 512u == a.Lx ? 0u : xb + 512u;
+    if (halo_lane) {
+        std::uint32_t Y = y0 + lane + a.Ly - 1u;               // < 3 Ly
+        if (Y >= a.Ly) Y -= a.Ly;
+        if (Y >= a.Ly) Y -= a.Ly;
+        hY = Y;
+        const std::uint32_t Yu = Y == 0 ? a.Ly - 1u : Y - 1u;
+        const std::uint32_t Yd = Y + 1u == a.Ly ? 0u : Y + 1u;
+        const std::uint32_t xlr = xl + 1u == a.Lx ? 0u : xl + 1u;
+        const std::uint32_t xrl = xr == 0 ? a.Lx - 1u : xr - 1u;
+        const std::size_t r0 = static_cast<std::size_t>(Y) * a.Lx;
+        const std::size_t ru = static_cast<std::size_t>(Yu) * a.Lx;
+        const std::size_t rd = static_cast<std::size_t>(Yd) * a.Lx;
+        // left site: (left, centre) as one aligned 16-bit load, right, up, down;
+        // right site: (centre, right) as one 16-bit load, left, up, down.  They
+        // are split only after the row loads are issued (below).
+        hv[0] = *reinterpret_cast<const std::uint16_t*>(inb + r0 + xl - 1u);
+        hv[2] = inb[r0 + xlr];
+        hv[3] = inb[ru + xl];
+        hv[4] = inb[rd + xl];
+        hv[5] = *reinterpret_cast<const std::uint16_t*>(inb + r0 + xr);
+        hv[6] = inb[r0 + xrl];
+        hv[8] = inb[ru + xr];
+        hv[9] = inb[rd + xr];
+    }
+
     uint2 raw[H + 4];
 #pragma unroll
     for (int j = 0; j < H + 4; ++j) raw[j] = load_raw(a.in, row_words, rowY(j), col);
@@ -255,33 +300,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
                 tb.lo[256 * c + n] = t == 0xFFFFFFFFu ? 0u : t & 0xFFFFu;
             }
     }
-    __syncthreads();   // the row loads stay in flight across it
+    __syncthreads();   // the loads stay in flight across it
 
-    // ---- halo pre-pass: one lane per window row 1 .. H + 2 (y0-1 .. y0+H) ----
-    // Lane l owns row Y = y0 - 1 + l and the two sites just outside the band,
-    // xl = x_band - 1 (odd) and xr = x_band + 512 (even).  All ten bytes it
-    // needs are requested at once; both candidate values are computed and the
-    // row parity picks one, so the pre-pass costs one memory latency.
-    const std::uint8_t* inb = reinterpret_cast<const std::uint8_t*>(a.in);
-    const bool halo_lane = active && lane < static_cast<std::uint32_t>(H + 2);
     if (halo_lane) {
-        std::uint32_t Y = y0 + lane + a.Ly - 1u;               // < 3 Ly
-        if (Y >= a.Ly) Y -= a.Ly;
-        if (Y >= a.Ly) Y -= a.Ly;
-        const std::uint32_t Yu = Y == 0 ? a.Ly - 1u : Y - 1u;
-        const std::uint32_t Yd = Y + 1u == a.Ly ? 0u : Y + 1u;
-        const std::uint32_t xb = band * 512u;
-        const std::uint32_t xl = xb == 0 ? a.Lx - 1u : xb - 1u;   // odd, >= 1
-        const std::uint32_t xr = xb + 512u == a.Lx ? 0u : xb + 512u;
-        const std::uint32_t xlr = xl + 1u == a.Lx ? 0u : xl + 1u;
-        const std::uint32_t xrl = xr == 0 ? a.Lx - 1u : xr - 1u;
-        const std::size_t r0 = static_cast<std::size_t>(Y) * a.Lx;
-        const std::size_t ru = static_cast<std::size_t>(Yu) * a.Lx;
-        const std::size_t rd = static_cast<std::size_t>(Yd) * a.Lx;
-        const std::uint32_t l_c = inb[r0 + xl], l_l = inb[r0 + xl - 1u], l_r = inb[r0 + xlr];
-        const std::uint32_t l_u = inb[ru + xl], l_d = inb[rd + xl];
-        const std::uint32_t r_c = inb[r0 + xr], r_l = inb[r0 + xrl], r_r = inb[r0 + xr + 1u];
-        const std::uint32_t r_u = inb[ru + xr], r_d = inb[rd + xr];
+        // an empty volatile asm that takes the bytes: nothing that reads them is
+        // hoisted above the row loads (their wait here is vmcnt(H + 4))
+        asm volatile("" : "+v"(hv[0]), "+v"(hv[2]), "+v"(hv[3]), "+v"(hv[4]), "+v"(hv[5]), "+v"(hv[6]),
+                          "+v"(hv[8]), "+v"(hv[9]));
+        hv[1] = hv[0] & 0xFFu;   // x_l - 1
+        hv[0] >>= 8;             // x_l
+        hv[7] = hv[5] >> 8;      // x_r + 1
+        hv[5] &= 0xFFu;          // x_r
+        const std::uint32_t Y = hY;
         auto b4 = [](std::uint32_t v) { return v & 4u; };   // +1 (0x01) -> 0, -1 (0xFF) -> 4
         auto update0 = [&](std::uint32_t x, std::uint32_t c, std::uint32_t s) -> std::uint32_t {
             // colour-0 update of site (x, Y) from its unmodified neighbours
@@ -294,9 +324,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
             return acc ? (c ^ 4u) : c;
         };
         const bool even = (Y & 1u) == 0;
-        const std::uint32_t lc = b4(l_c), rc = b4(r_c);
-        const std::uint32_t lnew = update0(xl, lc, b4(l_l) + b4(l_r) + b4(l_u) + b4(l_d));
-        const std::uint32_t rnew = update0(xr, rc, b4(r_l) + b4(r_r) + b4(r_u) + b4(r_d));
+        const std::uint32_t lc = b4(hv[0]), rc = b4(hv[5]);
+        const std::uint32_t lnew = update0(xl, lc, b4(hv[1]) + b4(hv[2]) + b4(hv[3]) + b4(hv[4]));
+        const std::uint32_t rnew = update0(xr, rc, b4(hv[6]) + b4(hv[7]) + b4(hv[8]) + b4(hv[9]));
         // left (odd x): even rows need the old value, odd rows the colour-0 value;
         // right (even x): even rows need the colour-0 value, odd rows the old one
         haloL[wid][lane] = (even ? lc : lnew) << 24;
@@ -310,49 +340,85 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
     U4 P[H + 4];
     auto philox_row = [&](int j) {
         P[j] = philox(U4{lane, rowY(j) * a.bands + band, kTagIsingHi, 0u}, a.key);
+        // an empty volatile asm that takes the words: keeps the rounds here
+        // rather than sunk next to their first use
+        asm volatile("" : "+v"(P[j].x), "+v"(P[j].y), "+v"(P[j].z), "+v"(P[j].w));
     };
-    // even-x half of window row j (neighbours: odd half of the row, with the
-    // left halo for lane 0, and the even halves of rows j - 1, j + 1)
-    auto updE = [&](Row& cur, const Row& up, const Row& dn, int j, bool count) {
-        const std::uint32_t Oprev = from_prev_lane(cur.O, haloL[wid][j - 1]);
-        const std::uint32_t Lft = __builtin_amdgcn_alignbit(cur.O, Oprev, 24);
-        const std::uint32_t S = Lft + cur.O + up.E + dn.E;
-        std::uint32_t c = 0;
-        cur.E = decide<COUNT>(cur.E, S, P[j].x, P[j].y, tb, a.key, lane, rowY(j) * a.bands + band, false, c);
-        if (count) acc += c;
+    // neighbour counts of the even-x half of row j (odd half of the row, with the
+    // left halo for lane 0, even halves of rows j - 1, j + 1) and of the odd-x half
+    auto sumE = [&](const Row (&R)[H + 4], int j) {
+        const std::uint32_t Oprev = from_prev_lane(R[j].O, haloL[wid][j - 1]);
+        const std::uint32_t Lft = __builtin_amdgcn_alignbit(R[j].O, Oprev, 24);
+        return Lft + R[j].O + R[j - 1].E + R[j + 1].E;
     };
-    auto updO = [&](Row& cur, const Row& up, const Row& dn, int j, bool count) {
-        const std::uint32_t Enext = from_next_lane(cur.E, haloR[wid][j - 1]);
-        const std::uint32_t Rgt = __builtin_amdgcn_alignbit(Enext, cur.E, 8);
-        const std::uint32_t S = cur.E + Rgt + up.O + dn.O;
-        std::uint32_t c = 0;
-        cur.O = decide<COUNT>(cur.O, S, P[j].z, P[j].w, tb, a.key, lane, rowY(j) * a.bands + band, true, c);
-        if (count) acc += c;
+    auto sumO = [&](const Row (&R)[H + 4], int j) {
+        const std::uint32_t Enext = from_next_lane(R[j].E, haloR[wid][j - 1]);
+        const std::uint32_t Rgt = __builtin_amdgcn_alignbit(Enext, R[j].E, 8);
+        return R[j].E + Rgt + R[j - 1].O + R[j + 1].O;
     };
     // y0 is even, so window row j has the parity of j: colour 0 is the even-x
-    // half on even j and the odd-x half on odd j.  Rows 1 and H + 2 (y0 - 1 and
-    // y0 + H) get their colour-0 update only as neighbours of the block's own
-    // rows; their owners update, store and count them.
-    auto colour0 = [&](Row (&R)[H + 4], int j) {
-        philox_row(j);
-        if (j & 1) updO(R[j], R[j - 1], R[j + 1], j, j != 1 && j != H + 2);
-        else updE(R[j], R[j - 1], R[j + 1], j, j != 1 && j != H + 2);
+    // half on even j and the odd-x half on odd j.
+    auto fast0 = [&](const Row (&R)[H + 4], int j) {
+        return (j & 1) ? decide_fast(R[j].O, sumO(R, j), P[j].z, P[j].w, tb)
+                       : decide_fast(R[j].E, sumE(R, j), P[j].x, P[j].y, tb);
     };
-    auto colour1 = [&](Row (&R)[H + 4], int j) {
-        if (j & 1) updE(R[j], R[j - 1], R[j + 1], j, true);
-        else updO(R[j], R[j - 1], R[j + 1], j, true);
+    auto fast1 = [&](const Row (&R)[H + 4], int j) {
+        return (j & 1) ? decide_fast(R[j].E, sumE(R, j), P[j].x, P[j].y, tb)
+                       : decide_fast(R[j].O, sumO(R, j), P[j].z, P[j].w, tb);
+    };
+    auto resolve = [&](WordDecision& d, int j, bool odd_half) {
+        d.out = resolve_ties(d.out, d.C, d.S, d.wa, d.wb, &tb, a.key, lane, rowY(j) * a.bands + band,
+                             odd_half);
+    };
+    // Rows 1 and H + 2 (y0 - 1 and y0 + H) get their colour-0 update only as
+    // neighbours of the block's own rows; their owners update, store and count them.
+    auto put0 = [&](Row (&R)[H + 4], int j, const WordDecision& d) {
+        if (j & 1) R[j].O = d.out;
+        else R[j].E = d.out;
+        if (COUNT && j != 1 && j != H + 2) acc += __builtin_popcount(d.out ^ d.C);
+    };
+    auto put1 = [&](Row (&R)[H + 4], int j, const WordDecision& d) {
+        if (j & 1) R[j].E = d.out;
+        else R[j].O = d.out;
+        if (COUNT) acc += __builtin_popcount(d.out ^ d.C);
     };
 
+    // Step j updates colour 0 of row j + 2 and colour 1 of row j.  The two are
+    // independent (colour 1 of row j reads colour 0 of rows j - 1 .. j + 1, done
+    // in earlier steps), so their ties share one rarely taken branch and the
+    // step is one scheduling region.  The Philox words of a row are computed
+    // kAhead steps before their first use, so their ten dependent rounds
+    // overlap the decisions in between.
+    constexpr int kAhead = UCMC_PHILOX_AHEAD;
     Row R[H + 4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) R[j] = convert_row(raw[j]);
-    colour0(R, 1);
-    colour0(R, 2);
+    for (int j = 1; j <= 3 + kAhead && j <= H + 2; ++j) philox_row(j);
+#pragma unroll
+    for (int j = 0; j < 5; ++j) R[j] = convert_row(raw[j]);
+#pragma unroll
+    for (int j = 1; j <= 3; ++j) {
+        WordDecision d = fast0(R, j);
+        if (d.tie) resolve(d, j, j & 1);
+        put0(R, j, d);
+    }
 #pragma unroll
     for (int j = 2; j <= H + 1; ++j) {
-        R[j + 2] = convert_row(raw[j + 2]);
-        colour0(R, j + 1);
-        colour1(R, j);
+        if (j + 2 + kAhead <= H + 2) philox_row(j + 2 + kAhead);
+        if (j + 2 <= H + 2) {
+            if (j + 3 <= H + 3 && j + 3 >= 5) R[j + 3] = convert_row(raw[j + 3]);
+            WordDecision d0 = fast0(R, j + 2);
+            WordDecision d1 = fast1(R, j);
+            if (d0.tie | d1.tie) {
+                if (d0.tie) resolve(d0, j + 2, (j + 2) & 1);
+                if (d1.tie) resolve(d1, j, !(j & 1));
+            }
+            put0(R, j + 2, d0);
+            put1(R, j, d1);
+        } else {
+            WordDecision d1 = fast1(R, j);
+            if (d1.tie) resolve(d1, j, !(j & 1));
+            put1(R, j, d1);
+        }
         store_row(a.out, row_words, rowY(j), col, R[j]);
     }
 
@@ -410,10 +476,6 @@ __global__ __launch_bounds__(256) void ising_reduce_csr(const std::int8_t* f, co
     }
 }
 
-// Rows per wave block of the fused kernel (even, <= 62).
-#ifndef UCMC_FUSED_H
-#define UCMC_FUSED_H 16
-#endif
 constexpr int kFusedH = UCMC_FUSED_H;
 
 bool fused_ok(const System& s)
