@@ -111,11 +111,15 @@ def suite(quick=False):
     rate = L ** 3 * 50 / (ms * 1e-3)
     out["heisenberg_128_metropolis"] = {"site_updates_per_s": rate,
                                         "hbm_frac_24B": 24 * rate / 1e9 / HBM_PEAK_GBS}
+    # Wolff reflection clusters (label propagation at this size), after 500
+    # untimed flips from the Metropolis-equilibrated state
+    v.metropolis(0.693, 200, 4, proposal=abi.PROP_ROTATION, width=0.5)
+    v.wolff(0.693, 500, 3)
     t0 = time.perf_counter()
-    sizes, _ = v.wolff(0.693, 200, 3)
+    sizes, _ = v.wolff(0.693, 500, 3, flip0=500)
     dt = time.perf_counter() - t0
     out["heisenberg_128_wolff"] = {"flipped_sites_per_s": float(sizes.sum()) / dt,
-                                   "mean_cluster": float(sizes.mean())}
+                                   "flips_per_s": 500 / dt, "mean_cluster": float(sizes.mean())}
     v.close()
     # C3: SU(3) 24^4 (16^4 quick), beta = 5.7 set once (model), SUShift 0.2, 4 hits
     for L in ((16,) if quick else (16, 24)):

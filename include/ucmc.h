@@ -106,6 +106,9 @@ typedef struct ucmc_system ucmc_system;
 #define UCMC_RECORD_ORDER    2u /* device FP64 order parameter after each step  */
 #define UCMC_RECORD_ACCEPTS  4u /* number of update() calls in each sweep       */
 #define UCMC_FORCE_SIMPLE    8u /* Ising: use the one-launch-per-colour kernel  */
+#define UCMC_WOLFF_BFS      16u /* Wolff: grow clusters breadth-first in one workgroup */
+#define UCMC_WOLFF_LABELS   32u /* Wolff: label all bond components grid-wide (default
+                                   from 32768 sites on); both give the same chain */
 
 /* The reference's PairGraph in CSR form (lattice/pair_graph.hpp:96-107):
  * row i is bonds[offsets[i] .. offsets[i+1]), bond.j only (NoPayload). */
@@ -254,21 +257,40 @@ int ucmc_pt_autotune(ucmc_system* sys, double width, uint32_t* tile, double* bes
 int ucmc_slmc_collect(ucmc_system* sys, double beta, uint64_t burn, uint64_t nconfigs,
                       uint64_t every, uint64_t run_seed, double* correlations);
 /* Least-squares fit theta = (E0/N, J1, J2, J3) of E_eff = E0 - sum_k J_k C_k
- * through the normal equations: X^T X by bf16 MFMA (hi/lo split), solved in
- * FP64.  gram (nullable) receives the 16 x 16 fp32 X^T X. */
+ * through the centred normal equations: the column means in FP64, the
+ * centred Gram matrix sum_c (x_c - mean)(x_c - mean)^T by bf16 MFMA (hi/lo
+ * split), the 3 x 3 system for J in FP64, E0/N from the means.  gram
+ * (nullable) receives that 16 x 16 fp32 matrix (rows / columns 1..3: -C_k/N,
+ * 15: E/N). */
 int ucmc_slmc_fit_normal(ucmc_system* sys, float* gram, double* theta);
+/* Local-field fit of the same effective couplings: over every site i of every
+ * collected configuration, least squares of the true energy change of
+ * flipping s_i against the effective one, sum_k J_k 2 s_i h_k(i) (h_k: the
+ * shell-k neighbour sum).  Rows are built per configuration on the device and
+ * their 16 x 16 Gram matrix accumulated with bf16 MFMA into an FP64 total
+ * during ucmc_slmc_collect; the 3 x 3 system is solved in FP64.  theta =
+ * (0, J1, J2, J3); gram (nullable) receives the FP64 total (rows / columns
+ * 0..2: features, 15: the true energy change). */
+int ucmc_slmc_fit_local(ucmc_system* sys, double* gram, double* theta);
 /* Forward + backward of the loss sum_c (x_c.theta - y_c)^2 over the stored
  * rows: out[0] = the loss sum, out[1..4] = its gradient. */
 int ucmc_slmc_loss_grad(ucmc_system* sys, const double* theta, double* out);
-/* `iters` Adam steps (lr, 0.9, 0.999) on the mean loss, from theta (in/out). */
+/* `iters` Adam steps (lr, 0.9, 0.999) on the mean loss, from theta (in/out);
+ * the steps are taken on the standardised features (each of columns 1..3
+ * centred and scaled to unit variance) and theta converted back. */
 int ucmc_slmc_fit_adam(ucmc_system* sys, int iters, double lr, double* theta, double* mean_loss);
 /* nflips SLMC steps: Wolff cluster on the effective couplings J_eff[3] (the
  * growth uses the keyed edge uniforms of edge 6 owner + slot), accepted
  * with min(1, exp(-beta (dE_true - dE_eff))) from the control stream's second
  * draw (k_f = key(run_seed, 3 2^61 + flip0 + f)).  series (nullable): true
- * energy, magnetisation, cluster size and 0/1 acceptance per step. */
+ * energy, magnetisation, cluster size and 0/1 acceptance per step.
+ * move_stats (nullable): per step the boundary sums of the proposed cluster,
+ * (c_1, c_2, c_3, P) with c_k = sum of s_i s_j over shell-k bonds leaving the
+ * cluster and P the summed products of the plaquettes it cuts an odd number
+ * of times, so dE_true = 2 J1 c_1 + 2 K P and dE_eff = 2 sum_k J_k c_k. */
 int ucmc_slmc_run(ucmc_system* sys, const double* J_eff, double beta, uint64_t run_seed,
-                  uint64_t flip0, uint64_t nflips, ucmc_observables* series, uint8_t* accepted);
+                  uint64_t flip0, uint64_t nflips, ucmc_observables* series, uint8_t* accepted,
+                  double* move_stats);
 
 /* ---- measurement ---- */
 /* Runs the same launches as ucmc_metropolis_run (no recording) `repeats`

@@ -588,16 +588,16 @@ void orc_slmc_keyed(size_t L, int8_t* s, double J1, double K, const double* Jeff
                 }
             }
         }
-        double dt = 0.0, de = 0.0;
+        /* boundary sums as integers (shells c0..c2, plaquettes P), combined
+         * once: dE_true = 2 J1 c0 + 2 K P, dE_eff = 2 (J0 c0 + J1 c1 + J2 c2) */
+        long long c[3] = {0, 0, 0}, P = 0;
         for (size_t k = 0; k < tail; ++k) {
             const uint32_t i = q[k];
             const size_t x = i % L, y = i / L;
             for (int d = 0; d < 12; ++d) {
                 const uint32_t j = (uint32_t)fpi_id(L, x + 2 * L + (size_t)(long)dx[d], y + 2 * L + (size_t)(long)dy[d]);
                 if (in[j]) continue;
-                const double b = 2.0 * s[i] * s[j];
-                de += Jeff[shell[d]] * b;
-                if (d < 4) dt += J1 * b;
+                c[shell[d]] += s[i] * s[j];
             }
         }
         /* plaquettes with an odd number of sites in the cluster change sign */
@@ -606,8 +606,10 @@ void orc_slmc_keyed(size_t L, int8_t* s, double J1, double K, const double* Jeff
                 const size_t cs[4] = {fpi_id(L, x0, y0), fpi_id(L, x0 + 1, y0), fpi_id(L, x0, y0 + 1),
                                       fpi_id(L, x0 + 1, y0 + 1)};
                 const int inside = in[cs[0]] + in[cs[1]] + in[cs[2]] + in[cs[3]];
-                if (inside & 1) dt += 2.0 * K * (double)(s[cs[0]] * s[cs[1]] * s[cs[2]] * s[cs[3]]);
+                if (inside & 1) P += s[cs[0]] * s[cs[1]] * s[cs[2]] * s[cs[3]];
             }
+        const double dt = 2.0 * J1 * (double)c[0] + 2.0 * K * (double)P;
+        const double de = 2.0 * (Jeff[0] * (double)c[0] + Jeff[1] * (double)c[1] + Jeff[2] * (double)c[2]);
         const double log_p = -beta * (dt - de) + 0.0;
         const int acc = log_p >= 0.0 || log(orc_pcg32_uni(&ctl)) < log_p;
         if (acc)

@@ -155,8 +155,13 @@ def test_slmc_training_kernels(gpu):
     X32, y32 = f32(X), f32(y)
     theta_ref, *_ = np.linalg.lstsq(X32, y32, rcond=None)
     theta, G = sysm.slmc_fit_normal()
-    assert np.allclose(G[:4, :4], X32.T @ X32, rtol=2e-4, atol=1e-3)
-    assert np.allclose(G[:4, 15], X32.T @ y32, rtol=2e-4, atol=1e-3)
+    # the device centres the rows on their FP64 column means before the MFMA Gram
+    Xc = X32[:, 1:] - f32(X32[:, 1:].astype(np.float64).mean(axis=0))
+    yc = y32 - f32(y32.astype(np.float64).mean())
+    Xc64, yc64 = Xc.astype(np.float64), yc.astype(np.float64)
+    scale = np.abs(Xc64.T @ Xc64).max()
+    assert np.allclose(G[1:4, 1:4], Xc64.T @ Xc64, rtol=1e-4, atol=1e-4 * scale)
+    assert np.allclose(G[1:4, 15], Xc64.T @ yc64, rtol=1e-4, atol=1e-4 * scale)
     assert np.allclose(theta, theta_ref, atol=2e-3)
     # gradient of sum (X theta - y)^2 at an arbitrary point
     th = np.array([0.1, 0.9, 0.05, -0.02])
@@ -170,8 +175,45 @@ def test_slmc_training_kernels(gpu):
     sysm.close()
 
 
-def test_slmc_step_matches_replay(gpu):
-    L = 16
+def local_rows(s, L, J1, K):
+    """numpy restatement of the local training rows: f_k = 2 s_i h_k(i) and
+    the true single-flip energy change (orc_fpi_energy_diff_flip)."""
+    g = s.reshape(L, L).astype(np.int64)   # [y, x]
+    sh = lambda dx, dy: np.roll(np.roll(g, -dy, axis=0), -dx, axis=1)
+    h1 = sh(1, 0) + sh(-1, 0) + sh(0, 1) + sh(0, -1)
+    h2 = sh(1, 1) + sh(1, -1) + sh(-1, -1) + sh(-1, 1)
+    h3 = sh(2, 0) + sh(0, 2) + sh(-2, 0) + sh(0, -2)
+    plaq = g * sh(1, 0) * sh(0, 1) * sh(1, 1)   # plaquette with lower-left corner (x, y)
+    pq = plaq + np.roll(plaq, 1, axis=1) + np.roll(plaq, 1, axis=0) + np.roll(np.roll(plaq, 1, axis=0), 1, axis=1)
+    F = np.stack([2 * g * h1, 2 * g * h2, 2 * g * h3], axis=-1).reshape(-1, 3).astype(np.float64)
+    y = (2.0 * g * (J1 * h1) + 2.0 * K * pq).reshape(-1)
+    return F, y
+
+
+def test_slmc_local_fit(gpu):
+    """Device local rows + MFMA Gram vs numpy on one collected configuration,
+    and the 3 x 3 solve vs numpy least squares; the rows also agree with the
+    oracle's single-flip energy change."""
+    L = 32
+    sysm = abi.System.plaquette_ising(L, 1.0, 0.2)
+    sysm.slmc_collect(0.42, 150, 1, 1, 9)
+    cfg = sysm.download()
+    F, y = local_rows(cfg, L, 1.0, 0.2)
+    o = oracle()
+    for i in [0, 5, L + 3, L * L - 1]:
+        assert y[i] == pytest.approx(o.orc_fpi_energy_diff_flip(L, cfg, 1.0, 0.2, i), abs=1e-12)
+    theta, G = sysm.slmc_fit_local()
+    y32 = f32(y).astype(np.float64)
+    assert np.allclose(G[:3, :3], F.T @ F, rtol=0, atol=1e-9)          # integer features, exact
+    assert np.allclose(G[:3, 15], F.T @ y32, rtol=1e-4, atol=1e-2)
+    J_ref, *_ = np.linalg.lstsq(F, y32, rcond=None)
+    assert np.allclose(theta[1:], J_ref, atol=1e-3)
+    assert theta[0] == 0.0
+    sysm.close()
+
+
+@pytest.mark.parametrize("L", [16, 64])   # breadth-first growth / label propagation
+def test_slmc_step_matches_replay(gpu, L):
     sysm = abi.System.plaquette_ising(L, 1.0, 0.2)
     s = random_spins(L * L, 6, 2)
     sysm.upload(s)

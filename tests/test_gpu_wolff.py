@@ -19,15 +19,19 @@ def edge_ids(off, bonds):
     return e
 
 
+PATHS = [abi.WOLFF_BFS, abi.WOLFF_LABELS]
+
+
+@pytest.mark.parametrize("path", PATHS)
 @pytest.mark.parametrize("L,beta,J", [(16, 0.44, 1), (8, 0.3, 1), (32, 0.2, -1), (64, 0.45, 2)])
-def test_ising_wolff_bit_exact(gpu, L, beta, J):
-    print(f"ising wolff L={L} beta={beta}", flush=True)
+def test_ising_wolff_bit_exact(gpu, L, beta, J, path):
+    print(f"ising wolff L={L} beta={beta} path={path}", flush=True)
     off, bonds = csr_square(L, L)
     sysm = abi.System.ising(off, bonds, J)
     spins = random_spins(L * L, 19, 2)
     sysm.upload(spins)
     nflips = 300
-    sizes, ser = sysm.wolff(beta, nflips, 55, flip0=10, record=True)
+    sizes, ser = sysm.wolff(beta, nflips, 55, flip0=10, record=True, flags=path)
     got = sysm.download()
     want = spins.copy()
     wsz = np.zeros(nflips, np.uint64)
@@ -58,9 +62,10 @@ def test_ising_wolff_dims_created_4096(gpu):
     sysm.close()
 
 
+@pytest.mark.parametrize("path", PATHS)
 @pytest.mark.parametrize("N,geom,beta", [(3, ("cubic", 8), 0.693), (2, ("square", 16), 1.0),
                                          (3, ("square", 8), 0.3)])
-def test_vector_wolff(gpu, N, geom, beta):
+def test_vector_wolff(gpu, N, geom, beta, path):
     kind, L = geom
     off, bonds = csr_cubic(L) if kind == "cubic" else csr_square(L, L)
     n = len(off) - 1
@@ -68,7 +73,7 @@ def test_vector_wolff(gpu, N, geom, beta):
     spins = random_unit(N, n, 23)
     sysm.upload(spins)
     nflips = 200
-    sizes, ser = sysm.wolff(beta, nflips, 77, record=True)
+    sizes, ser = sysm.wolff(beta, nflips, 77, record=True, flags=path)
     got = sysm.download()
     want = spins.copy()
     wsz = np.zeros(nflips, np.uint64)
@@ -78,4 +83,26 @@ def test_vector_wolff(gpu, N, geom, beta):
     assert np.max(np.abs(got - want)) < 2e-6
     o = sysm.observe()
     assert abs(ser[-1].energy - o.energy) < 1e-5 * max(1.0, abs(o.energy))
+    sysm.close()
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_wolff_sweep_touch_target(gpu, path):
+    """Wolff::sweep (driver.hpp:213-217): flips until the cluster sizes add up to
+    N; the flips not run report size 0, on both cluster builders."""
+    L = 32
+    off, bonds = csr_square(L, L)
+    sysm = abi.System.ising(off, bonds, 1)
+    spins = random_spins(L * L, 4, 1)
+    sysm.upload(spins)
+    sizes, _ = sysm.wolff(0.3, 400, 8, flags=path, touch_target=L * L)
+    k = int(np.argmax(sizes == 0)) if (sizes == 0).any() else len(sizes)
+    assert sizes[:k].sum() >= L * L and sizes[:k - 1].sum() < L * L
+    assert (sizes[k:] == 0).all()
+    wsz = np.zeros(k, np.uint64)
+    want = spins.copy()
+    oracle().orc_ising_wolff_keyed(L * L, off, bonds, edge_ids(off, bonds), want, 1, 0.3, 8, 0, k,
+                                   wsz, None)
+    assert np.array_equal(sizes[:k], wsz)
+    assert np.array_equal(sysm.download(), want)
     sysm.close()

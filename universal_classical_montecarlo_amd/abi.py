@@ -27,6 +27,7 @@ ISING, VECTOR, U1_GAUGE, SUN_GAUGE, XY_ANGLE, PLAQ_ISING = 1, 2, 3, 4, 5, 6
 PROP_FLIP, PROP_SPHERE, PROP_ROTATION, PROP_U1_SHIFT, PROP_SU_SHIFT = 1, 2, 3, 4, 5
 SCHED_CSR, SCHED_CHECKERBOARD, SCHED_FUSED, SCHED_CUBIC, SCHED_LINK8, SCHED_FOUR = 1, 2, 3, 4, 5, 6
 RECORD_ENERGY, RECORD_ORDER, RECORD_ACCEPTS, FORCE_SIMPLE = 1, 2, 4, 8
+WOLFF_BFS, WOLFF_LABELS = 16, 32
 
 # Every function the header declares (checked by the CPU test suite).
 EXPORTED = [
@@ -37,7 +38,7 @@ EXPORTED = [
     "ucmc_field_download", "ucmc_metropolis_run", "ucmc_metropolis_trace", "ucmc_wolff_run",
     "ucmc_observe", "ucmc_reunitarize", "ucmc_metropolis_bench", "ucmc_pt_create",
     "ucmc_pt_run", "ucmc_pt_stats", "ucmc_pt_autotune", "ucmc_slmc_collect",
-    "ucmc_slmc_fit_normal", "ucmc_slmc_loss_grad", "ucmc_slmc_fit_adam", "ucmc_slmc_run",
+    "ucmc_slmc_fit_normal", "ucmc_slmc_fit_local", "ucmc_slmc_loss_grad", "ucmc_slmc_fit_adam", "ucmc_slmc_run",
     "ucmc_measure_copy",
 ]
 
@@ -125,10 +126,11 @@ def lib():
     L.ucmc_slmc_collect.argtypes = [vp, C.c_double, C.c_uint64, C.c_uint64, C.c_uint64,
                                     C.c_uint64, C.c_void_p]
     L.ucmc_slmc_fit_normal.argtypes = [vp, C.c_void_p, C.c_void_p]
+    L.ucmc_slmc_fit_local.argtypes = [vp, C.c_void_p, C.c_void_p]
     L.ucmc_slmc_loss_grad.argtypes = [vp, C.c_void_p, C.c_void_p]
     L.ucmc_slmc_fit_adam.argtypes = [vp, C.c_int, C.c_double, C.c_void_p, C.POINTER(C.c_double)]
     L.ucmc_slmc_run.argtypes = [vp, C.c_void_p, C.c_double, C.c_uint64, C.c_uint64, C.c_uint64,
-                                C.c_void_p, C.c_void_p]
+                                C.c_void_p, C.c_void_p, C.c_void_p]
     L.ucmc_measure_copy.argtypes = [C.c_size_t, C.c_int, C.POINTER(C.c_double)]
     L.ucmc_pt_autotune.argtypes = [vp, C.c_double, C.POINTER(C.c_uint32), C.POINTER(C.c_double)]
     L.ucmc_metropolis_bench.argtypes = [vp, C.POINTER(MetropolisParams), C.c_int,
@@ -156,7 +158,7 @@ def measure_copy(nbytes, repeats=20):
 
 
 def _ptr(a):
-    return a.ctypes.data_as(C.c_void_p)
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
 def _graph(offsets, bonds):
@@ -272,6 +274,12 @@ class System:
         check(lib().ucmc_slmc_fit_normal(self.h, _ptr(G), _ptr(th)))
         return th, G
 
+    def slmc_fit_local(self):
+        G = np.zeros((16, 16))
+        th = np.zeros(4)
+        check(lib().ucmc_slmc_fit_local(self.h, _ptr(G), _ptr(th)))
+        return th, G
+
     def slmc_loss_grad(self, theta):
         th = np.ascontiguousarray(theta, dtype=np.float64)
         out = np.zeros(5)
@@ -284,13 +292,14 @@ class System:
         check(lib().ucmc_slmc_fit_adam(self.h, int(iters), float(lr), _ptr(th), C.byref(loss)))
         return th, loss.value
 
-    def slmc_run(self, J_eff, beta, nflips, run_seed, flip0=0, record=True):
+    def slmc_run(self, J_eff, beta, nflips, run_seed, flip0=0, record=True, stats=False):
         J = np.ascontiguousarray(J_eff, dtype=np.float64)
         acc = np.zeros(nflips, np.uint8)
         series = (Observables * nflips)() if record else None
+        st = np.zeros((nflips, 4)) if stats else None
         check(lib().ucmc_slmc_run(self.h, _ptr(J), float(beta), int(run_seed), int(flip0),
-                                  int(nflips), series, _ptr(acc)))
-        return series, acc
+                                  int(nflips), series, _ptr(acc), _ptr(st)))
+        return (series, acc, st) if stats else (series, acc)
 
     def close(self):
         if self.h:

@@ -519,6 +519,14 @@ int ucmc_slmc_fit_normal(ucmc_system* sys, float* gram, double* theta)
     });
 }
 
+int ucmc_slmc_fit_local(ucmc_system* sys, double* gram, double* theta)
+{
+    return guard([&] {
+        if (!theta) fail(UCMC_E_INVALID_ARGUMENT, "theta is null");
+        slmc_fit_local(*plaq(sys), gram, theta);
+    });
+}
+
 int ucmc_slmc_loss_grad(ucmc_system* sys, const double* theta, double* out)
 {
     return guard([&] {
@@ -536,11 +544,12 @@ int ucmc_slmc_fit_adam(ucmc_system* sys, int iters, double lr, double* theta, do
 }
 
 int ucmc_slmc_run(ucmc_system* sys, const double* J_eff, double beta, uint64_t run_seed,
-                  uint64_t flip0, uint64_t nflips, ucmc_observables* series, uint8_t* accepted)
+                  uint64_t flip0, uint64_t nflips, ucmc_observables* series, uint8_t* accepted,
+                  double* move_stats)
 {
     return guard([&] {
         if (!J_eff) fail(UCMC_E_INVALID_ARGUMENT, "null couplings");
-        slmc_run(*plaq(sys), J_eff, beta, run_seed, flip0, nflips, series, accepted);
+        slmc_run(*plaq(sys), J_eff, beta, run_seed, flip0, nflips, series, accepted, move_stats);
     });
 }
 

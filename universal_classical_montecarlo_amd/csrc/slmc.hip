@@ -26,6 +26,7 @@
 #include <cstring>
 #include <vector>
 
+#include "labels.hpp"
 #include "ucmc_core.hpp"
 #include "ucmc_device.hpp"
 
@@ -34,6 +35,7 @@ namespace ucmc {
 namespace {
 
 constexpr std::uint64_t kSlmcCounterBase = 3ull << 61;
+constexpr std::size_t kSlmcLabelMinSites = 4096;   // from 64 x 64 on, label propagation
 
 // --------------------------------------------------- plain Metropolis --
 struct FpiArgs {
@@ -90,9 +92,12 @@ __global__ __launch_bounds__(256) void fpi_colour(FpiArgs a)
 __global__ __launch_bounds__(256) void fpi_correlations(const std::int8_t* cfgs, std::uint32_t L,
                                                         std::size_t stride, long long* out)
 {
-    const std::int8_t* s = cfgs + blockIdx.x * stride;
+    // configuration blockIdx.y, its sites spread over gridDim.x blocks; the
+    // block sums are added into out (zeroed by the caller)
+    const std::int8_t* s = cfgs + blockIdx.y * stride;
+    out += 5 * static_cast<std::size_t>(blockIdx.y);
     long long c1 = 0, c2 = 0, c3 = 0, pp = 0, m = 0;
-    for (std::uint32_t i = threadIdx.x; i < L * L; i += blockDim.x) {
+    for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < L * L; i += gridDim.x * blockDim.x) {
         const std::uint32_t x = i % L, y = i / L;
         const int si = s[i];
         const int r = at(s, L, x + 1, y), d = at(s, L, x, y + 1);
@@ -118,7 +123,9 @@ __global__ __launch_bounds__(256) void fpi_correlations(const std::int8_t* cfgs,
         atomicAdd(reinterpret_cast<unsigned long long*>(&red[4]), static_cast<unsigned long long>(m));
     }
     __syncthreads();
-    if (threadIdx.x < 5) out[blockIdx.x * 5 + threadIdx.x] = red[threadIdx.x];
+    if (threadIdx.x < 5)
+        atomicAdd(reinterpret_cast<unsigned long long*>(out + threadIdx.x),
+                  static_cast<unsigned long long>(red[threadIdx.x]));
 }
 
 // Feature rows x = [1, -C1/N, -C2/N, -C3/N, 0 ... 0, E/N] (16 fp32 per config,
@@ -153,22 +160,40 @@ __device__ __forceinline__ float from_bf16(short h)
     return __uint_as_float(static_cast<std::uint32_t>(static_cast<unsigned short>(h)) << 16);
 }
 
-// G = X^T X (16 x 16, fp32) over n configs with bf16 MFMA: each wave owns a
-// slice of 32-config chunks; lane l supplies feature (l % 16) of configs
-// 8 (l / 16) .. +7 of the chunk, which is both the A (16 x 32) and the
-// B (32 x 16) operand.  The result lane l holds G[4 (l / 16) + i][l % 16].
-__global__ __launch_bounds__(256) void slmc_gram_mfma(const float* X, std::uint32_t n, float* G)
+// Column means of the feature rows (FP64 sums), mu[f] += sum_c X[c][f] / n.
+__global__ __launch_bounds__(256) void slmc_column_means(const float* X, std::uint32_t n, double* mu)
+{
+    const std::uint32_t f = threadIdx.x & 15u;
+    double s = 0.0;
+    for (std::uint32_t c = blockIdx.x * 16u + (threadIdx.x >> 4); c < n; c += gridDim.x * 16u)
+        s += X[16 * static_cast<std::size_t>(c) + f];
+    // lanes f, f + 16, f + 32, f + 48 of a wave hold the same column
+    s += __shfl_xor(s, 16, 64);
+    s += __shfl_xor(s, 32, 64);
+    if ((threadIdx.x & 63u) < 16u) atomicAdd(&mu[f], s / static_cast<double>(n));
+}
+
+// G = sum_c (x_c - mu)(x_c - mu)^T (16 x 16, fp32) over n configs with bf16
+// MFMA.  The features of one chain fluctuate by ~1/sqrt(N) around their means,
+// so X^T X itself would bury the covariance below bf16 resolution; the
+// centred rows keep it.  Each wave owns a slice of 32-config chunks; lane l
+// supplies feature (l % 16) of configs 8 (l / 16) .. +7 of the chunk, which is
+// both the A (16 x 32) and the B (32 x 16) operand, split into bf16 hi + lo
+// parts (hi.hi + hi.lo + lo.hi).  The result lane l holds G[4 (l / 16) + i][l % 16].
+__global__ __launch_bounds__(256) void slmc_gram_mfma(const float* X, std::uint32_t n, const double* mu,
+                                                      float* G)
 {
     const std::uint32_t lane = threadIdx.x & 63u, wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const std::uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
     const std::uint32_t f = lane & 15u, grp = lane >> 4;
+    const float m = mu ? static_cast<float>(mu[f]) : 0.f;
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     const std::uint32_t chunks = (n + 31) / 32;
     for (std::uint32_t ch = wave; ch < chunks; ch += nwaves) {
         bf16x8 hi, lo;
         for (int j = 0; j < 8; ++j) {
             const std::uint32_t c = ch * 32 + grp * 8 + j;
-            const float v = c < n ? X[16 * static_cast<std::size_t>(c) + f] : 0.f;
+            const float v = c < n ? X[16 * static_cast<std::size_t>(c) + f] - m : 0.f;
             const short h = to_bf16(v);
             hi[j] = h;
             lo[j] = to_bf16(v - from_bf16(h));
@@ -180,45 +205,137 @@ __global__ __launch_bounds__(256) void slmc_gram_mfma(const float* X, std::uint3
     for (int i = 0; i < 4; ++i) atomicAdd(&G[(4 * grp + i) * 16 + f], acc[i]);
 }
 
-// Solves the 4 x 4 normal equations (features 0..3, right-hand side column 15)
-// by Gaussian elimination with partial pivoting in FP64.
-__global__ void slmc_solve(const float* G, double* theta)
+// Least squares on the centred normal equations in FP64: the three couplings
+// from cov(x_1..3) J = cov(x_1..3, y) by Gaussian elimination with partial
+// pivoting, then theta_0 = mean(y) - sum_k J_k mean(x_k).
+__global__ void slmc_solve(const float* G, const double* mu, double* theta)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double A[4][5];
-    for (int r = 0; r < 4; ++r) {
-        for (int c = 0; c < 4; ++c) A[r][c] = G[r * 16 + c];
-        A[r][4] = G[r * 16 + 15];
+    double A[3][4];
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c) A[r][c] = G[(r + 1) * 16 + (c + 1)];
+        A[r][3] = G[(r + 1) * 16 + 15];
     }
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < 3; ++c) {
         int piv = c;
-        for (int r = c + 1; r < 4; ++r)
+        for (int r = c + 1; r < 3; ++r)
             if (fabs(A[r][c]) > fabs(A[piv][c])) piv = r;
-        for (int k = 0; k < 5; ++k) {
+        for (int k = 0; k < 4; ++k) {
             const double t = A[c][k];
             A[c][k] = A[piv][k];
             A[piv][k] = t;
         }
-        for (int r = 0; r < 4; ++r) {
+        for (int r = 0; r < 3; ++r) {
             if (r == c || A[c][c] == 0.0) continue;
             const double f = A[r][c] / A[c][c];
-            for (int k = c; k < 5; ++k) A[r][k] -= f * A[c][k];
+            for (int k = c; k < 4; ++k) A[r][k] -= f * A[c][k];
         }
     }
-    for (int r = 0; r < 4; ++r) theta[r] = A[r][r] != 0.0 ? A[r][4] / A[r][r] : 0.0;
+    double t0 = mu[15];
+    for (int r = 0; r < 3; ++r) {
+        theta[1 + r] = A[r][r] != 0.0 ? A[r][3] / A[r][r] : 0.0;
+        t0 -= theta[1 + r] * mu[1 + r];
+    }
+    theta[0] = t0;
+}
+
+// Neighbour direction d (0..11) of (x, y): shells (1,0), (1,1), (2,0); the
+// undirected edge of a forward direction is owned by (x, y), of a backward
+// one by the neighbour: id = 6 owner + slot.
+__constant__ int kDx[12] = {1, 0, -1, 0, 1, 1, -1, -1, 2, 0, -2, 0};
+__constant__ int kDy[12] = {0, 1, 0, -1, 1, -1, -1, 1, 0, 2, 0, -2};
+__constant__ int kShell[12] = {0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 2};
+__constant__ int kSlot[12] = {0, 1, 0, 1, 2, 3, 2, 3, 4, 5, 4, 5};
+__constant__ int kForward[12] = {1, 1, 0, 0, 1, 1, 0, 0, 1, 1, 0, 0};
+
+__device__ __forceinline__ std::uint32_t nbr(std::uint32_t L, std::uint32_t i, int d)
+{
+    const std::uint32_t x = i % L, y = i / L;
+    const std::uint32_t nx = (x + L + 2 * L + kDx[d]) % L, ny = (y + L + 2 * L + kDy[d]) % L;
+    return ny * L + nx;
+}
+
+// Local training rows of one configuration, one per site i: the features
+// f_k = 2 s_i h_k(i) (h_k: sum of the four shell-k neighbours) in columns 0..2
+// and the true energy change of flipping s_i, 2 J1 s_i h_1 + 2 K (sum of the
+// four plaquettes holding i), in column 15.  The effective model's change is
+// sum_k J_k f_k, so least squares over these rows fits exactly the local
+// energy differences a cluster flip adds up along its boundary.
+__global__ __launch_bounds__(256) void slmc_local_rows(const std::int8_t* s, std::uint32_t L, double J1, double K,
+                                                       float* X)
+{
+    const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= L * L) return;
+    const std::uint32_t x = i % L, y = i / L;
+    const int si = s[i];
+    int h[3] = {0, 0, 0};
+    for (int d = 0; d < 12; ++d) h[kShell[d]] += s[nbr(L, i, d)];
+    int pq = 0;
+    for (int py = -1; py <= 0; ++py)
+        for (int px = -1; px <= 0; ++px) {
+            const std::uint32_t x0 = (x + L + px) % L, y0 = (y + L + py) % L;
+            pq += at(s, L, x0, y0) * at(s, L, x0 + 1, y0) * at(s, L, x0, y0 + 1) * at(s, L, x0 + 1, y0 + 1);
+        }
+    float* r = X + 16 * static_cast<std::size_t>(i);
+    for (int k = 0; k < 16; ++k) r[k] = 0.f;
+    for (int k = 0; k < 3; ++k) r[k] = static_cast<float>(2 * si * h[k]);
+    r[15] = static_cast<float>(2.0 * static_cast<double>(si) * (J1 * static_cast<double>(h[0])) +
+                               2.0 * K * static_cast<double>(pq));
+}
+
+// G64 += G (FP64 running total over configurations), G = 0.
+__global__ void slmc_gram_accumulate(float* G, double* G64)
+{
+    const std::uint32_t k = threadIdx.x;
+    if (k < 256) {
+        G64[k] += static_cast<double>(G[k]);
+        G[k] = 0.f;
+    }
+}
+
+// Local fit: (J1, J2, J3) from G_ff J = G_fy (3 x 3, FP64, partial pivoting).
+__global__ void slmc_solve_local(const double* G, double* theta)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double A[3][4];
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c) A[r][c] = G[r * 16 + c];
+        A[r][3] = G[r * 16 + 15];
+    }
+    for (int c = 0; c < 3; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < 3; ++r)
+            if (fabs(A[r][c]) > fabs(A[piv][c])) piv = r;
+        for (int k = 0; k < 4; ++k) {
+            const double t = A[c][k];
+            A[c][k] = A[piv][k];
+            A[piv][k] = t;
+        }
+        for (int r = 0; r < 3; ++r) {
+            if (r == c || A[c][c] == 0.0) continue;
+            const double f = A[r][c] / A[c][c];
+            for (int k = c; k < 4; ++k) A[r][k] -= f * A[c][k];
+        }
+    }
+    theta[0] = 0.0;
+    for (int r = 0; r < 3; ++r) theta[1 + r] = A[r][r] != 0.0 ? A[r][3] / A[r][r] : 0.0;
 }
 
 // Forward + backward of the least-squares loss L = mean_c (x_c.theta - y_c)^2:
 // out[0] += sum (pred - y)^2, out[1..4] += sum 2 (pred - y) x_c (FP64 sums).
+// With mu, sd (nullable) the features 1..3 enter standardised, (x - mu) / sd.
 __global__ __launch_bounds__(256) void slmc_loss_grad(const float* X, std::uint32_t n, const double* theta,
-                                                      double* out)
+                                                      const double* mu, const double* sd, double* out)
 {
     double l = 0.0, g[4] = {0.0, 0.0, 0.0, 0.0};
     for (std::uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
-        const float* x = X + 16 * static_cast<std::size_t>(c);
+        const float* xr = X + 16 * static_cast<std::size_t>(c);
+        double x[4] = {xr[0], xr[1], xr[2], xr[3]};
+        if (mu)
+            for (int k = 1; k < 4; ++k) x[k] = (x[k] - mu[k]) / sd[k];
         double pred = 0.0;
         for (int k = 0; k < 4; ++k) pred += theta[k] * x[k];
-        const double r = pred - x[15];
+        const double r = pred - xr[15];
         l += r * r;
         for (int k = 0; k < 4; ++k) g[k] += 2.0 * r * x[k];
     }
@@ -244,21 +361,12 @@ __global__ void slmc_adam(double* theta, double* m, double* v, const double* lg,
 }
 
 // ------------------------------------------------------------ SLMC step --
-// Neighbour direction d (0..11) of (x, y): shells (1,0), (1,1), (2,0); the
-// undirected edge of a forward direction is owned by (x, y), of a backward
-// one by the neighbour: id = 6 owner + slot.
-__constant__ int kDx[12] = {1, 0, -1, 0, 1, 1, -1, -1, 2, 0, -2, 0};
-__constant__ int kDy[12] = {0, 1, 0, -1, 1, -1, -1, 1, 0, 2, 0, -2};
-__constant__ int kShell[12] = {0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 2};
-__constant__ int kSlot[12] = {0, 1, 0, 1, 2, 3, 2, 3, 4, 5, 4, 5};
-__constant__ int kForward[12] = {1, 1, 0, 0, 1, 1, 0, 0, 1, 1, 0, 0};
-
 struct SlmcArgs {
     std::int8_t* s;
     std::uint32_t L;
     std::uint32_t* stamp;
     std::uint32_t* queue;
-    double* out;              // per step: [size, accepted, dE_true, dE_eff, dM, pad]
+    double* out;              // per step: kOutStride doubles (slmc_out_record)
     std::uint64_t run_seed, flip0;
     std::uint32_t nflips, mark0;
     double beta, J1, K;
@@ -266,17 +374,69 @@ struct SlmcArgs {
     std::uint64_t thr[3][2];  // [shell][aligned]
 };
 
-__device__ __forceinline__ std::uint32_t nbr(std::uint32_t L, std::uint32_t i, int d)
+
+// Boundary sums of cluster site i (in(j): is j in the cluster): for each shell
+// k, c_k += s_i s_j over the neighbours j outside; the product of every
+// plaquette holding i whose first in-cluster corner is i and which has an odd
+// number of corners inside; and s_i (the magnetisation change is -2 sum s_i).
+template <class InCluster>
+__device__ __forceinline__ void bond_sums(const std::int8_t* s, std::uint32_t L, std::uint32_t i, InCluster in,
+                                          long long* cnt)
 {
+    const int si = s[i];
+    cnt[4] += si;
+    for (int d = 0; d < 12; ++d) {
+        const std::uint32_t j = nbr(L, i, d);
+        if (in(j)) continue;
+        cnt[kShell[d]] += si * s[j];
+    }
     const std::uint32_t x = i % L, y = i / L;
-    const std::uint32_t nx = (x + L + 2 * L + kDx[d]) % L, ny = (y + L + 2 * L + kDy[d]) % L;
-    return ny * L + nx;
+    for (int py = -1; py <= 0; ++py)
+        for (int px = -1; px <= 0; ++px) {
+            const std::uint32_t x0 = (x + L + px) % L, y0 = (y + L + py) % L;
+            const std::uint32_t cs[4] = {y0 * L + x0, y0 * L + (x0 + 1) % L, ((y0 + 1) % L) * L + x0,
+                                         ((y0 + 1) % L) * L + (x0 + 1) % L};
+            int inside = 0, first = -1, prod = 1;
+            for (int q = 0; q < 4; ++q) {
+                const bool inq = in(cs[q]);
+                inside += inq;
+                if (inq && first < 0) first = q;
+                prod *= s[cs[q]];
+            }
+            if (cs[first] == i && (inside & 1)) cnt[3] += prod;
+        }
+}
+
+// dE_true = 2 J1 c0 + 2 K P and dE_eff = 2 (J0 c0 + J1 c1 + J2 c2) of the
+// flip, in this order with no contraction (as the oracle).
+__device__ __forceinline__ void slmc_energies(double J1, double K, const double* Jeff, const long long* c,
+                                              double& dt, double& de)
+{
+    dt = __dadd_rn(__dmul_rn(__dmul_rn(2.0, J1), static_cast<double>(c[0])),
+                   __dmul_rn(__dmul_rn(2.0, K), static_cast<double>(c[3])));
+    de = __dmul_rn(2.0, __dadd_rn(__dadd_rn(__dmul_rn(Jeff[0], static_cast<double>(c[0])),
+                                            __dmul_rn(Jeff[1], static_cast<double>(c[1]))),
+                                  __dmul_rn(Jeff[2], static_cast<double>(c[2]))));
+}
+
+// One step's record: size, accepted, dE_true, dE_eff, dM, then the boundary
+// sums c_1, c_2, c_3, P.
+constexpr std::size_t kOutStride = 10;
+__device__ __forceinline__ void slmc_out_record(double* o, double size, int acc, double dt, double de,
+                                                const long long* c)
+{
+    o[0] = size;
+    o[1] = acc;
+    o[2] = dt;
+    o[3] = de;
+    o[4] = -2.0 * static_cast<double>(c[4]);
+    for (int k = 0; k < 4; ++k) o[5 + k] = static_cast<double>(c[k]);
 }
 
 __global__ __launch_bounds__(1024) void slmc_flips(SlmcArgs a)
 {
     __shared__ std::uint32_t s_head, s_tail;
-    __shared__ double s_red[4];
+    __shared__ long long s_cnt[5];
     __shared__ int s_accept;
     const std::uint32_t tid = threadIdx.x, lane = tid & 63u;
     const std::uint32_t L = a.L, n = L * L;
@@ -291,7 +451,7 @@ __global__ __launch_bounds__(1024) void slmc_flips(SlmcArgs a)
             a.queue[0] = seed;
             s_head = 0;
             s_tail = 1;
-            for (int k = 0; k < 4; ++k) s_red[k] = 0.0;
+            for (int k = 0; k < 5; ++k) s_cnt[k] = 0;
         }
         __syncthreads();
         for (;;) {
@@ -333,58 +493,29 @@ __global__ __launch_bounds__(1024) void slmc_flips(SlmcArgs a)
             __syncthreads();
         }
         const std::uint32_t size = s_tail;
-        // dE of flipping the cluster, true and effective, on the pre-flip field
-        double dt = 0.0, de = 0.0, dm = 0.0;
+        // dE of flipping the cluster, true and effective, on the pre-flip field:
+        // integer bond / plaquette sums, combined once (order independent)
+        long long cnt[5] = {0, 0, 0, 0, 0};   // shell sums c0..c2, plaquettes, magnetisation
         for (std::uint32_t idx = tid; idx < size; idx += blockDim.x) {
             const std::uint32_t i = a.queue[idx];
-            const int si = a.s[i];
-            dm += -2.0 * si;
-            for (int d = 0; d < 12; ++d) {
-                const std::uint32_t j = nbr(L, i, d);
-                if (a.stamp[j] == mark) continue;
-                const double b = 2.0 * si * a.s[j];
-                de += a.Jeff[kShell[d]] * b;
-                if (d < 4) dt += a.J1 * b;
-            }
-            // plaquettes holding i, each counted by its first in-cluster corner
-            const std::uint32_t x = i % L, y = i / L;
-            for (int py = -1; py <= 0; ++py)
-                for (int px = -1; px <= 0; ++px) {
-                    const std::uint32_t x0 = (x + L + px) % L, y0 = (y + L + py) % L;
-                    const std::uint32_t cs[4] = {y0 * L + x0, y0 * L + (x0 + 1) % L, ((y0 + 1) % L) * L + x0,
-                                                 ((y0 + 1) % L) * L + (x0 + 1) % L};
-                    int inside = 0, first = -1, prod = 1;
-                    for (int q = 0; q < 4; ++q) {
-                        const bool in = a.stamp[cs[q]] == mark;
-                        inside += in;
-                        if (in && first < 0) first = q;
-                        prod *= a.s[cs[q]];
-                    }
-                    if (cs[first] == i && (inside & 1)) dt += 2.0 * a.K * prod;
-                }
+            bond_sums(a.s, L, i, [&](std::uint32_t j) { return a.stamp[j] == mark; }, cnt);
         }
-        dt = wave_sum(dt);
-        de = wave_sum(de);
-        dm = wave_sum(dm);
-        if (lane == 0) {
-            atomicAdd(&s_red[0], dt);
-            atomicAdd(&s_red[1], de);
-            atomicAdd(&s_red[2], dm);
-        }
+        for (int k = 0; k < 5; ++k)
+            cnt[k] = static_cast<long long>(wave_sum_u64(static_cast<unsigned long long>(cnt[k])));
+        if (lane == 0)
+            for (int k = 0; k < 5; ++k)
+                atomicAdd(reinterpret_cast<unsigned long long*>(&s_cnt[k]), static_cast<unsigned long long>(cnt[k]));
         __syncthreads();
         if (tid == 0) {
             // min(1, exp(-beta (dE_true - dE_eff))), the draw after the seed draw
+            double dt, de;
+            slmc_energies(a.J1, a.K, a.Jeff, s_cnt, dt, de);
             Stream ctl;
             ctl.init(key, 0, kTagWolffCtrl);
             (void)ctl.next();
-            const double log_p = -a.beta * (s_red[0] - s_red[1]) + 0.0;
+            const double log_p = -a.beta * (dt - de) + 0.0;
             s_accept = (log_p >= 0.0 || log(ctl.uni()) < log_p) ? 1 : 0;
-            double* o = a.out + 6 * static_cast<std::size_t>(fi);
-            o[0] = size;
-            o[1] = s_accept;
-            o[2] = s_red[0];
-            o[3] = s_red[1];
-            o[4] = s_red[2];
+            slmc_out_record(a.out + kOutStride * static_cast<std::size_t>(fi), size, s_accept, dt, de, s_cnt);
         }
         __syncthreads();
         if (s_accept)
@@ -395,6 +526,94 @@ __global__ __launch_bounds__(1024) void slmc_flips(SlmcArgs a)
         __threadfence();
         __syncthreads();
     }
+}
+
+// ------------------------------------------------- label propagation path --
+// One SLMC step as six grid-wide kernels (the same cluster as slmc_flips):
+// parent[i] = i and the seed; every undirected edge (x, y) -> forward shell
+// neighbour activated with its keyed uniform and united; labels flattened;
+// integer boundary sums of the seed's component; the Metropolis-Hastings
+// decision; the flip.
+struct SlmcLabelArgs {
+    std::int8_t* s;
+    std::uint32_t L, n;
+    std::uint32_t* parent;
+    ClusterState* st;
+    double* out;              // this step's record (slmc_out_record)
+    std::uint64_t key;
+    double beta, J1, K;
+    double Jeff[3];
+    std::uint64_t thr[3][2];
+};
+
+__global__ __launch_bounds__(256) void slmc_label_begin(SlmcLabelArgs a)
+{
+    const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < a.n) a.parent[i] = i;
+    if (i == 0) {
+        Stream ctl;
+        ctl.init(a.key, 0, kTagWolffCtrl);
+        a.st->seed = static_cast<std::uint32_t>(ctl.uni() * static_cast<double>(a.n));
+        a.st->skip = 0;
+        for (int k = 0; k < 8; ++k) a.st->count[k] = 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void slmc_label_link(SlmcLabelArgs a)
+{
+    const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const int si = a.s[i];
+    for (int d = 0; d < 12; ++d) {
+        if (!kForward[d]) continue;   // each undirected edge from its owner
+        const std::uint32_t j = nbr(a.L, i, d);
+        const std::uint32_t u = edge_u32(a.key, 6u * i + kSlot[d]);
+        const bool aligned = si * a.s[j] > 0;
+        if (static_cast<std::uint64_t>(u) < a.thr[kShell[d]][aligned ? 1 : 0]) lab_unite(a.parent, i, j);
+    }
+}
+
+__global__ __launch_bounds__(256) void slmc_label_sums(SlmcLabelArgs a)
+{
+    const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const std::uint32_t root = a.parent[a.st->seed];
+    long long cnt[6] = {0, 0, 0, 0, 0, 0};   // c0..c2, plaquettes, sum s_i, size
+    if (i < a.n && a.parent[i] == root) {
+        bond_sums(a.s, a.L, i, [&](std::uint32_t j) { return a.parent[j] == root; }, cnt);
+        cnt[5] = 1;
+    }
+    const unsigned long long any = wave_sum_u64(static_cast<unsigned long long>(cnt[5]));
+    if (any == 0) return;   // wave-uniform
+    for (int k = 0; k < 5; ++k)
+        cnt[k] = static_cast<long long>(wave_sum_u64(static_cast<unsigned long long>(cnt[k])));
+    if ((threadIdx.x & 63u) == 0) {
+        for (int k = 0; k < 5; ++k) atomicAdd(&a.st->count[k], static_cast<unsigned long long>(cnt[k]));
+        atomicAdd(&a.st->count[5], any);
+    }
+}
+
+__global__ void slmc_label_decide(SlmcLabelArgs a)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    long long c[6];
+    for (int k = 0; k < 6; ++k) c[k] = static_cast<long long>(a.st->count[k]);
+    double dt, de;
+    slmc_energies(a.J1, a.K, a.Jeff, c, dt, de);
+    // min(1, exp(-beta (dE_true - dE_eff))), the draw after the seed draw
+    Stream ctl;
+    ctl.init(a.key, 0, kTagWolffCtrl);
+    (void)ctl.next();
+    const double log_p = -a.beta * (dt - de) + 0.0;
+    const int acc = (log_p >= 0.0 || log(ctl.uni()) < log_p) ? 1 : 0;
+    a.st->accept = static_cast<std::uint32_t>(acc);
+    slmc_out_record(a.out, static_cast<double>(c[5]), acc, dt, de, c);
+}
+
+__global__ __launch_bounds__(256) void slmc_label_apply(SlmcLabelArgs a)
+{
+    const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n || !a.st->accept) return;
+    if (a.parent[i] == a.parent[a.st->seed]) a.s[i] = static_cast<std::int8_t>(-a.s[i]);
 }
 
 void fpi_tables(const System& s, double beta, std::vector<std::uint32_t>& t)
@@ -432,9 +651,13 @@ void fpi_sweep(System& s, std::uint64_t key, const std::uint32_t* tbl, unsigned 
     }
 }
 
-void corr_launch(System& s, const std::int8_t* cfgs, std::uint32_t count, long long* out)
+// Pair / plaquette sums of `count` configurations into out[5 * count]; zeroes
+// out first unless the caller already did.
+void corr_launch(System& s, const std::int8_t* cfgs, std::uint32_t count, long long* out, bool zeroed = false)
 {
-    hipLaunchKernelGGL(fpi_correlations, dim3(count), dim3(256), 0, s.stream, cfgs,
+    if (!zeroed) UCMC_HIP(hipMemsetAsync(out, 0, static_cast<std::size_t>(count) * 5 * 8, s.stream));
+    const std::uint32_t blocks = static_cast<std::uint32_t>(std::min<std::size_t>((s.sites + 1023) / 1024, 64));
+    hipLaunchKernelGGL(fpi_correlations, dim3(blocks, count), dim3(256), 0, s.stream, cfgs,
                        static_cast<std::uint32_t>(s.L), s.sites, out);
     UCMC_HIP(hipGetLastError());
 }
@@ -491,7 +714,7 @@ void fpi_metropolis(System& s, const ucmc_metropolis_params& p, ucmc_observables
         fpi_sweep(s, key_of(p.run_seed, p.sweep0 + k), s.scratch2.as<std::uint32_t>(),
                   (rec && (p.flags & UCMC_RECORD_ACCEPTS)) ? slot : nullptr, d_dE, d_acc);
         if (rec && (p.flags & (UCMC_RECORD_ENERGY | UCMC_RECORD_ORDER)))
-            corr_launch(s, s.field[s.cur].as<std::int8_t>(), 1, reinterpret_cast<long long*>(slot + 1));
+            corr_launch(s, s.field[s.cur].as<std::int8_t>(), 1, reinterpret_cast<long long*>(slot + 1), true);
     }
     if (rec) {
         std::vector<long long> h(p.nsweeps * 6);
@@ -557,11 +780,29 @@ void slmc_collect(System& s, double beta, std::uint64_t burn, std::uint64_t ncon
     for (; sweep < burn; ++sweep) fpi_sweep(s, key_of(run_seed, sweep), s.scratch2.as<std::uint32_t>(), nullptr, nullptr, nullptr);
     DevBuf corr;
     corr.alloc(nconfigs * 5 * 8);
-    // correlations of each kept configuration, straight from the live field
+    UCMC_HIP(hipMemsetAsync(corr.p, 0, corr.bytes, s.stream));
+    // local Gram matrix over all kept configurations (FP64 total)
+    s.local_gram.alloc(256 * 8);
+    UCMC_HIP(hipMemsetAsync(s.local_gram.p, 0, 256 * 8, s.stream));
+    DevBuf rows, G;
+    rows.alloc(s.sites * 16 * 4);
+    G.alloc(256 * 4);
+    UCMC_HIP(hipMemsetAsync(G.p, 0, 256 * 4, s.stream));
+    const auto nsites = static_cast<std::uint32_t>(s.sites);
+    const std::uint32_t chunks = (nsites + 31) / 32;
+    const dim3 ggrid(std::max<std::uint32_t>(1, std::min<std::uint32_t>((chunks + 3) / 4, 256)));
+    // correlations and local rows of each kept configuration, straight from the live field
     for (std::uint64_t c = 0; c < nconfigs; ++c) {
         for (std::uint64_t k = 0; k < every; ++k, ++sweep)
             fpi_sweep(s, key_of(run_seed, sweep), s.scratch2.as<std::uint32_t>(), nullptr, nullptr, nullptr);
-        corr_launch(s, s.field[s.cur].as<std::int8_t>(), 1, corr.as<long long>() + 5 * c);
+        corr_launch(s, s.field[s.cur].as<std::int8_t>(), 1, corr.as<long long>() + 5 * c, true);
+        hipLaunchKernelGGL(slmc_local_rows, dim3((nsites + 255) / 256), dim3(256), 0, s.stream,
+                           s.field[s.cur].as<std::int8_t>(), static_cast<std::uint32_t>(s.L), s.J, s.K,
+                           rows.as<float>());
+        hipLaunchKernelGGL(slmc_gram_mfma, ggrid, dim3(256), 0, s.stream, rows.as<float>(), nsites,
+                           static_cast<const double*>(nullptr), G.as<float>());
+        hipLaunchKernelGGL(slmc_gram_accumulate, dim3(1), dim3(256), 0, s.stream, G.as<float>(),
+                           s.local_gram.as<double>());
     }
     s.flip_out.alloc(nconfigs * 16 * sizeof(float));
     const auto n = static_cast<std::uint32_t>(nconfigs);
@@ -581,15 +822,21 @@ void slmc_collect(System& s, double beta, std::uint64_t burn, std::uint64_t ncon
 void slmc_gram(System& s, float* G_out, double* theta_out)
 {
     if (!s.flip_out.p || s.n_edges == 0) fail(UCMC_E_INVALID_ARGUMENT, "no training data collected");
-    DevBuf G, th;
+    DevBuf G, th, mu;
     G.alloc(256 * 4);
     th.alloc(4 * 8);
+    mu.alloc(16 * 8);
     UCMC_HIP(hipMemsetAsync(G.p, 0, 256 * 4, s.stream));
+    UCMC_HIP(hipMemsetAsync(mu.p, 0, 16 * 8, s.stream));
     const auto n = static_cast<std::uint32_t>(s.n_edges);
+    const std::uint32_t mblocks = std::min<std::uint32_t>((n + 15) / 16, 256);
+    hipLaunchKernelGGL(slmc_column_means, dim3(mblocks), dim3(256), 0, s.stream, s.flip_out.as<float>(), n,
+                       mu.as<double>());
     const std::uint32_t chunks = (n + 31) / 32;
     const std::uint32_t blocks = std::max<std::uint32_t>(1, std::min<std::uint32_t>((chunks + 3) / 4, 256));
-    hipLaunchKernelGGL(slmc_gram_mfma, dim3(blocks), dim3(256), 0, s.stream, s.flip_out.as<float>(), n, G.as<float>());
-    hipLaunchKernelGGL(slmc_solve, dim3(1), dim3(64), 0, s.stream, G.as<float>(), th.as<double>());
+    hipLaunchKernelGGL(slmc_gram_mfma, dim3(blocks), dim3(256), 0, s.stream, s.flip_out.as<float>(), n,
+                       mu.as<double>(), G.as<float>());
+    hipLaunchKernelGGL(slmc_solve, dim3(1), dim3(64), 0, s.stream, G.as<float>(), mu.as<double>(), th.as<double>());
     UCMC_HIP(hipGetLastError());
     if (G_out) UCMC_HIP(hipMemcpyAsync(G_out, G.p, 256 * 4, hipMemcpyDeviceToHost, s.stream));
     if (theta_out) UCMC_HIP(hipMemcpyAsync(theta_out, th.p, 32, hipMemcpyDeviceToHost, s.stream));
@@ -606,45 +853,91 @@ void slmc_loss_grad_host(System& s, const double* theta, double* out5)
     UCMC_HIP(hipMemsetAsync(o.p, 0, 40, s.stream));
     const auto n = static_cast<std::uint32_t>(s.n_edges);
     hipLaunchKernelGGL(slmc_loss_grad, dim3(std::min<std::uint32_t>((n + 255) / 256, 256)), dim3(256), 0, s.stream,
-                       s.flip_out.as<float>(), n, th.as<double>(), o.as<double>());
+                       s.flip_out.as<float>(), n, th.as<double>(), nullptr, nullptr, o.as<double>());
     UCMC_HIP(hipGetLastError());
     UCMC_HIP(hipMemcpyAsync(out5, o.p, 40, hipMemcpyDeviceToHost, s.stream));
     UCMC_HIP(hipStreamSynchronize(s.stream));
 }
 
+void slmc_fit_local(System& s, double* gram_out, double* theta_out)
+{
+    if (!s.local_gram.p) fail(UCMC_E_INVALID_ARGUMENT, "no training data collected");
+    DevBuf th;
+    th.alloc(4 * 8);
+    hipLaunchKernelGGL(slmc_solve_local, dim3(1), dim3(64), 0, s.stream, s.local_gram.as<double>(), th.as<double>());
+    UCMC_HIP(hipGetLastError());
+    if (gram_out) UCMC_HIP(hipMemcpyAsync(gram_out, s.local_gram.p, 256 * 8, hipMemcpyDeviceToHost, s.stream));
+    if (theta_out) UCMC_HIP(hipMemcpyAsync(theta_out, th.p, 32, hipMemcpyDeviceToHost, s.stream));
+    UCMC_HIP(hipStreamSynchronize(s.stream));
+}
+
+// Adam on the standardised features: phi_k = theta_k sd_k, phi_0 = theta_0 +
+// sum_k theta_k mu_k, converted back at the end (the raw features of one chain
+// sit at ~1.5 with fluctuations of ~0.01, which makes the raw problem badly
+// conditioned for a first-order method).
 void slmc_adam_fit(System& s, int iters, double lr, double* theta, double* loss)
 {
     if (!s.flip_out.p || s.n_edges == 0) fail(UCMC_E_INVALID_ARGUMENT, "no training data collected");
-    DevBuf th, m, v, o;
+    const auto n = static_cast<std::uint32_t>(s.n_edges);
+    DevBuf th, m, v, o, mu, sd, G;
     th.alloc(32);
     m.alloc(32);
     v.alloc(32);
     o.alloc(40);
-    UCMC_HIP(hipMemcpyAsync(th.p, theta, 32, hipMemcpyHostToDevice, s.stream));
+    mu.alloc(16 * 8);
+    sd.alloc(16 * 8);
+    G.alloc(256 * 4);
+    UCMC_HIP(hipMemsetAsync(mu.p, 0, 16 * 8, s.stream));
+    UCMC_HIP(hipMemsetAsync(G.p, 0, 256 * 4, s.stream));
+    hipLaunchKernelGGL(slmc_column_means, dim3(std::min<std::uint32_t>((n + 15) / 16, 256)), dim3(256), 0, s.stream,
+                       s.flip_out.as<float>(), n, mu.as<double>());
+    const std::uint32_t chunks = (n + 31) / 32;
+    hipLaunchKernelGGL(slmc_gram_mfma, dim3(std::max<std::uint32_t>(1, std::min<std::uint32_t>((chunks + 3) / 4, 256))),
+                       dim3(256), 0, s.stream, s.flip_out.as<float>(), n, mu.as<double>(), G.as<float>());
+    double hmu[16];
+    float hG[256];
+    UCMC_HIP(hipMemcpyAsync(hmu, mu.p, sizeof hmu, hipMemcpyDeviceToHost, s.stream));
+    UCMC_HIP(hipMemcpyAsync(hG, G.p, sizeof hG, hipMemcpyDeviceToHost, s.stream));
+    UCMC_HIP(hipStreamSynchronize(s.stream));
+    double hsd[16];
+    for (int k = 0; k < 16; ++k) {
+        const double var = static_cast<double>(hG[k * 16 + k]) / n;
+        hsd[k] = var > 0.0 ? std::sqrt(var) : 1.0;
+    }
+    double phi[4] = {theta[0], theta[1] * hsd[1], theta[2] * hsd[2], theta[3] * hsd[3]};
+    for (int k = 1; k < 4; ++k) phi[0] += theta[k] * hmu[k];
+    UCMC_HIP(hipMemcpyAsync(sd.p, hsd, sizeof hsd, hipMemcpyHostToDevice, s.stream));
+    UCMC_HIP(hipMemcpyAsync(th.p, phi, 32, hipMemcpyHostToDevice, s.stream));
     UCMC_HIP(hipMemsetAsync(m.p, 0, 32, s.stream));
     UCMC_HIP(hipMemsetAsync(v.p, 0, 32, s.stream));
-    const auto n = static_cast<std::uint32_t>(s.n_edges);
     const dim3 grid(std::min<std::uint32_t>((n + 255) / 256, 256));
     for (int t = 1; t <= iters; ++t) {
         UCMC_HIP(hipMemsetAsync(o.p, 0, 40, s.stream));
         hipLaunchKernelGGL(slmc_loss_grad, grid, dim3(256), 0, s.stream, s.flip_out.as<float>(), n, th.as<double>(),
-                           o.as<double>());
+                           mu.as<double>(), sd.as<double>(), o.as<double>());
         hipLaunchKernelGGL(slmc_adam, dim3(1), dim3(64), 0, s.stream, th.as<double>(), m.as<double>(), v.as<double>(),
                            o.as<double>(), n, t, lr, 0.9, 0.999, 1e-12);
     }
+    UCMC_HIP(hipMemcpyAsync(phi, th.p, 32, hipMemcpyDeviceToHost, s.stream));
+    UCMC_HIP(hipStreamSynchronize(s.stream));
+    theta[0] = phi[0];
+    for (int k = 1; k < 4; ++k) {
+        theta[k] = phi[k] / hsd[k];
+        theta[0] -= theta[k] * hmu[k];
+    }
+    UCMC_HIP(hipMemcpyAsync(th.p, theta, 32, hipMemcpyHostToDevice, s.stream));
     UCMC_HIP(hipMemsetAsync(o.p, 0, 40, s.stream));
     hipLaunchKernelGGL(slmc_loss_grad, grid, dim3(256), 0, s.stream, s.flip_out.as<float>(), n, th.as<double>(),
-                       o.as<double>());
+                       nullptr, nullptr, o.as<double>());
     UCMC_HIP(hipGetLastError());
     double h[5];
-    UCMC_HIP(hipMemcpyAsync(theta, th.p, 32, hipMemcpyDeviceToHost, s.stream));
     UCMC_HIP(hipMemcpyAsync(h, o.p, 40, hipMemcpyDeviceToHost, s.stream));
     UCMC_HIP(hipStreamSynchronize(s.stream));
     if (loss) *loss = h[0] / n;
 }
 
 void slmc_run(System& s, const double* Jeff, double beta, std::uint64_t run_seed, std::uint64_t flip0,
-              std::uint64_t nflips, ucmc_observables* series, std::uint8_t* accepted)
+              std::uint64_t nflips, ucmc_observables* series, std::uint8_t* accepted, double* move_stats)
 {
     if (!(beta == beta)) fail(UCMC_E_INVALID_ARGUMENT, "beta is NaN");
     if (nflips == 0) return;
@@ -661,7 +954,7 @@ void slmc_run(System& s, const double* Jeff, double beta, std::uint64_t run_seed
     ucmc_observables start{};
     if (series) fpi_observe(s, start);
     DevBuf out;
-    out.alloc(nflips * 6 * 8);
+    out.alloc(nflips * kOutStride * 8);
     SlmcArgs a{};
     a.s = s.field[s.cur].as<std::int8_t>();
     a.L = static_cast<std::uint32_t>(s.L);
@@ -681,25 +974,61 @@ void slmc_run(System& s, const double* Jeff, double beta, std::uint64_t run_seed
         a.thr[k][1] = wolff_threshold(beta, 2.0 * Jeff[k]);
         a.thr[k][0] = wolff_threshold(beta, -2.0 * Jeff[k]);
     }
-    hipLaunchKernelGGL(slmc_flips, dim3(1), dim3(1024), 0, s.stream, a);
+    if (s.sites >= kSlmcLabelMinSites) {
+        // label propagation: O(N) grid-wide work per step instead of one
+        // workgroup's breadth-first levels
+        if (!s.labels.p) s.labels.alloc(s.sites * 4);
+        if (!s.cluster_state.p) s.cluster_state.alloc(sizeof(ClusterState));
+        SlmcLabelArgs la{};
+        la.s = a.s;
+        la.L = a.L;
+        la.n = static_cast<std::uint32_t>(s.sites);
+        la.parent = s.labels.as<std::uint32_t>();
+        la.st = static_cast<ClusterState*>(s.cluster_state.p);
+        la.beta = a.beta;
+        la.J1 = a.J1;
+        la.K = a.K;
+        for (int k = 0; k < 3; ++k) {
+            la.Jeff[k] = a.Jeff[k];
+            la.thr[k][0] = a.thr[k][0];
+            la.thr[k][1] = a.thr[k][1];
+        }
+        const dim3 grid((la.n + 255) / 256);
+        for (std::uint64_t f = 0; f < nflips; ++f) {
+            la.key = key_of(run_seed, kSlmcCounterBase + flip0 + f);
+            la.out = a.out + kOutStride * f;
+            hipLaunchKernelGGL(slmc_label_begin, grid, dim3(256), 0, s.stream, la);
+            hipLaunchKernelGGL(slmc_label_link, grid, dim3(256), 0, s.stream, la);
+            hipLaunchKernelGGL(lab_flatten, grid, dim3(256), 0, s.stream, la.parent, la.n,
+                               static_cast<const ClusterState*>(la.st));
+            hipLaunchKernelGGL(slmc_label_sums, grid, dim3(256), 0, s.stream, la);
+            hipLaunchKernelGGL(slmc_label_decide, dim3(1), dim3(64), 0, s.stream, la);
+            hipLaunchKernelGGL(slmc_label_apply, grid, dim3(256), 0, s.stream, la);
+        }
+    } else {
+        hipLaunchKernelGGL(slmc_flips, dim3(1), dim3(1024), 0, s.stream, a);
+    }
     UCMC_HIP(hipGetLastError());
     s.wolff_mark += static_cast<std::uint32_t>(nflips);
-    std::vector<double> h(nflips * 6);
+    std::vector<double> h(nflips * kOutStride);
     UCMC_HIP(hipMemcpyAsync(h.data(), out.p, h.size() * 8, hipMemcpyDeviceToHost, s.stream));
     UCMC_HIP(hipStreamSynchronize(s.stream));
     double e = start.energy, m = start.order[0];
     for (std::uint64_t f = 0; f < nflips; ++f) {
-        const bool acc = h[6 * f + 1] != 0.0;
+        const double* r = &h[kOutStride * f];
+        const bool acc = r[1] != 0.0;
         if (acc) {
-            e += h[6 * f + 2];
-            m += h[6 * f + 4];
+            e += r[2];
+            m += r[4];
         }
         if (accepted) accepted[f] = acc ? 1 : 0;
+        if (move_stats)
+            for (int k = 0; k < 4; ++k) move_stats[4 * f + k] = r[5 + k];
         if (series) {
             ucmc_observables o{};
             o.energy = e;
             o.order[0] = m;
-            o.cluster = static_cast<std::uint64_t>(h[6 * f]);
+            o.cluster = static_cast<std::uint64_t>(r[0]);
             o.accepted = acc ? 1 : 0;
             series[f] = o;
         }

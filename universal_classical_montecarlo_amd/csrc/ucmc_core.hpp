@@ -87,6 +87,9 @@ struct System {
     // Wolff: per-site membership stamps (site is in flip f's cluster iff
     // stamp == mark_f, so no per-flip clear), the BFS queue, per-flip outputs
     DevBuf stamp, queue, flip_out;
+    // label-propagation clusters: parent array and the per-flip control block
+    DevBuf labels, cluster_state;
+    DevBuf local_gram;   // SLMC: FP64 local-field Gram matrix of the training set
     std::uint32_t wolff_mark = 0;
     // parallel tempering batch (XY replicas)
     struct Pt {
@@ -145,10 +148,11 @@ void slmc_collect(System& s, double beta, std::uint64_t burn, std::uint64_t ncon
                   std::uint64_t every, std::uint64_t run_seed, double* corr_out);
 void slmc_gram(System& s, float* G_out, double* theta_out);
 void slmc_loss_grad_host(System& s, const double* theta, double* out5);
+void slmc_fit_local(System& s, double* gram_out, double* theta_out);
 void slmc_adam_fit(System& s, int iters, double lr, double* theta, double* loss);
 void slmc_run(System& s, const double* Jeff, double beta, std::uint64_t run_seed,
               std::uint64_t flip0, std::uint64_t nflips, ucmc_observables* series,
-              std::uint8_t* accepted);
+              std::uint8_t* accepted, double* move_stats);
 void square_csr(System& s);  // materialise square_lattice(Lx, Ly) for a dims-created system
 void wolff_run(System& s, const ucmc_wolff_params& p, std::uint64_t* sizes,
                ucmc_observables* series);

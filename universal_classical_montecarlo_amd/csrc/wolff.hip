@@ -24,6 +24,7 @@
 #include <cstring>
 #include <vector>
 
+#include "labels.hpp"
 #include "ucmc_core.hpp"
 #include "ucmc_device.hpp"
 
@@ -33,6 +34,7 @@ namespace {
 
 constexpr double kPi = 3.14159265358979323846;
 constexpr int kThreads = 1024;
+constexpr std::size_t kLabelMinSites = 32768;   // lattices from this size on use label propagation
 
 struct WolffArgs {
     void* field;                   // int8 [n] (Ising) or fp32 SoA [N][n]
@@ -261,6 +263,177 @@ __global__ __launch_bounds__(kThreads) void wolff_flips(WolffArgs a)
     }
 }
 
+// ------------------------------------------------- label propagation path --
+// One flip as five grid-wide kernels: (1) parent[i] = i, axis and seed site;
+// (2) every undirected edge, from its smaller endpoint, is activated with the
+// same test as the breadth-first growth and its endpoints united; (3) labels
+// flattened; (4) size, energy and order change of the seed's component on the
+// pre-flip field; (5) the component reflected.  Cost O(N) per flip whatever
+// the cluster size, against O(cluster) levels of latency for the growth.
+struct LabelArgs {
+    void* field;
+    std::uint32_t n;
+    const std::uint32_t* offsets;
+    const std::uint32_t* bonds;
+    const std::uint32_t* edge_id;
+    std::uint32_t* parent;
+    ClusterState* st;
+    double* out;                   // this flip's [size, dE, dM0..dM3]
+    std::uint64_t key;
+    std::uint64_t touch_target;
+    double beta;
+    double J;
+    std::uint64_t thr_aligned;
+    std::uint64_t thr_anti;
+};
+
+template <int N>
+__global__ __launch_bounds__(256) void label_begin(LabelArgs a)
+{
+    const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < a.n) a.parent[i] = i;
+    if (i == 0) {
+        ClusterState* st = a.st;
+        st->skip = (a.touch_target && st->touched >= a.touch_target) ? 1u : 0u;
+        // driver.hpp:154-159: the axis first, then the seed site
+        Stream ctl;
+        ctl.init(a.key, 0, kTagWolffCtrl);
+        double r[4] = {1.0, 0.0, 0.0, 0.0};
+        if constexpr (N > 1) sphere_d<N>(ctl, r);
+        for (int k = 0; k < 4; ++k) st->r[k] = r[k];
+        st->seed = static_cast<std::uint32_t>(dmul(ctl.uni(), static_cast<double>(a.n)));
+    }
+}
+
+template <int N>
+__global__ __launch_bounds__(256) void label_link(LabelArgs a)
+{
+    const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n || a.st->skip) return;
+    double r[4];
+    for (int k = 0; k < 4; ++k) r[k] = a.st->r[k];
+    Field<N> fld;
+    if constexpr (N == 1) fld.f = static_cast<const std::int8_t*>(a.field);
+    else {
+        fld.f = static_cast<const float*>(a.field);
+        fld.n = a.n;
+    }
+    const double ai = fld.proj(i, r);
+    for (std::uint32_t q = a.offsets[i]; q < a.offsets[i + 1]; ++q) {
+        const std::uint32_t j = a.bonds[q];
+        if (j <= i) continue;   // each undirected edge from its smaller endpoint
+        const std::uint32_t u = edge_u32(a.key, a.edge_id[q]);
+        bool act;
+        if constexpr (N == 1) {
+            const bool aligned = ai * fld.proj(j, r) > 0.0;
+            act = static_cast<std::uint64_t>(u) < (aligned ? a.thr_aligned : a.thr_anti);
+        } else {
+            const double aj = fld.proj(j, r);
+            const double dE = dmul(2.0 * a.J, dmul(ai, aj));
+            const double p = 1.0 - exp(-a.beta * dE);
+            act = static_cast<double>(u) * 0x1.0p-32 < p;
+        }
+        if (act) lab_unite(a.parent, i, j);
+    }
+}
+
+template <int N>
+__global__ __launch_bounds__(256) void label_delta(LabelArgs a)
+{
+    const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a.st->skip) return;
+    double r[4];
+    for (int k = 0; k < 4; ++k) r[k] = a.st->r[k];
+    const std::uint32_t root = a.parent[a.st->seed];
+    Field<N> fld;
+    if constexpr (N == 1) fld.f = static_cast<const std::int8_t*>(a.field);
+    else {
+        fld.f = static_cast<const float*>(a.field);
+        fld.n = a.n;
+    }
+    unsigned long long cnt = 0;
+    double de = 0.0, dm[4] = {0.0, 0.0, 0.0, 0.0};
+    if (i < a.n && a.parent[i] == root) {
+        cnt = 1;
+        if constexpr (N == 1) {
+            const int si = fld.f[i];
+            int sum = 0;
+            for (std::uint32_t q = a.offsets[i]; q < a.offsets[i + 1]; ++q) {
+                const std::uint32_t j = a.bonds[q];
+                if (a.parent[j] != root) sum += fld.f[j];
+            }
+            de = 2.0 * a.J * static_cast<double>(si * sum);
+            dm[0] = -2.0 * si;
+        } else {
+            const double ai = fld.proj(i, r);
+            double h[N];
+            for (int k = 0; k < N; ++k) h[k] = 0.0;
+            for (std::uint32_t q = a.offsets[i]; q < a.offsets[i + 1]; ++q) {
+                const std::uint32_t j = a.bonds[q];
+                if (a.parent[j] != root)
+                    for (int k = 0; k < N; ++k) h[k] += fld.f[k * a.n + j];
+            }
+            for (int k = 0; k < N; ++k) {
+                const double s = fld.f[k * a.n + i];
+                const float sn = static_cast<float>(dadd(s, -dmul(dmul(2.0, ai), r[k])));
+                de += -a.J * (static_cast<double>(sn) - s) * h[k];
+                dm[k] = static_cast<double>(sn) - s;
+            }
+        }
+    }
+    cnt = wave_sum_u64(cnt);
+    if (__ballot(cnt != 0) == 0) return;   // wave-uniform: no cluster site in this wave
+    de = wave_sum(de);
+    for (int k = 0; k < N; ++k) dm[k] = wave_sum(dm[k]);
+    if ((threadIdx.x & 63u) == 0) {
+        atomicAdd(&a.st->touched, cnt);
+        atomicAdd(&a.out[0], static_cast<double>(cnt));
+        atomicAdd(&a.out[1], de);
+        for (int k = 0; k < N; ++k) atomicAdd(&a.out[2 + k], dm[k]);
+    }
+}
+
+template <int N>
+__global__ __launch_bounds__(256) void label_apply(LabelArgs a)
+{
+    const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n || a.st->skip) return;
+    const std::uint32_t root = a.parent[a.st->seed];
+    if (a.parent[i] != root) return;
+    if constexpr (N == 1) {
+        std::int8_t* f = static_cast<std::int8_t*>(a.field);
+        f[i] = static_cast<std::int8_t>(-f[i]);
+    } else {
+        float* f = static_cast<float*>(a.field);
+        double r[4];
+        for (int k = 0; k < 4; ++k) r[k] = a.st->r[k];
+        double ai = 0.0;
+        for (int k = 0; k < N; ++k) ai = dadd(ai, dmul(static_cast<double>(f[k * a.n + i]), r[k]));
+        for (int k = 0; k < N; ++k) {
+            const double s = f[k * a.n + i];
+            f[k * a.n + i] = static_cast<float>(dadd(s, -dmul(dmul(2.0, ai), r[k])));
+        }
+    }
+}
+
+template <int N>
+void label_flips(System& s, LabelArgs a, const ucmc_wolff_params& p)
+{
+    const dim3 grid((a.n + 255) / 256);
+    for (std::uint64_t f = 0; f < p.nflips; ++f) {
+        a.key = key_of(p.run_seed, kWolffCounterBase + p.flip0 + f);
+        LabelArgs af = a;
+        af.out = a.out + 6 * f;
+        hipLaunchKernelGGL((label_begin<N>), grid, dim3(256), 0, s.stream, af);
+        hipLaunchKernelGGL((label_link<N>), grid, dim3(256), 0, s.stream, af);
+        hipLaunchKernelGGL(lab_flatten, grid, dim3(256), 0, s.stream, af.parent, af.n,
+                           static_cast<const ClusterState*>(af.st));
+        hipLaunchKernelGGL((label_delta<N>), grid, dim3(256), 0, s.stream, af);
+        hipLaunchKernelGGL((label_apply<N>), grid, dim3(256), 0, s.stream, af);
+    }
+    UCMC_HIP(hipGetLastError());
+}
+
 }  // namespace
 
 void wolff_run(System& s, const ucmc_wolff_params& p, std::uint64_t* sizes, ucmc_observables* series)
@@ -289,6 +462,40 @@ void wolff_run(System& s, const ucmc_wolff_params& p, std::uint64_t* sizes, ucmc
     DevBuf out;
     out.alloc(p.nflips * 6 * sizeof(double));
     UCMC_HIP(hipMemsetAsync(out.p, 0, out.bytes, s.stream));
+    // Label propagation for large lattices (or on request), breadth-first
+    // growth in one workgroup for small ones; both build the same cluster.
+    const bool labels = (p.flags & UCMC_WOLFF_LABELS) ||
+                        (!(p.flags & UCMC_WOLFF_BFS) && s.sites >= kLabelMinSites);
+    if (labels) {
+        if (!s.labels.p) s.labels.alloc(s.sites * 4);
+        if (!s.cluster_state.p) s.cluster_state.alloc(sizeof(ClusterState));
+        UCMC_HIP(hipMemsetAsync(s.cluster_state.p, 0, sizeof(ClusterState), s.stream));
+        LabelArgs la{};
+        la.field = s.field[s.cur].p;
+        la.n = static_cast<std::uint32_t>(s.sites);
+        la.offsets = s.offsets.as<std::uint32_t>();
+        la.bonds = s.bonds.as<std::uint32_t>();
+        la.edge_id = s.edge_id.as<std::uint32_t>();
+        la.parent = s.labels.as<std::uint32_t>();
+        la.st = static_cast<ClusterState*>(s.cluster_state.p);
+        la.out = out.as<double>();
+        la.touch_target = p.touch_target;
+        la.beta = p.beta;
+        if (s.kind == UCMC_ISING) {
+            la.J = static_cast<double>(s.Ji);
+            la.thr_aligned = wolff_threshold(p.beta, 2.0 * s.Ji);
+            la.thr_anti = wolff_threshold(p.beta, -2.0 * s.Ji);
+            label_flips<1>(s, la, p);
+        } else {
+            la.J = s.J;
+            switch (s.N) {
+            case 2: label_flips<2>(s, la, p); break;
+            case 3: label_flips<3>(s, la, p); break;
+            case 4: label_flips<4>(s, la, p); break;
+            default: fail(UCMC_E_UNSUPPORTED, "N");
+            }
+        }
+    }
     WolffArgs a{};
     a.field = s.field[s.cur].p;
     a.n = static_cast<std::uint32_t>(s.sites);
@@ -309,8 +516,8 @@ void wolff_run(System& s, const ucmc_wolff_params& p, std::uint64_t* sizes, ucmc
         // driver.hpp:195 with dE = energy_diff_bond = 2 J s_i s_j
         a.thr_aligned = wolff_threshold(p.beta, 2.0 * s.Ji);
         a.thr_anti = wolff_threshold(p.beta, -2.0 * s.Ji);
-        hipLaunchKernelGGL(wolff_flips<1>, dim3(1), dim3(kThreads), 0, s.stream, a);
-    } else {
+        if (!labels) hipLaunchKernelGGL(wolff_flips<1>, dim3(1), dim3(kThreads), 0, s.stream, a);
+    } else if (!labels) {
         a.J = s.J;
         switch (s.N) {
         case 2: hipLaunchKernelGGL(wolff_flips<2>, dim3(1), dim3(kThreads), 0, s.stream, a); break;

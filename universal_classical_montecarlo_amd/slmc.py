@@ -17,16 +17,48 @@ from . import abi, stats
 
 
 def fit_effective_model(sysm, beta, burn=2000, nconfigs=20000, every=1, run_seed=1,
-                        method="normal", adam_iters=2000, adam_lr=0.01):
-    """Returns (theta, J_eff[3], rms residual per site) of E_eff/N = theta0 -
-    sum_k J_k C_k / N fitted on `nconfigs` configurations."""
+                        method="local", adam_iters=2000, adam_lr=0.01):
+    """Returns (theta, J_eff[3], rms residual, correlations) of the effective
+    model E_eff = E0 - sum_k J_k C_k fitted on `nconfigs` configurations.
+
+    method "local": least squares of the single-flip energy changes over every
+    site (the differences a cluster flip's acceptance depends on; residual per
+    flip); "normal" / "adam": least squares of E/N over configurations
+    (residual per site)."""
     corr = sysm.slmc_collect(beta, burn, nconfigs, every, run_seed)
+    if method == "local":
+        theta, G = sysm.slmc_fit_local()
+        J = theta[1:4]
+        rows = nconfigs * sysm.info.sites
+        sse = G[15, 15] - 2.0 * J @ G[:3, 15] + J @ G[:3, :3] @ J
+        return theta, J.copy(), float(np.sqrt(max(sse, 0.0) / rows)), corr
     if method == "normal":
         theta, _ = sysm.slmc_fit_normal()
     else:
         theta, _ = sysm.slmc_fit_adam(adam_iters, adam_lr, np.zeros(4))
     loss, _ = sysm.slmc_loss_grad(theta)
     return theta, theta[1:4].copy(), float(np.sqrt(loss / nconfigs)), corr
+
+
+def refine_from_moves(sysm, J_eff, beta, J1, K, nsteps=2000, iters=2, run_seed=101):
+    """Self-learning refinement on the sampler's own proposals: the acceptance
+    of a cluster move is exp(-beta (dE_true - dE_eff)) with dE_true = 2 J1 c_1 +
+    2 K P and dE_eff = 2 sum_k J_k c_k (boundary sums from ucmc_slmc_run), so
+    the couplings that maximise it in the mean-square sense regress
+    J1 c_1 + K P on (c_1, c_2, c_3) over proposed moves.  Repeated `iters`
+    times, each on fresh proposals of the updated couplings; the chain's state
+    is restored afterwards."""
+    J = np.asarray(J_eff, float).copy()
+    saved = sysm.download()
+    history = []
+    for it in range(iters):
+        _, acc, st = sysm.slmc_run(J, beta, nsteps, run_seed + it, record=False, stats=True)
+        c, P = st[:, :3], st[:, 3]
+        y = J1 * c[:, 0] + K * P
+        J, *_ = np.linalg.lstsq(c, y, rcond=None)
+        history.append({"acceptance_before": float(acc.mean()), "J_eff": J.tolist()})
+    sysm.upload(saved)
+    return J, history
 
 
 def plain_series(sysm, beta, nsweeps, run_seed, sweep0=0, chunk=5000):
@@ -61,14 +93,28 @@ def slmc_series(sysm, J_eff, beta, nsteps, run_seed, flip0=0, chunk=2000):
 
 
 def compare(L=256, J1=1.0, K=0.2, beta=0.40, nconfigs=20000, plain_sweeps=200000,
-            slmc_steps=20000, run_seed=7, baseline_J=None):
+            slmc_steps=20000, run_seed=7, baseline_J=None, every=10, burn=5000, method="local",
+            refine_steps=0, refine_iters=2, train_beta=None):
     """Independent samples / s of plain GPU Metropolis and of SLMC on the same
-    model (and optionally of the untrained nearest-neighbour proposal)."""
+    model (and optionally of the untrained nearest-neighbour proposal).  The
+    training set keeps one configuration every `every` sweeps of the plain
+    chain, after `burn` sweeps."""
     out = {"L": L, "J1": J1, "K": K, "beta": beta}
     sysm = abi.System.plaquette_ising(L, J1, K)
-    theta, J_eff, rms, _ = fit_effective_model(sysm, beta, nconfigs=nconfigs, run_seed=run_seed)
+    # training configurations from plain Metropolis at train_beta (default: beta)
+    tb = beta if train_beta is None else train_beta
+    theta, J_eff, rms, _ = fit_effective_model(sysm, tb, burn=burn, nconfigs=nconfigs, every=every,
+                                               run_seed=run_seed, method=method)
+    out["train_beta"] = tb
+    if tb != beta:   # bring the chain to the sampling temperature
+        sysm.metropolis(beta, burn, run_seed + 5)
     out["J_eff"] = J_eff.tolist()
-    out["fit_rms_per_site"] = rms
+    out["fit"] = method
+    out["fit_rms"] = rms
+    if refine_steps:
+        J_eff, hist = refine_from_moves(sysm, J_eff, beta, J1, K, refine_steps, refine_iters)
+        out["refined"] = hist
+        out["J_eff"] = J_eff.tolist()
     # plain Metropolis continues from the training chain's state
     e, m, secs = plain_series(sysm, beta, plain_sweeps, run_seed + 1)
     out["plain"] = _rates(e, m, secs, "sweeps")
