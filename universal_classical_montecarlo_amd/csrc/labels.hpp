@@ -76,11 +76,15 @@ __device__ __forceinline__ void lab_unite(std::uint32_t* parent, std::uint32_t a
 namespace {
 
 // parent[i] = root of i for every site (one thread per site, after all unites).
+// The walk to the root only reads: a halving store here could replace a root
+// pointer another thread has just written with a non-root ancestor.
 __global__ void lab_flatten(std::uint32_t* parent, std::uint32_t n, const ClusterState* st)
 {
     const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n || st->skip) return;
-    lab_store(parent + i, lab_find(parent, i));
+    std::uint32_t x = i;
+    for (std::uint32_t p = lab_load(parent + x); p != x; p = lab_load(parent + x)) x = p;
+    lab_store(parent + i, x);
 }
 
 }  // namespace
