@@ -111,9 +111,10 @@ def suite(quick=False):
     rate = L ** 3 * 50 / (ms * 1e-3)
     out["heisenberg_128_metropolis"] = {"site_updates_per_s": rate,
                                         "hbm_frac_24B": 24 * rate / 1e9 / HBM_PEAK_GBS}
-    # Wolff reflection clusters (label propagation at this size), after 500
-    # untimed flips from the Metropolis-equilibrated state
-    v.metropolis(0.693, 200, 4, proposal=abi.PROP_ROTATION, width=0.5)
+    # Wolff reflection clusters (label propagation at this size): 2000 Metropolis
+    # sweeps bring the random start close to equilibrium at beta = 0.693 (~T_c),
+    # then 500 untimed flips
+    v.metropolis(0.693, 2000, 4, proposal=abi.PROP_ROTATION, width=0.5)
     v.wolff(0.693, 500, 3)
     t0 = time.perf_counter()
     sizes, _ = v.wolff(0.693, 500, 3, flip0=500)

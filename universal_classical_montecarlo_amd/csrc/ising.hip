@@ -216,7 +216,7 @@ __device__ __forceinline__ WordDecision decide_fast(std::uint32_t C, std::uint32
 }
 
 template <int H, bool COUNT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void ising_fused(FusedArgs a)
+__global__ __launch_bounds__(256) void ising_fused(FusedArgs a)
 {
     static_assert(H % 2 == 0 && H + 2 <= 64, "block height: even, halo rows fit one wave");
     __shared__ FusedTables tb;

@@ -567,14 +567,17 @@ __global__ __launch_bounds__(256) void slmc_label_link(SlmcLabelArgs a)
     for (int d = 0; d < 12; ++d) {
         if (!kForward[d]) continue;   // each undirected edge from its owner
         const std::uint32_t j = nbr(a.L, i, d);
-        const std::uint32_t u = edge_u32(a.key, 6u * i + kSlot[d]);
         const bool aligned = si * a.s[j] > 0;
-        if (static_cast<std::uint64_t>(u) < a.thr[kShell[d]][aligned ? 1 : 0]) lab_unite(a.parent, i, j);
+        const std::uint64_t thr = a.thr[kShell[d]][aligned ? 1 : 0];
+        if (thr == 0) continue;   // never activated: no draw needed
+        const std::uint32_t u = edge_u32(a.key, 6u * i + kSlot[d]);
+        if (static_cast<std::uint64_t>(u) < thr) lab_unite(a.parent, i, j);
     }
 }
 
 __global__ __launch_bounds__(256) void slmc_label_sums(SlmcLabelArgs a)
 {
+    __shared__ long long part[4][6];
     const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const std::uint32_t root = a.parent[a.st->seed];
     long long cnt[6] = {0, 0, 0, 0, 0, 0};   // c0..c2, plaquettes, sum s_i, size
@@ -582,13 +585,17 @@ __global__ __launch_bounds__(256) void slmc_label_sums(SlmcLabelArgs a)
         bond_sums(a.s, a.L, i, [&](std::uint32_t j) { return a.parent[j] == root; }, cnt);
         cnt[5] = 1;
     }
-    const unsigned long long any = wave_sum_u64(static_cast<unsigned long long>(cnt[5]));
-    if (any == 0) return;   // wave-uniform
-    for (int k = 0; k < 5; ++k)
-        cnt[k] = static_cast<long long>(wave_sum_u64(static_cast<unsigned long long>(cnt[k])));
-    if ((threadIdx.x & 63u) == 0) {
-        for (int k = 0; k < 5; ++k) atomicAdd(&a.st->count[k], static_cast<unsigned long long>(cnt[k]));
-        atomicAdd(&a.st->count[5], any);
+    // wave sums, then one atomic per counter per block (the counters are few,
+    // so per-wave atomics would queue on them)
+    const std::uint32_t w = threadIdx.x >> 6;
+    for (int k = 0; k < 6; ++k) {
+        const long long v = static_cast<long long>(wave_sum_u64(static_cast<unsigned long long>(cnt[k])));
+        if ((threadIdx.x & 63u) == 0) part[w][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const long long v = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] + part[3][threadIdx.x];
+        if (v) atomicAdd(&a.st->count[threadIdx.x], static_cast<unsigned long long>(v));
     }
 }
 

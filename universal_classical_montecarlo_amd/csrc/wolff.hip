@@ -322,16 +322,18 @@ __global__ __launch_bounds__(256) void label_link(LabelArgs a)
     for (std::uint32_t q = a.offsets[i]; q < a.offsets[i + 1]; ++q) {
         const std::uint32_t j = a.bonds[q];
         if (j <= i) continue;   // each undirected edge from its smaller endpoint
-        const std::uint32_t u = edge_u32(a.key, a.edge_id[q]);
         bool act;
         if constexpr (N == 1) {
             const bool aligned = ai * fld.proj(j, r) > 0.0;
-            act = static_cast<std::uint64_t>(u) < (aligned ? a.thr_aligned : a.thr_anti);
+            const std::uint64_t thr = aligned ? a.thr_aligned : a.thr_anti;
+            if (thr == 0) continue;   // never activated: no draw needed
+            act = static_cast<std::uint64_t>(edge_u32(a.key, a.edge_id[q])) < thr;
         } else {
             const double aj = fld.proj(j, r);
             const double dE = dmul(2.0 * a.J, dmul(ai, aj));
+            if (!(a.beta * dE > 0.0)) continue;   // p <= 0 (or NaN): never activated
             const double p = 1.0 - exp(-a.beta * dE);
-            act = static_cast<double>(u) * 0x1.0p-32 < p;
+            act = static_cast<double>(edge_u32(a.key, a.edge_id[q])) * 0x1.0p-32 < p;
         }
         if (act) lab_unite(a.parent, i, j);
     }
@@ -381,15 +383,19 @@ __global__ __launch_bounds__(256) void label_delta(LabelArgs a)
             }
         }
     }
-    cnt = wave_sum_u64(cnt);
-    if (__ballot(cnt != 0) == 0) return;   // wave-uniform: no cluster site in this wave
-    de = wave_sum(de);
-    for (int k = 0; k < N; ++k) dm[k] = wave_sum(dm[k]);
-    if ((threadIdx.x & 63u) == 0) {
-        atomicAdd(&a.st->touched, cnt);
-        atomicAdd(&a.out[0], static_cast<double>(cnt));
-        atomicAdd(&a.out[1], de);
-        for (int k = 0; k < N; ++k) atomicAdd(&a.out[2 + k], dm[k]);
+    // wave sums, then one atomic per quantity per block
+    __shared__ double part[4][6];
+    const std::uint32_t w = threadIdx.x >> 6;
+    const double vals[6] = {static_cast<double>(cnt), de, dm[0], dm[1], dm[2], dm[3]};
+    for (int k = 0; k < 2 + N; ++k) {
+        const double v = wave_sum(vals[k]);
+        if ((threadIdx.x & 63u) == 0) part[w][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < static_cast<unsigned>(2 + N)) {
+        const double v = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] + part[3][threadIdx.x];
+        if (threadIdx.x == 0 && v != 0.0) atomicAdd(&a.st->touched, static_cast<unsigned long long>(v));
+        if (v != 0.0) atomicAdd(&a.out[threadIdx.x], v);
     }
 }
 
