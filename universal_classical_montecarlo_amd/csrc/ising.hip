@@ -36,6 +36,10 @@
 #ifndef UCMC_PHILOX_AHEAD
 #define UCMC_PHILOX_AHEAD 2
 #endif
+// Rows of Philox words computed while the block's loads are in flight.
+#ifndef UCMC_PHILOX_EARLY
+#define UCMC_PHILOX_EARLY 0
+#endif
 
 namespace ucmc {
 
@@ -290,6 +294,21 @@ __global__ __launch_bounds__(256) void ising_fused(FusedArgs a)
 #pragma unroll
     for (int j = 0; j < H + 4; ++j) raw[j] = load_raw(a.in, row_words, rowY(j), col);
 
+    // high-half Philox words of window row j: (x, y) even-x sites, (z, w) odd-x
+    U4 P[H + 4];
+    auto philox_row = [&](int j) {
+        P[j] = philox(U4{lane, rowY(j) * a.bands + band, kTagIsingHi, 0u}, a.key);
+        // an empty volatile asm that takes the words: keeps the rounds here
+        // rather than sunk next to their first use
+        asm volatile("" : "+v"(P[j].x), "+v"(P[j].y), "+v"(P[j].z), "+v"(P[j].w));
+    };
+    constexpr int kAhead = UCMC_PHILOX_AHEAD;
+    constexpr int kEarly = UCMC_PHILOX_EARLY < H + 2 ? UCMC_PHILOX_EARLY : H + 2;
+    // rows 1 .. kFirst have their words before the sweep loop starts
+    constexpr int kFirst = kEarly > 3 + kAhead ? kEarly : (3 + kAhead < H + 2 ? 3 + kAhead : H + 2);
+#pragma unroll
+    for (int j = 1; j <= kEarly; ++j) philox_row(j);
+
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int n = 0; n < 5; ++n)
@@ -336,14 +355,6 @@ __global__ __launch_bounds__(256) void ising_fused(FusedArgs a)
     if (!active) return;
 
     std::uint32_t acc = 0;
-    // high-half Philox words of window row j: (x, y) even-x sites, (z, w) odd-x
-    U4 P[H + 4];
-    auto philox_row = [&](int j) {
-        P[j] = philox(U4{lane, rowY(j) * a.bands + band, kTagIsingHi, 0u}, a.key);
-        // an empty volatile asm that takes the words: keeps the rounds here
-        // rather than sunk next to their first use
-        asm volatile("" : "+v"(P[j].x), "+v"(P[j].y), "+v"(P[j].z), "+v"(P[j].w));
-    };
     // neighbour counts of the even-x half of row j (odd half of the row, with the
     // left halo for lane 0, even halves of rows j - 1, j + 1) and of the odd-x half
     auto sumE = [&](const Row (&R)[H + 4], int j) {
@@ -389,10 +400,9 @@ __global__ __launch_bounds__(256) void ising_fused(FusedArgs a)
     // step is one scheduling region.  The Philox words of a row are computed
     // kAhead steps before their first use, so their ten dependent rounds
     // overlap the decisions in between.
-    constexpr int kAhead = UCMC_PHILOX_AHEAD;
     Row R[H + 4];
 #pragma unroll
-    for (int j = 1; j <= 3 + kAhead && j <= H + 2; ++j) philox_row(j);
+    for (int j = kEarly + 1; j <= kFirst; ++j) philox_row(j);
 #pragma unroll
     for (int j = 0; j < 5; ++j) R[j] = convert_row(raw[j]);
 #pragma unroll
@@ -403,7 +413,7 @@ __global__ __launch_bounds__(256) void ising_fused(FusedArgs a)
     }
 #pragma unroll
     for (int j = 2; j <= H + 1; ++j) {
-        if (j + 2 + kAhead <= H + 2) philox_row(j + 2 + kAhead);
+        if (j + 2 + kAhead > kFirst && j + 2 + kAhead <= H + 2) philox_row(j + 2 + kAhead);
         if (j + 2 <= H + 2) {
             if (j + 3 <= H + 3 && j + 3 >= 5) R[j + 3] = convert_row(raw[j + 3]);
             WordDecision d0 = fast0(R, j + 2);
@@ -506,8 +516,6 @@ IsingTables make_tables(const System& s, double beta)
         }
     return t;
 }
-
-void observe_into(System& s, unsigned long long* dev_out, ucmc_observables& o);
 
 void launch_sweep(System& s, const IsingTables& t, std::uint64_t key, bool simple,
                   unsigned long long* dacc, float* trace_dE, std::uint8_t* trace_acc)

@@ -33,6 +33,7 @@ struct VecArgs {
     std::uint32_t count;
     std::uint32_t L;                // cubic: edge length
     std::uint32_t colour;           // cubic: parity of the class
+    std::uint32_t log2L;            // cubic: log2(L) when L is a power of two, else 0
     std::uint64_t key;
     float J;
     double beta;
@@ -67,48 +68,97 @@ __device__ __forceinline__ void sphere(Stream& r, float* out)
     }
 }
 
-// dof/vector.hpp:79-118 in fp32: uniform oriented 2-frame by projection and
-// rejection, rotation by delta (2u - 1) in that plane.
-template <int N>
-__device__ __forceinline__ void rotate(Stream& r, float delta, const float* cur, float* out)
-{
-    float u[N], v[N];
-    sphere<N>(r, u);
-    const float eps = static_cast<float>(64.0 * 2.220446049250313e-16);
-    for (;;) {
-        sphere<N>(r, v);
-        float proj = 0.f;
-#pragma unroll
-        for (int k = 0; k < N; ++k) proj += v[k] * u[k];
-        float r2 = 0.f;
-#pragma unroll
-        for (int k = 0; k < N; ++k) {
-            v[k] -= proj * u[k];
-            r2 += v[k] * v[k];
-        }
-        if (r2 > eps) {
-            const float inv = 1.0f / sqrtf(r2);
-#pragma unroll
-            for (int k = 0; k < N; ++k) v[k] *= inv;
-            // second Gram-Schmidt pass: when v was nearly parallel to u, the
-            // first pass leaves an fp32 component along u of ~1e-7 / |v_perp|;
-            // the double-precision frame of the reference has none
-            float p2 = 0.f;
-#pragma unroll
-            for (int k = 0; k < N; ++k) p2 += v[k] * u[k];
-            float q2 = 0.f;
-#pragma unroll
-            for (int k = 0; k < N; ++k) {
-                v[k] -= p2 * u[k];
-                q2 += v[k] * v[k];
-            }
-            const float inv2 = rsqrtf(q2);
-#pragma unroll
-            for (int k = 0; k < N; ++k) v[k] *= inv2;
-            break;
-        }
+// The first eight words of a site's stream (two Philox blocks) in registers,
+// indexed at compile time: the proposal's common path draws at most 2N words.
+// The second block is computed only by proposals that reach it.
+struct Words8 {
+    U4 b0, b1;
+    __device__ __forceinline__ void init(std::uint64_t key, std::uint32_t id)
+    {
+        b0 = philox(U4{id, 0u, kTagSite, 0u}, key);
     }
-    const float th = delta * (2.0f * uf(r.next()) - 1.0f);
+    __device__ __forceinline__ void more(std::uint64_t key, std::uint32_t id)
+    {
+        b1 = philox(U4{id, 0u, kTagSite, 1u}, key);
+    }
+    template <int K>
+    __device__ __forceinline__ std::uint32_t w() const
+    {
+        static_assert(K >= 0 && K < 8, "word index");
+        if constexpr (K == 0) return b0.x;
+        else if constexpr (K == 1) return b0.y;
+        else if constexpr (K == 2) return b0.z;
+        else if constexpr (K == 3) return b0.w;
+        else if constexpr (K == 4) return b1.x;
+        else if constexpr (K == 5) return b1.y;
+        else if constexpr (K == 6) return b1.z;
+        else return b1.w;
+    }
+};
+
+// sphere<N> (dof/vector.hpp:35-61) on words K0 .. K0 + N - 2 (same order and
+// arithmetic as sphere<N>(Stream&)).
+template <int N, int K0>
+__device__ __forceinline__ void sphere_words(const Words8& wd, float* out)
+{
+    if constexpr (N == 2) {
+        const float phi = static_cast<float>(2.0 * kPi) * uf(wd.template w<K0>());
+        float s, c;
+        sincosf(phi, &s, &c);
+        out[0] = c;
+        out[1] = s;
+    } else {
+        const float z = 2.0f * uf(wd.template w<K0>()) - 1.0f;
+        const float rad = sqrtf(fmaxf(1.0f - z * z, 0.0f));
+        sphere_words<N - 1, K0 + 1>(wd, out);
+#pragma unroll
+        for (int k = 0; k < N - 1; ++k) out[k] *= rad;
+        out[N - 1] = z;
+    }
+}
+
+// dof/vector.hpp:79-118 in fp32: v is projected off the unit u and normalised;
+// false when v was (numerically) parallel to u (the reference's eps test, a new
+// v is drawn).  A second Gram-Schmidt pass follows: when v was nearly parallel
+// to u, the first pass leaves an fp32 component along u of ~1e-7 / |v_perp|;
+// the double-precision frame of the reference has none.
+template <int N>
+__device__ __forceinline__ bool orthonormalise(const float* u, float* v)
+{
+    const float eps = static_cast<float>(64.0 * 2.220446049250313e-16);
+    float proj = 0.f;
+#pragma unroll
+    for (int k = 0; k < N; ++k) proj += v[k] * u[k];
+    float r2 = 0.f;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        v[k] -= proj * u[k];
+        r2 += v[k] * v[k];
+    }
+    if (!(r2 > eps)) return false;
+    const float inv = 1.0f / sqrtf(r2);
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] *= inv;
+    float p2 = 0.f;
+#pragma unroll
+    for (int k = 0; k < N; ++k) p2 += v[k] * u[k];
+    float q2 = 0.f;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        v[k] -= p2 * u[k];
+        q2 += v[k] * v[k];
+    }
+    const float inv2 = rsqrtf(q2);
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] *= inv2;
+    return true;
+}
+
+template <int N>
+__device__ __forceinline__ void rotate_by(float delta, std::uint32_t wth, const float* u, const float* v,
+                                          const float* cur, float* out)
+{
+    const float th = delta * (2.0f * uf(wth) - 1.0f);
     float sn, c;
     sincosf(th, &sn, &c);
     float cu = 0.f, cv = 0.f;
@@ -122,6 +172,38 @@ __device__ __forceinline__ void rotate(Stream& r, float delta, const float* cur,
         out[k] = cur[k] + (c - 1.0f) * (cu * u[k] + cv * v[k]) + sn * (cu * v[k] - cv * u[k]);
 }
 
+// The proposal of site i and the word its accept test draws (when needed):
+// UniformSphereN<N> (kind 0) or LocalRotation<N>{delta} (kind 1), in the
+// reference's draw order.  The common path reads the first 2N words from
+// registers; a rejected frame (probability ~1e-7) continues on the stream.
+template <int N>
+__device__ __forceinline__ std::uint32_t propose(int kind, float delta, const float* cur, std::uint64_t key,
+                                                 std::uint32_t i, float* p)
+{
+    Words8 wd;
+    wd.init(key, i);
+    static_assert(N <= 4, "the common path holds 2N <= 8 words");
+    if (kind == 0) {   // N words <= one block
+        sphere_words<N, 0>(wd, p);
+        return wd.template w<N - 1>();
+    }
+    if constexpr (N > 2) wd.more(key, i);
+    float u[N], v[N];
+    sphere_words<N, 0>(wd, u);
+    sphere_words<N, N - 1>(wd, v);
+    if (orthonormalise<N>(u, v)) {
+        rotate_by<N>(delta, wd.template w<2 * N - 2>(), u, v, cur, p);
+        return wd.template w<2 * N - 1>();
+    }
+    Stream r;   // the stream after the 2N - 2 words used so far
+    r.init(key, i, kTagSite);
+    for (int k = 0; k < 2 * N - 2; ++k) (void)r.next();
+    do sphere<N>(r, v);
+    while (!orthonormalise<N>(u, v));
+    rotate_by<N>(delta, r.next(), u, v, cur, p);
+    return r.next();
+}
+
 template <int N, bool CUBIC, bool TRACE>
 __global__ __launch_bounds__(256) void vector_colour(VecArgs a)
 {
@@ -132,7 +214,18 @@ __global__ __launch_bounds__(256) void vector_colour(VecArgs a)
         int deg = 0;
         if constexpr (CUBIC) {
             const std::uint32_t L = a.L, half = L >> 1;
-            const std::uint32_t xh = t % half, yz = t / half, y = yz % L, z = yz / L;
+            std::uint32_t xh, yz, y, z;
+            if (a.log2L) {   // power-of-two edge: shifts and masks
+                xh = t & (half - 1u);
+                yz = t >> (a.log2L - 1u);
+                y = yz & (L - 1u);
+                z = yz >> a.log2L;
+            } else {
+                xh = t % half;
+                yz = t / half;
+                y = yz % L;
+                z = yz / L;
+            }
             const std::uint32_t x = 2 * xh + ((a.colour + y + z) & 1u);
             i = (z * L + y) * L + x;
             nb[0] = (z * L + y) * L + (x + 1 == L ? 0 : x + 1);
@@ -164,17 +257,14 @@ __global__ __launch_bounds__(256) void vector_colour(VecArgs a)
             }
         }
         (void)deg;
-        Stream r;
-        r.init(a.key, i, kTagSite);
-        if (a.kind == 0) sphere<N>(r, p);
-        else rotate<N>(r, a.delta, s, p);
+        const std::uint32_t waccept = propose<N>(a.kind, a.delta, s, a.key, i, p);
         // example/vector_spin.hpp:50-60: -J sum_j (p.s_j - s.s_j) = -J (p - s).h
         float d = 0.f;
 #pragma unroll
         for (int k = 0; k < N; ++k) d += (p[k] - s[k]) * h[k];
         const float dE = -a.J * d;
         const double log_p = -a.beta * static_cast<double>(dE) + 0.0;
-        acc = log_p >= 0.0 || accept_draw(r.next(), log_p);
+        acc = log_p >= 0.0 || accept_draw(waccept, log_p);
         if (acc) {
             // keep |S| = 1 against fp32 drift
             float nn = 0.f;
@@ -262,6 +352,9 @@ void launch_colour(System& s, const ucmc_metropolis_params& p, std::uint64_t key
         a.L = static_cast<std::uint32_t>(s.L);
         a.colour = static_cast<std::uint32_t>(c);
         a.count = static_cast<std::uint32_t>(s.sites / 2);
+        a.log2L = 0;
+        if ((a.L & (a.L - 1u)) == 0u)
+            while ((1u << a.log2L) < a.L) ++a.log2L;
     } else {
         a.sites = s.colour_sites.as<std::uint32_t>() + s.colour_start[c];
         a.count = static_cast<std::uint32_t>(s.colour_start[c + 1] - s.colour_start[c]);
