@@ -45,6 +45,27 @@ def test_ising_wolff_bit_exact(gpu, L, beta, J, path):
     sysm.close()
 
 
+@pytest.mark.parametrize("L", [64, 128])
+def test_ising_wolff_reordered_csr(gpu, L):
+    """A square lattice whose CSR rows list the neighbours in another order
+    numbers its edges differently: the label path must then read the CSR ids
+    (the tiled path's coordinate ids would be wrong), still bit-exact."""
+    off, bonds = csr_square(L, L)
+    for i in range(L * L):
+        bonds[off[i]:off[i + 1]] = bonds[off[i]:off[i + 1]][::-1].copy()
+    sysm = abi.System.ising(off, bonds, 1)
+    spins = random_spins(L * L, 5, 3)
+    sysm.upload(spins)
+    sizes, _ = sysm.wolff(0.44, 100, 12, flags=abi.WOLFF_LABELS)
+    want = spins.copy()
+    wsz = np.zeros(100, np.uint64)
+    oracle().orc_ising_wolff_keyed(L * L, off, bonds, edge_ids(off, bonds), want, 1, 0.44, 12, 0, 100,
+                                   wsz, None)
+    assert np.array_equal(sizes, wsz)
+    assert np.array_equal(sysm.download(), want)
+    sysm.close()
+
+
 def test_ising_wolff_dims_created_4096(gpu):
     L = 4096
     off, bonds = csr_square(L, L)
@@ -64,7 +85,9 @@ def test_ising_wolff_dims_created_4096(gpu):
 
 @pytest.mark.parametrize("path", PATHS)
 @pytest.mark.parametrize("N,geom,beta", [(3, ("cubic", 8), 0.693), (2, ("square", 16), 1.0),
-                                         (3, ("square", 8), 0.3)])
+                                         (3, ("square", 8), 0.3),
+                                         # tiled label path (coordinates, not the CSR)
+                                         (3, ("cubic", 16), 0.693), (2, ("square", 64), 1.0)])
 def test_vector_wolff(gpu, N, geom, beta, path):
     kind, L = geom
     off, bonds = csr_cubic(L) if kind == "cubic" else csr_square(L, L)

@@ -126,7 +126,7 @@ void upload_colour_lists(System& s)
     UCMC_HIP(hipMemcpy(s.colour_sites.p, list.data(), list.size() * 4, hipMemcpyHostToDevice));
 }
 
-std::size_t build_edge_ids(System& s)
+std::size_t build_edge_ids(System& s, std::vector<std::uint32_t>* host_ids)
 {
     const std::size_t n = s.sites;
     const auto& off = s.h_offsets;
@@ -155,6 +155,7 @@ std::size_t build_edge_ids(System& s)
     s.edge_id.alloc(id.size() * 4);
     UCMC_HIP(hipMemcpy(s.edge_id.p, id.data(), id.size() * 4, hipMemcpyHostToDevice));
     s.n_edges = next;
+    if (host_ids) host_ids->swap(id);
     return next;
 }
 

@@ -2,9 +2,10 @@
 // whole-lattice way of building a Wolff cluster: every bond's activation is a
 // pure function of its keyed uniform and the pre-flip spins, so the cluster of
 // the seed is its component in the graph of activated bonds whatever order the
-// bonds are examined in.  A grid-wide lock-free union-find (hooks a root under
-// a smaller root with a CAS; finds halve paths) labels all components in
-// parallel; the seed's label then marks its cluster.
+// bonds are examined in.  A lock-free union-find (hooks a root under a smaller
+// root with a CAS; finds halve paths), first inside blocks of consecutive
+// sites in LDS and then across blocks on the grid-wide array, labels all
+// components in parallel; the seed's label then marks its cluster.
 //
 // Loads and stores of the parent array go through agent-scope atomics so that
 // every CU sees the links made by the others (plain loads may be served from a
@@ -30,36 +31,43 @@ struct ClusterState {
     std::uint32_t accept;         // SLMC: the Metropolis-Hastings decision
 };
 
-__device__ __forceinline__ std::uint32_t lab_load(const std::uint32_t* p)
+// Union-find primitives at a memory scope: agent scope for the grid-wide
+// parent array, workgroup scope for a block's array in LDS.
+template <int Scope>
+__device__ __forceinline__ std::uint32_t uf_load(const std::uint32_t* p)
 {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, Scope);
 }
-__device__ __forceinline__ void lab_store(std::uint32_t* p, std::uint32_t v)
+template <int Scope>
+__device__ __forceinline__ void uf_store(std::uint32_t* p, std::uint32_t v)
 {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, Scope);
 }
 
 // Root of x, halving the path on the way (every store points a non-root at an
 // ancestor, so concurrent finds and hooks keep each tree a tree).
-__device__ __forceinline__ std::uint32_t lab_find(std::uint32_t* parent, std::uint32_t x)
+template <int Scope>
+__device__ __forceinline__ std::uint32_t uf_find(std::uint32_t* parent, std::uint32_t x)
 {
     for (;;) {
-        const std::uint32_t p = lab_load(parent + x);
+        const std::uint32_t p = uf_load<Scope>(parent + x);
         if (p == x) return x;
-        const std::uint32_t g = lab_load(parent + p);
+        const std::uint32_t g = uf_load<Scope>(parent + p);
         if (g == p) return p;
-        lab_store(parent + x, g);
+        uf_store<Scope>(parent + x, g);
         x = g;
     }
 }
 
 // Merge the components of a and b: the larger root is hooked under the
-// smaller one, retried until the hook lands on a root.
-__device__ __forceinline__ void lab_unite(std::uint32_t* parent, std::uint32_t a, std::uint32_t b)
+// smaller one, retried until the hook lands on a root.  Every root is so the
+// smallest index of its component.
+template <int Scope>
+__device__ __forceinline__ void uf_unite(std::uint32_t* parent, std::uint32_t a, std::uint32_t b)
 {
     for (;;) {
-        a = lab_find(parent, a);
-        b = lab_find(parent, b);
+        a = uf_find<Scope>(parent, a);
+        b = uf_find<Scope>(parent, b);
         if (a == b) return;
         if (a < b) {
             const std::uint32_t t = a;
@@ -68,8 +76,128 @@ __device__ __forceinline__ void lab_unite(std::uint32_t* parent, std::uint32_t a
         }
         std::uint32_t expected = a;
         if (__hip_atomic_compare_exchange_strong(parent + a, &expected, b, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT))
+                                                 Scope))
             return;
+    }
+}
+
+__device__ __forceinline__ std::uint32_t lab_load(const std::uint32_t* p)
+{
+    return uf_load<__HIP_MEMORY_SCOPE_AGENT>(p);
+}
+__device__ __forceinline__ void lab_store(std::uint32_t* p, std::uint32_t v)
+{
+    uf_store<__HIP_MEMORY_SCOPE_AGENT>(p, v);
+}
+__device__ __forceinline__ std::uint32_t lab_find(std::uint32_t* parent, std::uint32_t x)
+{
+    return uf_find<__HIP_MEMORY_SCOPE_AGENT>(parent, x);
+}
+__device__ __forceinline__ void lab_unite(std::uint32_t* parent, std::uint32_t a, std::uint32_t b)
+{
+    uf_unite<__HIP_MEMORY_SCOPE_AGENT>(parent, a, b);
+}
+
+// Two-level labelling.  Level one (lab_block, one workgroup per block of
+// kLabBlock consecutive sites) labels the activated edges with both ends in
+// the block in LDS and sets parent[i] to the global index of i's block-local
+// root (a root of the global forest: the smallest index of its block
+// component).  Level two (a grid-wide kernel over lab_unite_once) unites the
+// activated edges that cross blocks on the global array.  The components are
+// those of a single-level union-find; the global atomics are only those of the
+// edges between blocks, on trees of depth one.
+#ifndef UCMC_LAB_BLOCK
+#define UCMC_LAB_BLOCK 1024
+#endif
+#ifndef UCMC_LAB_THREADS
+#define UCMC_LAB_THREADS 256
+#endif
+constexpr std::uint32_t kLabBlock = UCMC_LAB_BLOCK;     // sites per level-one block
+constexpr std::uint32_t kLabThreads = UCMC_LAB_THREADS; // threads of a level-one workgroup
+
+// A level-one block: `count` sites with local indices 0 .. count - 1 in
+// increasing global order.  site(t) is the global index of local t, holds(j)
+// whether global j is in the block, local(j) its local index, and run(t)
+// whether local t + 1 is the site after t (the only candidate for the run
+// pass below).  LabSpan: the consecutive sites [base, base + count).
+struct LabSpan {
+    std::uint32_t base, count;
+    __device__ __forceinline__ bool holds(std::uint32_t j) const { return j - base < count; }
+    __device__ __forceinline__ std::uint32_t site(std::uint32_t t) const { return base + t; }
+    __device__ __forceinline__ std::uint32_t local(std::uint32_t j) const { return j - base; }
+    __device__ __forceinline__ bool run(std::uint32_t t) const { return t + 1u < count; }
+};
+__device__ __forceinline__ LabSpan lab_span_of(std::uint32_t i, std::uint32_t n)
+{
+    const std::uint32_t base = i / kLabBlock * kLabBlock;
+    return LabSpan{base, n - base < kLabBlock ? n - base : kLabBlock};
+}
+
+// Level one for this workgroup's block (blockDim.x a multiple of 64).
+//   next_active(i): whether the edge (i, i + 1) is activated (its first
+//     occurrence among i's edges; false when there is none);
+//   inside_edges(i, blk, skip_next, unite): calls unite(j) for every
+//     activated edge (i, j) owned by i with blk.holds(j), except the first
+//     (i, i + 1) when skip_next.
+// Runs of sites linked by (i, i + 1) edges inside a wave's 64 consecutive
+// local indices are labelled first, at once, from one ballot (each site points
+// at its run's first site); a union-find that united them pairwise would build
+// chains as long as the runs.  The remaining internal edges are then united in
+// LDS.
+template <class Block, class NextActive, class InsideEdges>
+__device__ __forceinline__ void lab_block(std::uint32_t* parent, const Block& blk, NextActive next_active,
+                                          InsideEdges inside_edges)
+{
+    __shared__ std::uint32_t lp[kLabBlock];
+    const std::uint32_t lane = threadIdx.x & 63u;
+    for (std::uint32_t t0 = 0; t0 < blk.count; t0 += blockDim.x) {   // uniform trip count
+        const std::uint32_t t = t0 + threadIdx.x;
+        const bool valid = t < blk.count;
+        const bool linked = valid && lane != 63u && blk.run(t) && next_active(blk.site(t));
+        const unsigned long long m = __ballot(linked);
+        const unsigned long long starts = ~(m << 1);   // bit k: a run starts at lane k (bit 0 always)
+        const unsigned long long upto = lane == 63u ? ~0ull : (2ull << lane) - 1ull;
+        const std::uint32_t first = 63u - static_cast<std::uint32_t>(__clzll(starts & upto));
+        if (valid) lp[t] = t - lane + first;
+    }
+    __syncthreads();
+    for (std::uint32_t t = threadIdx.x; t < blk.count; t += blockDim.x)
+        inside_edges(blk.site(t), blk, (t & 63u) != 63u && blk.run(t), [&](std::uint32_t j) {
+            uf_unite<__HIP_MEMORY_SCOPE_WORKGROUP>(lp, t, blk.local(j));
+        });
+    __syncthreads();
+    // every unite has landed: plain walks to the roots
+    for (std::uint32_t t = threadIdx.x; t < blk.count; t += blockDim.x) {
+        std::uint32_t x = t;
+        for (std::uint32_t q = lp[x]; q != x; q = lp[x]) x = q;
+        parent[blk.site(t)] = blk.site(x);
+    }
+}
+
+// Level two, called by every lane of the wave at once: unites i and j where
+// `active`, except where the previous lane unites the same two roots in this
+// call (edges between two blocks mostly join the same pair of components).
+__device__ __forceinline__ void lab_unite_once(std::uint32_t* parent, bool active, std::uint32_t i, std::uint32_t j)
+{
+    std::uint32_t a = 0, b = 0;
+    if (active) {
+        a = lab_find(parent, i);
+        b = lab_find(parent, j);
+        if (a > b) {
+            const std::uint32_t t = a;
+            a = b;
+            b = t;
+        }
+    }
+    const std::uint32_t pa = __shfl_up(a, 1, 64), pb = __shfl_up(b, 1, 64);
+    const int pact = __shfl_up(active ? 1 : 0, 1, 64);
+    const bool dup = (threadIdx.x & 63u) != 0u && pact && pa == a && pb == b;
+    if (active && a != b && !dup) {
+        // a < b were roots when found: try the hook at once, else the full unite
+        std::uint32_t expected = b;
+        if (!__hip_atomic_compare_exchange_strong(parent + b, &expected, a, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT))
+            lab_unite(parent, expected, a);
     }
 }
 

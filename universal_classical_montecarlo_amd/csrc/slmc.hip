@@ -529,11 +529,12 @@ __global__ __launch_bounds__(1024) void slmc_flips(SlmcArgs a)
 }
 
 // ------------------------------------------------- label propagation path --
-// One SLMC step as six grid-wide kernels (the same cluster as slmc_flips):
-// parent[i] = i and the seed; every undirected edge (x, y) -> forward shell
-// neighbour activated with its keyed uniform and united; labels flattened;
-// integer boundary sums of the seed's component; the Metropolis-Hastings
-// decision; the flip.
+// One SLMC step as five grid-wide kernels (the same cluster as slmc_flips):
+// every undirected edge (x, y) -> forward shell neighbour activated with its
+// keyed uniform and united, inside blocks of sites in LDS (with the seed
+// draw), then across blocks; labels flattened; integer boundary sums of the
+// seed's component, the last block taking the Metropolis-Hastings decision;
+// the flip.
 struct SlmcLabelArgs {
     std::int8_t* s;
     std::uint32_t L, n;
@@ -546,32 +547,61 @@ struct SlmcLabelArgs {
     std::uint64_t thr[3][2];
 };
 
-__global__ __launch_bounds__(256) void slmc_label_begin(SlmcLabelArgs a)
+// Whether the edge from site i (spin si) to its direction-d neighbour j is
+// activated (the test of slmc_flips; no draw for an edge that cannot be).
+__device__ __forceinline__ bool slmc_edge_active(const SlmcLabelArgs& a, std::uint32_t i, int si, int d,
+                                                 std::uint32_t j)
 {
-    const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < a.n) a.parent[i] = i;
-    if (i == 0) {
+    const bool aligned = si * a.s[j] > 0;
+    const std::uint64_t thr = a.thr[kShell[d]][aligned ? 1 : 0];
+    if (thr == 0) return false;
+    return static_cast<std::uint64_t>(edge_u32(a.key, 6u * i + kSlot[d])) < thr;
+}
+
+// Level one of the labelling (each undirected edge from its owner, the site
+// of its forward direction), one workgroup per block of sites; block 0 also
+// draws the seed and clears the step's counters (read from the sums on).
+__global__ __launch_bounds__(kLabThreads) void slmc_label_local(SlmcLabelArgs a)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         Stream ctl;
         ctl.init(a.key, 0, kTagWolffCtrl);
         a.st->seed = static_cast<std::uint32_t>(ctl.uni() * static_cast<double>(a.n));
         a.st->skip = 0;
         for (int k = 0; k < 8; ++k) a.st->count[k] = 0;
     }
+    // direction 0 is +x: the edge (i, i + 1) unless x = L - 1
+    auto next_active = [&](std::uint32_t i) {
+        return i % a.L != a.L - 1u && slmc_edge_active(a, i, a.s[i], 0, i + 1u);
+    };
+    auto inside_edges = [&](std::uint32_t i, LabSpan span, bool skip_next, auto unite) {
+        const int si = a.s[i];
+        for (int d = 0; d < 12; ++d) {
+            if (!kForward[d]) continue;
+            const std::uint32_t j = nbr(a.L, i, d);
+            if (!span.holds(j) || (d == 0 && skip_next && j == i + 1u)) continue;
+            if (slmc_edge_active(a, i, si, d, j)) unite(j);
+        }
+    };
+    lab_block(a.parent, lab_span_of(blockIdx.x * kLabBlock, a.n), next_active, inside_edges);
 }
 
-__global__ __launch_bounds__(256) void slmc_label_link(SlmcLabelArgs a)
+// Level two: the activated edges between blocks.
+__global__ __launch_bounds__(256) void slmc_label_cross(SlmcLabelArgs a)
 {
     const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
-    const int si = a.s[i];
+    const bool valid = i < a.n;   // every lane takes part in each lab_unite_once
+    const LabSpan span = lab_span_of(valid ? i : 0u, a.n);
+    const int si = valid ? a.s[i] : 0;
     for (int d = 0; d < 12; ++d) {
-        if (!kForward[d]) continue;   // each undirected edge from its owner
-        const std::uint32_t j = nbr(a.L, i, d);
-        const bool aligned = si * a.s[j] > 0;
-        const std::uint64_t thr = a.thr[kShell[d]][aligned ? 1 : 0];
-        if (thr == 0) continue;   // never activated: no draw needed
-        const std::uint32_t u = edge_u32(a.key, 6u * i + kSlot[d]);
-        if (static_cast<std::uint64_t>(u) < thr) lab_unite(a.parent, i, j);
+        if (!kForward[d]) continue;
+        std::uint32_t j = 0;
+        bool act = false;
+        if (valid) {
+            j = nbr(a.L, i, d);
+            act = !span.holds(j) && slmc_edge_active(a, i, si, d, j);
+        }
+        lab_unite_once(a.parent, act, i, j);
     }
 }
 
@@ -599,27 +629,30 @@ __global__ __launch_bounds__(256) void slmc_label_sums(SlmcLabelArgs a)
     }
 }
 
-__global__ void slmc_label_decide(SlmcLabelArgs a)
-{
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    long long c[6];
-    for (int k = 0; k < 6; ++k) c[k] = static_cast<long long>(a.st->count[k]);
-    double dt, de;
-    slmc_energies(a.J1, a.K, a.Jeff, c, dt, de);
-    // min(1, exp(-beta (dE_true - dE_eff))), the draw after the seed draw
-    Stream ctl;
-    ctl.init(a.key, 0, kTagWolffCtrl);
-    (void)ctl.next();
-    const double log_p = -a.beta * (dt - de) + 0.0;
-    const int acc = (log_p >= 0.0 || log(ctl.uni()) < log_p) ? 1 : 0;
-    a.st->accept = static_cast<std::uint32_t>(acc);
-    slmc_out_record(a.out, static_cast<double>(c[5]), acc, dt, de, c);
-}
-
+// The Metropolis-Hastings decision on the complete sums (taken by every block,
+// block 0 writing the record), then the flip.
 __global__ __launch_bounds__(256) void slmc_label_apply(SlmcLabelArgs a)
 {
+    __shared__ int accept;
+    if (threadIdx.x == 0) {
+        long long c[6];
+        for (int k = 0; k < 6; ++k) c[k] = static_cast<long long>(a.st->count[k]);
+        double dt, de;
+        slmc_energies(a.J1, a.K, a.Jeff, c, dt, de);
+        // min(1, exp(-beta (dE_true - dE_eff))), the draw after the seed draw
+        Stream ctl;
+        ctl.init(a.key, 0, kTagWolffCtrl);
+        (void)ctl.next();
+        const double log_p = -a.beta * (dt - de) + 0.0;
+        accept = (log_p >= 0.0 || log(ctl.uni()) < log_p) ? 1 : 0;
+        if (blockIdx.x == 0) {
+            a.st->accept = static_cast<std::uint32_t>(accept);
+            slmc_out_record(a.out, static_cast<double>(c[5]), accept, dt, de, c);
+        }
+    }
+    __syncthreads();
     const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n || !a.st->accept) return;
+    if (i >= a.n || !accept) return;
     if (a.parent[i] == a.parent[a.st->seed]) a.s[i] = static_cast<std::int8_t>(-a.s[i]);
 }
 
@@ -1000,16 +1033,15 @@ void slmc_run(System& s, const double* Jeff, double beta, std::uint64_t run_seed
             la.thr[k][0] = a.thr[k][0];
             la.thr[k][1] = a.thr[k][1];
         }
-        const dim3 grid((la.n + 255) / 256);
+        const dim3 grid((la.n + 255) / 256), blocks((la.n + kLabBlock - 1) / kLabBlock);
         for (std::uint64_t f = 0; f < nflips; ++f) {
             la.key = key_of(run_seed, kSlmcCounterBase + flip0 + f);
             la.out = a.out + kOutStride * f;
-            hipLaunchKernelGGL(slmc_label_begin, grid, dim3(256), 0, s.stream, la);
-            hipLaunchKernelGGL(slmc_label_link, grid, dim3(256), 0, s.stream, la);
+            hipLaunchKernelGGL(slmc_label_local, blocks, dim3(kLabThreads), 0, s.stream, la);
+            hipLaunchKernelGGL(slmc_label_cross, grid, dim3(256), 0, s.stream, la);
             hipLaunchKernelGGL(lab_flatten, grid, dim3(256), 0, s.stream, la.parent, la.n,
                                static_cast<const ClusterState*>(la.st));
             hipLaunchKernelGGL(slmc_label_sums, grid, dim3(256), 0, s.stream, la);
-            hipLaunchKernelGGL(slmc_label_decide, dim3(1), dim3(64), 0, s.stream, la);
             hipLaunchKernelGGL(slmc_label_apply, grid, dim3(256), 0, s.stream, la);
         }
     } else {

@@ -78,6 +78,10 @@ struct System {
     // device tables
     DevBuf offsets, bonds, edge_id;        // CSR + undirected edge ids (Wolff)
     std::size_t n_edges = 0;
+    // Wolff on a periodic square (2) / simple-cubic (3) lattice whose CSR and
+    // edge ids are the canonical ones: neighbours and ids from coordinates.
+    // -1 not yet checked, 0 no, else the dimension.
+    int box_dims = -1;
     DevBuf colour_sites;                   // sites grouped by colour
     std::vector<std::size_t> colour_start; // ncolours + 1 offsets into colour_sites
     DevBuf field[2];                       // [cur] authoritative; [cur^1] scratch
@@ -165,7 +169,8 @@ bool detect_cubic(const std::uint32_t* off, const std::uint32_t* bonds, std::siz
 int greedy_colouring(const std::uint32_t* off, const std::uint32_t* bonds, std::size_t n,
                      std::vector<std::uint8_t>& colour);
 void upload_colour_lists(System& s);
-std::size_t build_edge_ids(System& s); // uploads edge_id; returns #undirected edges
+// uploads edge_id (and hands the host copy over when asked); returns #undirected edges
+std::size_t build_edge_ids(System& s, std::vector<std::uint32_t>* host_ids = nullptr);
 
 // Accept threshold of driver.hpp:83-84 as a count of accepted 32-bit draws
 // (see ising.hip): k* such that log(k 2^-32) < -beta dE  <=>  k < k*.

@@ -104,7 +104,7 @@ struct Field {
 template <int N>
 __global__ __launch_bounds__(kThreads) void wolff_flips(WolffArgs a)
 {
-    __shared__ std::uint32_t s_head, s_tail, s_seed;
+    __shared__ std::uint32_t s_head, s_tail;
     __shared__ double s_r[4];
     __shared__ double s_red[1 + 4];
     __shared__ unsigned long long s_touched;
@@ -132,7 +132,6 @@ __global__ __launch_bounds__(kThreads) void wolff_flips(WolffArgs a)
             for (int k = 0; k < 4; ++k) s_r[k] = r[k];
             const std::uint32_t seed =
                 static_cast<std::uint32_t>(dmul(ctl.uni(), static_cast<double>(a.n)));
-            s_seed = seed;
             a.stamp[seed] = mark;
             a.queue[0] = seed;
             s_head = 0;
@@ -264,11 +263,12 @@ __global__ __launch_bounds__(kThreads) void wolff_flips(WolffArgs a)
 }
 
 // ------------------------------------------------- label propagation path --
-// One flip as five grid-wide kernels: (1) parent[i] = i, axis and seed site;
-// (2) every undirected edge, from its smaller endpoint, is activated with the
-// same test as the breadth-first growth and its endpoints united; (3) labels
-// flattened; (4) size, energy and order change of the seed's component on the
-// pre-flip field; (5) the component reflected.  Cost O(N) per flip whatever
+// One flip as six kernels: (1) axis and seed site; (2, 3) every undirected
+// edge, from its smaller endpoint, is activated with the same test as the
+// breadth-first growth and its endpoints united, inside blocks of sites in LDS
+// and then across blocks; (4) labels flattened; (5) size, energy and order
+// change of the seed's component on the pre-flip field; (6) the component
+// reflected.  Cost O(N) per flip whatever
 // the cluster size, against O(cluster) levels of latency for the growth.
 struct LabelArgs {
     void* field;
@@ -290,9 +290,7 @@ struct LabelArgs {
 template <int N>
 __global__ __launch_bounds__(256) void label_begin(LabelArgs a)
 {
-    const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < a.n) a.parent[i] = i;
-    if (i == 0) {
+    if (threadIdx.x == 0) {
         ClusterState* st = a.st;
         st->skip = (a.touch_target && st->touched >= a.touch_target) ? 1u : 0u;
         // driver.hpp:154-159: the axis first, then the seed site
@@ -305,37 +303,91 @@ __global__ __launch_bounds__(256) void label_begin(LabelArgs a)
     }
 }
 
+// Whether CSR edge q = (i, j) is activated: the test of the breadth-first
+// growth on the pre-flip field (no draw for an edge that cannot be).
 template <int N>
-__global__ __launch_bounds__(256) void label_link(LabelArgs a)
+__device__ __forceinline__ bool label_edge_active(const LabelArgs& a, const Field<N>& fld, const double (&r)[4],
+                                                  double ai, std::uint32_t e, std::uint32_t j)
 {
-    const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n || a.st->skip) return;
-    double r[4];
-    for (int k = 0; k < 4; ++k) r[k] = a.st->r[k];
+    if constexpr (N == 1) {
+        const bool aligned = ai * fld.proj(j, r) > 0.0;
+        const std::uint64_t thr = aligned ? a.thr_aligned : a.thr_anti;
+        if (thr == 0) return false;
+        return static_cast<std::uint64_t>(edge_u32(a.key, e)) < thr;
+    } else {
+        const double aj = fld.proj(j, r);
+        const double dE = dmul(2.0 * a.J, dmul(ai, aj));
+        if (!(a.beta * dE > 0.0)) return false;   // p <= 0 (or NaN): never activated
+        const double p = 1.0 - exp(-a.beta * dE);
+        return static_cast<double>(edge_u32(a.key, e)) * 0x1.0p-32 < p;
+    }
+}
+
+template <int N>
+__device__ __forceinline__ Field<N> label_field(const LabelArgs& a)
+{
     Field<N> fld;
     if constexpr (N == 1) fld.f = static_cast<const std::int8_t*>(a.field);
     else {
         fld.f = static_cast<const float*>(a.field);
         fld.n = a.n;
     }
-    const double ai = fld.proj(i, r);
-    for (std::uint32_t q = a.offsets[i]; q < a.offsets[i + 1]; ++q) {
-        const std::uint32_t j = a.bonds[q];
-        if (j <= i) continue;   // each undirected edge from its smaller endpoint
-        bool act;
-        if constexpr (N == 1) {
-            const bool aligned = ai * fld.proj(j, r) > 0.0;
-            const std::uint64_t thr = aligned ? a.thr_aligned : a.thr_anti;
-            if (thr == 0) continue;   // never activated: no draw needed
-            act = static_cast<std::uint64_t>(edge_u32(a.key, a.edge_id[q])) < thr;
-        } else {
-            const double aj = fld.proj(j, r);
-            const double dE = dmul(2.0 * a.J, dmul(ai, aj));
-            if (!(a.beta * dE > 0.0)) continue;   // p <= 0 (or NaN): never activated
-            const double p = 1.0 - exp(-a.beta * dE);
-            act = static_cast<double>(edge_u32(a.key, a.edge_id[q])) * 0x1.0p-32 < p;
+    return fld;
+}
+
+// Level one of the labelling (each undirected edge from its smaller
+// endpoint), one workgroup per block of sites.
+template <int N>
+__global__ __launch_bounds__(kLabThreads) void label_local(LabelArgs a)
+{
+    if (a.st->skip) return;
+    double r[4];
+    for (int k = 0; k < 4; ++k) r[k] = a.st->r[k];
+    const Field<N> fld = label_field<N>(a);
+    auto next_active = [&](std::uint32_t i) {
+        for (std::uint32_t q = a.offsets[i]; q < a.offsets[i + 1]; ++q)
+            if (a.bonds[q] == i + 1u) return label_edge_active<N>(a, fld, r, fld.proj(i, r), a.edge_id[q], i + 1u);
+        return false;
+    };
+    auto inside_edges = [&](std::uint32_t i, LabSpan span, bool skip_next, auto unite) {
+        const double ai = fld.proj(i, r);
+        bool skip = skip_next;
+        for (std::uint32_t q = a.offsets[i]; q < a.offsets[i + 1]; ++q) {
+            const std::uint32_t j = a.bonds[q];
+            if (j <= i || !span.holds(j)) continue;
+            if (skip && j == i + 1u) {   // its first occurrence was labelled by the run pass
+                skip = false;
+                continue;
+            }
+            if (label_edge_active<N>(a, fld, r, ai, a.edge_id[q], j)) unite(j);
         }
-        if (act) lab_unite(a.parent, i, j);
+    };
+    lab_block(a.parent, lab_span_of(blockIdx.x * kLabBlock, a.n), next_active, inside_edges);
+}
+
+// Level two: the activated edges between blocks, the wave's lanes stepping
+// through their edge lists together.
+template <int N>
+__global__ __launch_bounds__(256) void label_cross(LabelArgs a)
+{
+    if (a.st->skip) return;
+    const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < a.n;   // every lane takes part in each lab_unite_once
+    double r[4];
+    for (int k = 0; k < 4; ++k) r[k] = a.st->r[k];
+    const Field<N> fld = label_field<N>(a);
+    const LabSpan span = lab_span_of(valid ? i : 0u, a.n);
+    const std::uint32_t q0 = valid ? a.offsets[i] : 0u, q1 = valid ? a.offsets[i + 1] : 0u;
+    const double ai = valid ? fld.proj(i, r) : 0.0;
+    for (std::uint32_t k = 0; __ballot(q0 + k < q1); ++k) {
+        const std::uint32_t q = q0 + k;
+        std::uint32_t j = 0;
+        bool act = false;
+        if (q < q1) {
+            j = a.bonds[q];
+            act = j > i && !span.holds(j) && label_edge_active<N>(a, fld, r, ai, a.edge_id[q], j);
+        }
+        lab_unite_once(a.parent, act, i, j);
     }
 }
 
@@ -422,16 +474,324 @@ __global__ __launch_bounds__(256) void label_apply(LabelArgs a)
     }
 }
 
-template <int N>
-void label_flips(System& s, LabelArgs a, const ucmc_wolff_params& p)
+// ------------------------------------------------------ periodic boxes --
+// Periodic square (D = 2) and simple-cubic (D = 3) lattices, x fastest, whose
+// CSR rows and undirected edge ids are the canonical ones (square_csr and the
+// cubic builder, ids numbered by build_edge_ids; checked on the host by
+// box_check): neighbours and edge ids come from the coordinates, so no CSR
+// array is read, and the level-one blocks are 2D / 3D tiles, so that few
+// edges cross blocks.
+//
+// Edge ids: site c owns its +d edges where c_d < L_d - 1 and its -d wrap
+// edges where c_d = 0, listed +x, +y, (+z), -x, -y, (-z); ids run over the
+// sites in index order and over each site's list.  With a_d(t) = [t < L_d - 1]
+// + [t = 0] (its owned edges along d) and A(t) = sum_{t' < t} a_d(t') = (t ? t + 1
+// : 0), the ids before site (x, y) are y Lx + Lx A(y) + A(x) + x a_y(y), and
+// before (x, y, z): 2 z Lx Ly + Lx Ly A(z) + y Lx + Lx A(y) + y Lx a_z(z) +
+// A(x) + x (a_y(y) + a_z(z)).
+template <int D>
+struct Box {
+    std::uint32_t L[3];   // extents (L[2] unused for D = 2)
+
+    __device__ __forceinline__ void coords(std::uint32_t i, std::uint32_t* c) const
+    {
+        c[0] = i % L[0];
+        const std::uint32_t r = i / L[0];
+        if constexpr (D == 2) c[1] = r;
+        else {
+            c[1] = r % L[1];
+            c[2] = r / L[1];
+        }
+    }
+    __device__ __forceinline__ std::uint32_t index(const std::uint32_t* c) const
+    {
+        if constexpr (D == 2) return c[1] * L[0] + c[0];
+        else return (c[2] * L[1] + c[1]) * L[0] + c[0];
+    }
+    // neighbour of c one step along +d (sign > 0) or -d, periodic
+    __device__ __forceinline__ std::uint32_t step(const std::uint32_t* c, int d, int sign) const
+    {
+        std::uint32_t n[3] = {c[0], c[1], D == 3 ? c[2] : 0u};
+        n[d] = sign > 0 ? (c[d] + 1u == L[d] ? 0u : c[d] + 1u) : (c[d] == 0u ? L[d] - 1u : c[d] - 1u);
+        return index(n);
+    }
+    __device__ static __forceinline__ std::uint32_t own(std::uint32_t t, std::uint32_t Ld)
+    {
+        return (t + 1u < Ld ? 1u : 0u) + (t == 0u ? 1u : 0u);
+    }
+    __device__ static __forceinline__ std::uint32_t own_before(std::uint32_t t) { return t ? t + 1u : 0u; }
+    __device__ __forceinline__ std::uint32_t prefix(const std::uint32_t* c) const
+    {
+        if constexpr (D == 2)
+            return c[1] * L[0] + L[0] * own_before(c[1]) + own_before(c[0]) + c[0] * own(c[1], L[1]);
+        else {
+            const std::uint32_t LxLy = L[0] * L[1], az = own(c[2], L[2]);
+            return 2u * c[2] * LxLy + LxLy * own_before(c[2]) + c[1] * L[0] + L[0] * own_before(c[1]) +
+                   c[1] * L[0] * az + own_before(c[0]) + c[0] * (own(c[1], L[1]) + az);
+        }
+    }
+    // rank of list entry `slot` (0 .. D-1: +d, D .. 2D-1: -d wrap) among c's owned edges
+    __device__ __forceinline__ std::uint32_t rank(const std::uint32_t* c, int slot) const
+    {
+        std::uint32_t r = 0;
+        for (int k = 0; k < slot; ++k) r += k < D ? (c[k] + 1u < L[k] ? 1u : 0u) : (c[k - D] == 0u ? 1u : 0u);
+        return r;
+    }
+    // id of the undirected edge between c and its +d neighbour
+    __device__ __forceinline__ std::uint32_t edge(const std::uint32_t* c, int d) const
+    {
+        if (c[d] + 1u < L[d]) return prefix(c) + rank(c, d);
+        std::uint32_t o[3] = {c[0], c[1], D == 3 ? c[2] : 0u};
+        o[d] = 0;   // the wrap edge belongs to the site at coordinate 0
+        return prefix(o) + rank(o, D + d);
+    }
+};
+
+// Level-one tiles: 64 x 16 (D = 2), 16 x 8 x 8 (D = 3); kLabBlock sites each.
+template <int D>
+struct BoxTile {
+    static constexpr std::uint32_t T[3] = {D == 2 ? 64u : 16u, D == 2 ? 16u : 8u, D == 2 ? 1u : 8u};
+    static_assert(T[0] * T[1] * T[2] == kLabBlock, "one tile per level-one block");
+    Box<D> box;
+    std::uint32_t o[3];   // origin
+    std::uint32_t count;
+
+    __device__ __forceinline__ BoxTile(const Box<D>& b, std::uint32_t tile) : box(b), count(kLabBlock)
+    {
+        const std::uint32_t n0 = b.L[0] / T[0], n1 = b.L[1] / T[1];
+        o[0] = tile % n0 * T[0];
+        o[1] = (tile / n0) % n1 * T[1];
+        o[2] = D == 3 ? tile / (n0 * n1) * T[2] : 0u;
+    }
+    __device__ __forceinline__ std::uint32_t site(std::uint32_t t) const
+    {
+        const std::uint32_t c[3] = {o[0] + t % T[0], o[1] + (t / T[0]) % T[1], o[2] + t / (T[0] * T[1])};
+        return box.index(c);
+    }
+    __device__ __forceinline__ bool holds(std::uint32_t j) const
+    {
+        std::uint32_t c[3];
+        box.coords(j, c);
+        bool in = c[0] - o[0] < T[0] && c[1] - o[1] < T[1];
+        if constexpr (D == 3) in = in && c[2] - o[2] < T[2];
+        return in;
+    }
+    __device__ __forceinline__ std::uint32_t local(std::uint32_t j) const
+    {
+        std::uint32_t c[3];
+        box.coords(j, c);
+        const std::uint32_t z = D == 3 ? c[2] - o[2] : 0u;
+        return (z * T[1] + (c[1] - o[1])) * T[0] + (c[0] - o[0]);
+    }
+    __device__ __forceinline__ bool run(std::uint32_t t) const { return t % T[0] != T[0] - 1u; }
+};
+
+template <int N, int D>
+__global__ __launch_bounds__(kLabThreads) void box_local(LabelArgs a, Box<D> box)
 {
-    const dim3 grid((a.n + 255) / 256);
+    if (a.st->skip) return;
+    double r[4];
+    for (int k = 0; k < 4; ++k) r[k] = a.st->r[k];
+    const Field<N> fld = label_field<N>(a);
+    auto next_active = [&](std::uint32_t i) {   // the +x edge, inside the tile
+        std::uint32_t c[3];
+        box.coords(i, c);
+        return label_edge_active<N>(a, fld, r, fld.proj(i, r), box.edge(c, 0), i + 1u);
+    };
+    auto inside_edges = [&](std::uint32_t i, const BoxTile<D>& blk, bool skip_next, auto unite) {
+        std::uint32_t c[3];
+        box.coords(i, c);
+        const double ai = fld.proj(i, r);
+        for (int d = skip_next ? 1 : 0; d < D; ++d) {
+            const std::uint32_t j = box.step(c, d, 1);
+            if (blk.holds(j) && label_edge_active<N>(a, fld, r, ai, box.edge(c, d), j)) unite(j);
+        }
+    };
+    lab_block(a.parent, BoxTile<D>(box, blockIdx.x), next_active, inside_edges);
+}
+
+// Level two: the +d edges leaving a tile (c_d at the tile's last layer).
+template <int N, int D>
+__global__ __launch_bounds__(256) void box_cross(LabelArgs a, Box<D> box)
+{
+    if (a.st->skip) return;
+    const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < a.n;   // every lane takes part in each lab_unite_once
+    double r[4];
+    for (int k = 0; k < 4; ++k) r[k] = a.st->r[k];
+    const Field<N> fld = label_field<N>(a);
+    std::uint32_t c[3] = {0, 0, 0};
+    if (valid) box.coords(i, c);
+    for (int d = 0; d < D; ++d) {
+        std::uint32_t j = 0;
+        bool act = false;
+        if (valid && c[d] % BoxTile<D>::T[d] == BoxTile<D>::T[d] - 1u) {
+            j = box.step(c, d, 1);
+            act = label_edge_active<N>(a, fld, r, fld.proj(i, r), box.edge(c, d), j);
+        }
+        lab_unite_once(a.parent, act, i, j);
+    }
+}
+
+// label_delta with the 2D neighbours from the coordinates.
+template <int N, int D>
+__global__ __launch_bounds__(256) void box_delta(LabelArgs a, Box<D> box)
+{
+    const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a.st->skip) return;
+    double r[4];
+    for (int k = 0; k < 4; ++k) r[k] = a.st->r[k];
+    const std::uint32_t root = a.parent[a.st->seed];
+    const Field<N> fld = label_field<N>(a);
+    unsigned long long cnt = 0;
+    double de = 0.0, dm[4] = {0.0, 0.0, 0.0, 0.0};
+    if (i < a.n && a.parent[i] == root) {
+        cnt = 1;
+        std::uint32_t c[3];
+        box.coords(i, c);
+        std::uint32_t nb[2 * D];
+        for (int d = 0; d < D; ++d) {   // the CSR row order does not matter to the sums below
+            nb[2 * d] = box.step(c, d, 1);
+            nb[2 * d + 1] = box.step(c, d, -1);
+        }
+        if constexpr (N == 1) {
+            const int si = fld.f[i];
+            int sum = 0;
+            for (int q = 0; q < 2 * D; ++q)
+                if (a.parent[nb[q]] != root) sum += fld.f[nb[q]];
+            de = 2.0 * a.J * static_cast<double>(si * sum);
+            dm[0] = -2.0 * si;
+        } else {
+            const double ai = fld.proj(i, r);
+            double h[N];
+            for (int k = 0; k < N; ++k) h[k] = 0.0;
+            for (int q = 0; q < 2 * D; ++q)
+                if (a.parent[nb[q]] != root)
+                    for (int k = 0; k < N; ++k) h[k] += fld.f[k * a.n + nb[q]];
+            for (int k = 0; k < N; ++k) {
+                const double s = fld.f[k * a.n + i];
+                const float sn = static_cast<float>(dadd(s, -dmul(dmul(2.0, ai), r[k])));
+                de += -a.J * (static_cast<double>(sn) - s) * h[k];
+                dm[k] = static_cast<double>(sn) - s;
+            }
+        }
+    }
+    __shared__ double part[4][6];
+    const std::uint32_t w = threadIdx.x >> 6;
+    const double vals[6] = {static_cast<double>(cnt), de, dm[0], dm[1], dm[2], dm[3]};
+    for (int k = 0; k < 2 + N; ++k) {
+        const double v = wave_sum(vals[k]);
+        if ((threadIdx.x & 63u) == 0) part[w][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < static_cast<unsigned>(2 + N)) {
+        const double v = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] + part[3][threadIdx.x];
+        if (threadIdx.x == 0 && v != 0.0) atomicAdd(&a.st->touched, static_cast<unsigned long long>(v));
+        if (v != 0.0) atomicAdd(&a.out[threadIdx.x], v);
+    }
+}
+
+template <int N, int D>
+void box_flips(System& s, LabelArgs a, const ucmc_wolff_params& p)
+{
+    Box<D> box{};
+    if constexpr (D == 2) {
+        box.L[0] = static_cast<std::uint32_t>(s.Lx);
+        box.L[1] = static_cast<std::uint32_t>(s.Ly);
+    } else {
+        box.L[0] = box.L[1] = box.L[2] = static_cast<std::uint32_t>(s.L);
+    }
+    const dim3 grid((a.n + 255) / 256), tiles(a.n / kLabBlock);
     for (std::uint64_t f = 0; f < p.nflips; ++f) {
         a.key = key_of(p.run_seed, kWolffCounterBase + p.flip0 + f);
         LabelArgs af = a;
         af.out = a.out + 6 * f;
-        hipLaunchKernelGGL((label_begin<N>), grid, dim3(256), 0, s.stream, af);
-        hipLaunchKernelGGL((label_link<N>), grid, dim3(256), 0, s.stream, af);
+        hipLaunchKernelGGL((label_begin<N>), dim3(1), dim3(64), 0, s.stream, af);
+        hipLaunchKernelGGL((box_local<N, D>), tiles, dim3(kLabThreads), 0, s.stream, af, box);
+        hipLaunchKernelGGL((box_cross<N, D>), grid, dim3(256), 0, s.stream, af, box);
+        hipLaunchKernelGGL(lab_flatten, grid, dim3(256), 0, s.stream, af.parent, af.n,
+                           static_cast<const ClusterState*>(af.st));
+        hipLaunchKernelGGL((box_delta<N, D>), grid, dim3(256), 0, s.stream, af, box);
+        hipLaunchKernelGGL((label_apply<N>), grid, dim3(256), 0, s.stream, af);
+    }
+    UCMC_HIP(hipGetLastError());
+}
+
+// Whether the system's lattice takes the box path: a periodic square lattice
+// with Lx % 64 == 0 and Ly % 16 == 0, or a cubic one with L % 16 == 0, whose
+// CSR neighbours and edge ids are the canonical ones (checked entry by entry
+// once; any other order keeps the CSR path, which reads the ids).
+int box_check(System& s, const std::vector<std::uint32_t>& id)
+{
+    int D = 0;
+    std::size_t L[3] = {1, 1, 1};
+    if (s.geom == Geom::Square && s.Lx % 64 == 0 && s.Ly % 16 == 0) {
+        D = 2;
+        L[0] = s.Lx;
+        L[1] = s.Ly;
+    } else if (s.geom == Geom::Cubic && s.L % 16 == 0) {
+        D = 3;
+        L[0] = L[1] = L[2] = s.L;
+    }
+    if (!D || L[0] * L[1] * L[2] != s.sites) return 0;
+    auto own = [](std::size_t t, std::size_t Ld) { return (t + 1 < Ld ? 1u : 0u) + (t == 0 ? 1u : 0u); };
+    auto before = [](std::size_t t) { return t ? t + 1 : 0; };
+    auto prefix = [&](const std::size_t* c) -> std::size_t {
+        if (D == 2) return c[1] * L[0] + L[0] * before(c[1]) + before(c[0]) + c[0] * own(c[1], L[1]);
+        const std::size_t LxLy = L[0] * L[1], az = own(c[2], L[2]);
+        return 2 * c[2] * LxLy + LxLy * before(c[2]) + c[1] * L[0] + L[0] * before(c[1]) + c[1] * L[0] * az +
+               before(c[0]) + c[0] * (own(c[1], L[1]) + az);
+    };
+    auto rank = [&](const std::size_t* c, int slot) {
+        std::size_t r = 0;
+        for (int k = 0; k < slot; ++k) r += k < D ? (c[k] + 1 < L[k]) : (c[k - D] == 0);
+        return r;
+    };
+    auto edge = [&](const std::size_t* c, int d) {
+        if (c[d] + 1 < L[d]) return prefix(c) + rank(c, d);
+        std::size_t o[3] = {c[0], c[1], c[2]};
+        o[d] = 0;
+        return prefix(o) + rank(o, D + d);
+    };
+    const auto& off = s.h_offsets;
+    const auto& b = s.h_bonds;
+    std::size_t c[3] = {0, 0, 0};
+    for (std::size_t i = 0; i < s.sites; ++i) {
+        c[0] = i % L[0];
+        c[1] = (i / L[0]) % L[1];
+        c[2] = i / (L[0] * L[1]);
+        if (off[i + 1] - off[i] != static_cast<std::uint32_t>(2 * D)) return 0;
+        int seen = 0;
+        for (std::uint32_t q = off[i]; q < off[i + 1]; ++q) {
+            bool ok = false;
+            for (int d = 0; d < D && !ok; ++d)
+                for (int sg = 0; sg < 2 && !ok; ++sg) {
+                    std::size_t n[3] = {c[0], c[1], c[2]};
+                    n[d] = sg == 0 ? (c[d] + 1) % L[d] : (c[d] + L[d] - 1) % L[d];
+                    if (b[q] != (n[2] * L[1] + n[1]) * L[0] + n[0]) continue;
+                    const std::size_t e = sg == 0 ? edge(c, d) : edge(n, d);
+                    ok = id[q] == e && !(seen & (1 << (2 * d + sg)));
+                    seen |= 1 << (2 * d + sg);
+                }
+            if (!ok) return 0;
+        }
+    }
+    return D;
+}
+
+template <int N>
+void label_flips(System& s, LabelArgs a, const ucmc_wolff_params& p)
+{
+    if (s.box_dims == 2) return box_flips<N, 2>(s, a, p);
+    if (s.box_dims == 3) return box_flips<N, 3>(s, a, p);
+    const dim3 grid((a.n + 255) / 256), blocks((a.n + kLabBlock - 1) / kLabBlock);
+    for (std::uint64_t f = 0; f < p.nflips; ++f) {
+        a.key = key_of(p.run_seed, kWolffCounterBase + p.flip0 + f);
+        LabelArgs af = a;
+        af.out = a.out + 6 * f;
+        hipLaunchKernelGGL((label_begin<N>), dim3(1), dim3(64), 0, s.stream, af);
+        hipLaunchKernelGGL((label_local<N>), blocks, dim3(kLabThreads), 0, s.stream, af);
+        hipLaunchKernelGGL((label_cross<N>), grid, dim3(256), 0, s.stream, af);
         hipLaunchKernelGGL(lab_flatten, grid, dim3(256), 0, s.stream, af.parent, af.n,
                            static_cast<const ClusterState*>(af.st));
         hipLaunchKernelGGL((label_delta<N>), grid, dim3(256), 0, s.stream, af);
@@ -448,7 +808,11 @@ void wolff_run(System& s, const ucmc_wolff_params& p, std::uint64_t* sizes, ucmc
     if (p.nflips == 0) return;
     if (p.nflips > (1u << 30)) fail(UCMC_E_INVALID_ARGUMENT, "too many flips in one call");
     if (s.kind == UCMC_ISING && !s.offsets.p) square_csr(s);
-    if (!s.edge_id.p) build_edge_ids(s);
+    if (!s.edge_id.p) {
+        std::vector<std::uint32_t> ids;
+        build_edge_ids(s, &ids);
+        s.box_dims = box_check(s, ids);
+    }
     if (!s.stamp.p) {
         s.stamp.alloc(s.sites * 4);
         UCMC_HIP(hipMemsetAsync(s.stamp.p, 0, s.sites * 4, s.stream));
