@@ -133,13 +133,15 @@ __device__ __forceinline__ LabSpan lab_span_of(std::uint32_t i, std::uint32_t n)
     return LabSpan{base, n - base < kLabBlock ? n - base : kLabBlock};
 }
 
-// Level one for this workgroup's block (blockDim.x a multiple of 64).
-//   next_active(i): whether the edge (i, i + 1) is activated (its first
-//     occurrence among i's edges; false when there is none);
-//   inside_edges(i, blk, skip_next, unite): calls unite(j) for every
-//     activated edge (i, j) owned by i with blk.holds(j), except the first
-//     (i, i + 1) when skip_next.
-// Runs of sites linked by (i, i + 1) edges inside a wave's 64 consecutive
+// Level one for this workgroup's block (blockDim.x a multiple of 64), with
+// local indices t:
+//   next_active(t): whether the edge (site(t), site(t) + 1) is activated (its
+//     first occurrence among the site's edges; false when there is none);
+//     called only where blk.run(t);
+//   inside_edges(t, skip_next, unite): calls unite(u) for every activated edge
+//     owned by site(t) whose other end is local u of the block, except that
+//     first (site(t), site(t) + 1) edge when skip_next.
+// Runs of sites linked by (t, t + 1) edges inside a wave's 64 consecutive
 // local indices are labelled first, at once, from one ballot (each site points
 // at its run's first site); a union-find that united them pairwise would build
 // chains as long as the runs.  The remaining internal edges are then united in
@@ -153,7 +155,7 @@ __device__ __forceinline__ void lab_block(std::uint32_t* parent, const Block& bl
     for (std::uint32_t t0 = 0; t0 < blk.count; t0 += blockDim.x) {   // uniform trip count
         const std::uint32_t t = t0 + threadIdx.x;
         const bool valid = t < blk.count;
-        const bool linked = valid && lane != 63u && blk.run(t) && next_active(blk.site(t));
+        const bool linked = valid && lane != 63u && blk.run(t) && next_active(t);
         const unsigned long long m = __ballot(linked);
         const unsigned long long starts = ~(m << 1);   // bit k: a run starts at lane k (bit 0 always)
         const unsigned long long upto = lane == 63u ? ~0ull : (2ull << lane) - 1ull;
@@ -162,9 +164,8 @@ __device__ __forceinline__ void lab_block(std::uint32_t* parent, const Block& bl
     }
     __syncthreads();
     for (std::uint32_t t = threadIdx.x; t < blk.count; t += blockDim.x)
-        inside_edges(blk.site(t), blk, (t & 63u) != 63u && blk.run(t), [&](std::uint32_t j) {
-            uf_unite<__HIP_MEMORY_SCOPE_WORKGROUP>(lp, t, blk.local(j));
-        });
+        inside_edges(t, (t & 63u) != 63u && blk.run(t),
+                     [&](std::uint32_t u) { uf_unite<__HIP_MEMORY_SCOPE_WORKGROUP>(lp, t, u); });
     __syncthreads();
     // every unite has landed: plain walks to the roots
     for (std::uint32_t t = threadIdx.x; t < blk.count; t += blockDim.x) {
@@ -205,14 +206,19 @@ namespace {
 
 // parent[i] = root of i for every site (one thread per site, after all unites).
 // The walk to the root only reads: a halving store here could replace a root
-// pointer another thread has just written with a non-root ancestor.
+// pointer another thread has just written with a non-root ancestor.  Every
+// unite has finished (an earlier kernel), so the only concurrent stores are
+// this kernel's own root pointers, and any value a load returns, old or new,
+// is an ancestor: the loads may be served from the caches (workgroup scope).
 __global__ void lab_flatten(std::uint32_t* parent, std::uint32_t n, const ClusterState* st)
 {
     const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n || st->skip) return;
     std::uint32_t x = i;
-    for (std::uint32_t p = lab_load(parent + x); p != x; p = lab_load(parent + x)) x = p;
-    lab_store(parent + i, x);
+    for (std::uint32_t p = uf_load<__HIP_MEMORY_SCOPE_WORKGROUP>(parent + x); p != x;
+         p = uf_load<__HIP_MEMORY_SCOPE_WORKGROUP>(parent + x))
+        x = p;
+    uf_store<__HIP_MEMORY_SCOPE_WORKGROUP>(parent + i, x);
 }
 
 }  // namespace

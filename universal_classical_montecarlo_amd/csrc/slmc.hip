@@ -561,7 +561,7 @@ __device__ __forceinline__ bool slmc_edge_active(const SlmcLabelArgs& a, std::ui
 // Level one of the labelling (each undirected edge from its owner, the site
 // of its forward direction), one workgroup per block of sites; block 0 also
 // draws the seed and clears the step's counters (read from the sums on).
-__global__ __launch_bounds__(kLabThreads) void slmc_label_local(SlmcLabelArgs a)
+__global__ __launch_bounds__(1024) void slmc_label_local(SlmcLabelArgs a)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         Stream ctl;
@@ -570,20 +570,23 @@ __global__ __launch_bounds__(kLabThreads) void slmc_label_local(SlmcLabelArgs a)
         a.st->skip = 0;
         for (int k = 0; k < 8; ++k) a.st->count[k] = 0;
     }
+    const LabSpan span = lab_span_of(blockIdx.x * kLabBlock, a.n);
     // direction 0 is +x: the edge (i, i + 1) unless x = L - 1
-    auto next_active = [&](std::uint32_t i) {
+    auto next_active = [&](std::uint32_t t) {
+        const std::uint32_t i = span.site(t);
         return i % a.L != a.L - 1u && slmc_edge_active(a, i, a.s[i], 0, i + 1u);
     };
-    auto inside_edges = [&](std::uint32_t i, LabSpan span, bool skip_next, auto unite) {
+    auto inside_edges = [&](std::uint32_t t, bool skip_next, auto unite) {
+        const std::uint32_t i = span.site(t);
         const int si = a.s[i];
         for (int d = 0; d < 12; ++d) {
             if (!kForward[d]) continue;
             const std::uint32_t j = nbr(a.L, i, d);
             if (!span.holds(j) || (d == 0 && skip_next && j == i + 1u)) continue;
-            if (slmc_edge_active(a, i, si, d, j)) unite(j);
+            if (slmc_edge_active(a, i, si, d, j)) unite(span.local(j));
         }
     };
-    lab_block(a.parent, lab_span_of(blockIdx.x * kLabBlock, a.n), next_active, inside_edges);
+    lab_block(a.parent, span, next_active, inside_edges);
 }
 
 // Level two: the activated edges between blocks.
@@ -1037,7 +1040,7 @@ void slmc_run(System& s, const double* Jeff, double beta, std::uint64_t run_seed
         for (std::uint64_t f = 0; f < nflips; ++f) {
             la.key = key_of(run_seed, kSlmcCounterBase + flip0 + f);
             la.out = a.out + kOutStride * f;
-            hipLaunchKernelGGL(slmc_label_local, blocks, dim3(kLabThreads), 0, s.stream, la);
+            hipLaunchKernelGGL(slmc_label_local, blocks, dim3(1024), 0, s.stream, la);   // one site per thread: few blocks at these sizes
             hipLaunchKernelGGL(slmc_label_cross, grid, dim3(256), 0, s.stream, la);
             hipLaunchKernelGGL(lab_flatten, grid, dim3(256), 0, s.stream, la.parent, la.n,
                                static_cast<const ClusterState*>(la.st));

@@ -92,7 +92,7 @@ struct System {
     // stamp == mark_f, so no per-flip clear), the BFS queue, per-flip outputs
     DevBuf stamp, queue, flip_out;
     // label-propagation clusters: parent array and the per-flip control block
-    DevBuf labels, cluster_state;
+    DevBuf labels, cluster_state, label_part;   // + per-block partial sums of a flip
     DevBuf local_gram;   // SLMC: FP64 local-field Gram matrix of the training set
     std::uint32_t wolff_mark = 0;
     // parallel tempering batch (XY replicas)

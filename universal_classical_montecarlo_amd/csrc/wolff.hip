@@ -279,6 +279,7 @@ struct LabelArgs {
     std::uint32_t* parent;
     ClusterState* st;
     double* out;                   // this flip's [size, dE, dM0..dM3]
+    double* part;                  // per-block partial sums of out, 6 per block
     std::uint64_t key;
     std::uint64_t touch_target;
     double beta;
@@ -344,12 +345,15 @@ __global__ __launch_bounds__(kLabThreads) void label_local(LabelArgs a)
     double r[4];
     for (int k = 0; k < 4; ++k) r[k] = a.st->r[k];
     const Field<N> fld = label_field<N>(a);
-    auto next_active = [&](std::uint32_t i) {
+    const LabSpan span = lab_span_of(blockIdx.x * kLabBlock, a.n);
+    auto next_active = [&](std::uint32_t t) {
+        const std::uint32_t i = span.site(t);
         for (std::uint32_t q = a.offsets[i]; q < a.offsets[i + 1]; ++q)
             if (a.bonds[q] == i + 1u) return label_edge_active<N>(a, fld, r, fld.proj(i, r), a.edge_id[q], i + 1u);
         return false;
     };
-    auto inside_edges = [&](std::uint32_t i, LabSpan span, bool skip_next, auto unite) {
+    auto inside_edges = [&](std::uint32_t t, bool skip_next, auto unite) {
+        const std::uint32_t i = span.site(t);
         const double ai = fld.proj(i, r);
         bool skip = skip_next;
         for (std::uint32_t q = a.offsets[i]; q < a.offsets[i + 1]; ++q) {
@@ -359,10 +363,10 @@ __global__ __launch_bounds__(kLabThreads) void label_local(LabelArgs a)
                 skip = false;
                 continue;
             }
-            if (label_edge_active<N>(a, fld, r, ai, a.edge_id[q], j)) unite(j);
+            if (label_edge_active<N>(a, fld, r, ai, a.edge_id[q], j)) unite(span.local(j));
         }
     };
-    lab_block(a.parent, lab_span_of(blockIdx.x * kLabBlock, a.n), next_active, inside_edges);
+    lab_block(a.parent, span, next_active, inside_edges);
 }
 
 // Level two: the activated edges between blocks, the wave's lanes stepping
@@ -391,64 +395,99 @@ __global__ __launch_bounds__(256) void label_cross(LabelArgs a)
     }
 }
 
+// The block's sums of size, dE and dM (wave sums, then the four waves): one
+// row of `part` per block, reduced in a fixed order by label_reduce (no
+// atomics on a few hot words from every block that holds a cluster site).
+template <int N>
+__device__ __forceinline__ void block_partials(const LabelArgs& a, bool any_in_wave, const double (&vals)[6])
+{
+    __shared__ double wpart[4][6];
+    const std::uint32_t w = threadIdx.x >> 6;
+    for (int k = 0; k < 2 + N; ++k) {
+        const double v = any_in_wave ? wave_sum(vals[k]) : 0.0;
+        if ((threadIdx.x & 63u) == 0) wpart[w][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < static_cast<unsigned>(2 + N))
+        a.part[6 * static_cast<std::size_t>(blockIdx.x) + threadIdx.x] =
+            wpart[0][threadIdx.x] + wpart[1][threadIdx.x] + wpart[2][threadIdx.x] + wpart[3][threadIdx.x];
+}
+
+// out[k] = sum over blocks of part[.][k]; touched += size.  One workgroup.
+template <int N>
+__global__ __launch_bounds__(256) void label_reduce(LabelArgs a, std::uint32_t nblocks)
+{
+    if (a.st->skip) return;
+    __shared__ double red[4][6];
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (std::uint32_t b = threadIdx.x; b < nblocks; b += blockDim.x)
+        for (int k = 0; k < 2 + N; ++k) acc[k] += a.part[6 * static_cast<std::size_t>(b) + k];
+    for (int k = 0; k < 2 + N; ++k) {
+        const double v = wave_sum(acc[k]);
+        if ((threadIdx.x & 63u) == 0) red[threadIdx.x >> 6][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < static_cast<unsigned>(2 + N)) {
+        const double v = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+        a.out[threadIdx.x] = v;
+        if (threadIdx.x == 0) a.st->touched += static_cast<unsigned long long>(v);
+    }
+}
+
+// Adds site i's share, if it is in the cluster, to the flip's size, energy
+// change and order change on the pre-flip field (acc[0], acc[1], acc[2 ..]).
+// nbrs(visit) calls visit(j) for every neighbour j of i.
+template <int N, class Nbrs>
+__device__ __forceinline__ void delta_site(const LabelArgs& a, const Field<N>& fld, const double (&r)[4],
+                                           std::uint32_t root, std::uint32_t i, Nbrs nbrs, double (&acc)[6])
+{
+    if (a.parent[i] != root) return;
+    acc[0] += 1.0;
+    if constexpr (N == 1) {
+        const int si = fld.f[i];
+        int sum = 0;
+        nbrs([&](std::uint32_t j) {
+            if (a.parent[j] != root) sum += fld.f[j];
+        });
+        acc[1] += 2.0 * a.J * static_cast<double>(si * sum);
+        acc[2] += -2.0 * si;
+    } else {
+        const double ai = fld.proj(i, r);
+        double h[N];
+        for (int k = 0; k < N; ++k) h[k] = 0.0;
+        nbrs([&](std::uint32_t j) {
+            if (a.parent[j] != root)
+                for (int k = 0; k < N; ++k) h[k] += fld.f[k * a.n + j];
+        });
+        double de = 0.0;
+        for (int k = 0; k < N; ++k) {
+            const double s = fld.f[k * a.n + i];
+            const float sn = static_cast<float>(dadd(s, -dmul(dmul(2.0, ai), r[k])));
+            de += -a.J * (static_cast<double>(sn) - s) * h[k];
+            acc[2 + k] += static_cast<double>(sn) - s;
+        }
+        acc[1] += de;
+    }
+}
+
+// Blocks of the delta kernels (grid-stride over the sites), so that the
+// partial sums label_reduce adds up stay few.
+constexpr std::uint32_t kDeltaBlocks = 2048;
+
 template <int N>
 __global__ __launch_bounds__(256) void label_delta(LabelArgs a)
 {
-    const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (a.st->skip) return;
     double r[4];
     for (int k = 0; k < 4; ++k) r[k] = a.st->r[k];
     const std::uint32_t root = a.parent[a.st->seed];
-    Field<N> fld;
-    if constexpr (N == 1) fld.f = static_cast<const std::int8_t*>(a.field);
-    else {
-        fld.f = static_cast<const float*>(a.field);
-        fld.n = a.n;
-    }
-    unsigned long long cnt = 0;
-    double de = 0.0, dm[4] = {0.0, 0.0, 0.0, 0.0};
-    if (i < a.n && a.parent[i] == root) {
-        cnt = 1;
-        if constexpr (N == 1) {
-            const int si = fld.f[i];
-            int sum = 0;
-            for (std::uint32_t q = a.offsets[i]; q < a.offsets[i + 1]; ++q) {
-                const std::uint32_t j = a.bonds[q];
-                if (a.parent[j] != root) sum += fld.f[j];
-            }
-            de = 2.0 * a.J * static_cast<double>(si * sum);
-            dm[0] = -2.0 * si;
-        } else {
-            const double ai = fld.proj(i, r);
-            double h[N];
-            for (int k = 0; k < N; ++k) h[k] = 0.0;
-            for (std::uint32_t q = a.offsets[i]; q < a.offsets[i + 1]; ++q) {
-                const std::uint32_t j = a.bonds[q];
-                if (a.parent[j] != root)
-                    for (int k = 0; k < N; ++k) h[k] += fld.f[k * a.n + j];
-            }
-            for (int k = 0; k < N; ++k) {
-                const double s = fld.f[k * a.n + i];
-                const float sn = static_cast<float>(dadd(s, -dmul(dmul(2.0, ai), r[k])));
-                de += -a.J * (static_cast<double>(sn) - s) * h[k];
-                dm[k] = static_cast<double>(sn) - s;
-            }
-        }
-    }
-    // wave sums, then one atomic per quantity per block
-    __shared__ double part[4][6];
-    const std::uint32_t w = threadIdx.x >> 6;
-    const double vals[6] = {static_cast<double>(cnt), de, dm[0], dm[1], dm[2], dm[3]};
-    for (int k = 0; k < 2 + N; ++k) {
-        const double v = wave_sum(vals[k]);
-        if ((threadIdx.x & 63u) == 0) part[w][k] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < static_cast<unsigned>(2 + N)) {
-        const double v = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] + part[3][threadIdx.x];
-        if (threadIdx.x == 0 && v != 0.0) atomicAdd(&a.st->touched, static_cast<unsigned long long>(v));
-        if (v != 0.0) atomicAdd(&a.out[threadIdx.x], v);
-    }
+    const Field<N> fld = label_field<N>(a);
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x)
+        delta_site<N>(a, fld, r, root, i, [&](auto visit) {
+            for (std::uint32_t q = a.offsets[i]; q < a.offsets[i + 1]; ++q) visit(a.bonds[q]);
+        }, acc);
+    block_partials<N>(a, __ballot(acc[0] != 0.0) != 0, acc);
 }
 
 template <int N>
@@ -593,102 +632,94 @@ __global__ __launch_bounds__(kLabThreads) void box_local(LabelArgs a, Box<D> box
     double r[4];
     for (int k = 0; k < 4; ++k) r[k] = a.st->r[k];
     const Field<N> fld = label_field<N>(a);
-    auto next_active = [&](std::uint32_t i) {   // the +x edge, inside the tile
-        std::uint32_t c[3];
-        box.coords(i, c);
+    const BoxTile<D> blk(box, blockIdx.x);
+    using T = BoxTile<D>;
+    // local t -> coordinates without divisions (the tile extents are constants)
+    auto coords = [&](std::uint32_t t, std::uint32_t* c, std::uint32_t* l) {
+        l[0] = t % T::T[0];
+        l[1] = (t / T::T[0]) % T::T[1];
+        l[2] = t / (T::T[0] * T::T[1]);
+        for (int d = 0; d < 3; ++d) c[d] = blk.o[d] + l[d];
+    };
+    auto next_active = [&](std::uint32_t t) {   // the +x edge, inside the tile
+        std::uint32_t c[3], l[3];
+        coords(t, c, l);
+        const std::uint32_t i = box.index(c);
         return label_edge_active<N>(a, fld, r, fld.proj(i, r), box.edge(c, 0), i + 1u);
     };
-    auto inside_edges = [&](std::uint32_t i, const BoxTile<D>& blk, bool skip_next, auto unite) {
-        std::uint32_t c[3];
-        box.coords(i, c);
+    auto inside_edges = [&](std::uint32_t t, bool skip_next, auto unite) {
+        std::uint32_t c[3], l[3];
+        coords(t, c, l);
+        const std::uint32_t i = box.index(c);
         const double ai = fld.proj(i, r);
+        constexpr std::uint32_t stride[3] = {1u, T::T[0], T::T[0] * T::T[1]};
         for (int d = skip_next ? 1 : 0; d < D; ++d) {
+            if (l[d] + 1u == T::T[d]) continue;   // leaves the tile: level two
             const std::uint32_t j = box.step(c, d, 1);
-            if (blk.holds(j) && label_edge_active<N>(a, fld, r, ai, box.edge(c, d), j)) unite(j);
+            if (label_edge_active<N>(a, fld, r, ai, box.edge(c, d), j)) unite(t + stride[d]);
         }
     };
-    lab_block(a.parent, BoxTile<D>(box, blockIdx.x), next_active, inside_edges);
+    lab_block(a.parent, blk, next_active, inside_edges);
 }
 
-// Level two: the +d edges leaving a tile (c_d at the tile's last layer).
+// Level two: the +d edges leaving a tile, from the sites of the tiles' last
+// layer along d (blockIdx.y = d).  Consecutive lanes take sites next to each
+// other within that layer, whose crossing edges mostly join the same two
+// components, so lab_unite_once drops most of them.
 template <int N, int D>
 __global__ __launch_bounds__(256) void box_cross(LabelArgs a, Box<D> box)
 {
     if (a.st->skip) return;
-    const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = i < a.n;   // every lane takes part in each lab_unite_once
+    using T = BoxTile<D>;
+    const int d = static_cast<int>(blockIdx.y);
+    const std::uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const std::uint32_t faces = a.n / T::T[d];   // sites in the last layers along d
+    const bool valid = k < faces;                 // every lane takes part in each lab_unite_once
     double r[4];
-    for (int k = 0; k < 4; ++k) r[k] = a.st->r[k];
+    for (int q = 0; q < 4; ++q) r[q] = a.st->r[q];
     const Field<N> fld = label_field<N>(a);
+    // lanes run along the first axis other than d
+    const int u = d == 0 ? 1 : 0;
     std::uint32_t c[3] = {0, 0, 0};
-    if (valid) box.coords(i, c);
-    for (int d = 0; d < D; ++d) {
-        std::uint32_t j = 0;
-        bool act = false;
-        if (valid && c[d] % BoxTile<D>::T[d] == BoxTile<D>::T[d] - 1u) {
-            j = box.step(c, d, 1);
-            act = label_edge_active<N>(a, fld, r, fld.proj(i, r), box.edge(c, d), j);
-        }
-        lab_unite_once(a.parent, act, i, j);
+    if (valid) {
+        std::uint32_t rest = k;
+        c[u] = rest % box.L[u];
+        rest /= box.L[u];
+        const std::uint32_t nt = box.L[d] / T::T[d];
+        c[d] = rest % nt * T::T[d] + T::T[d] - 1u;
+        rest /= nt;
+        if constexpr (D == 3) c[3 - u - d] = rest;   // the remaining axis
     }
+    std::uint32_t i = 0, j = 0;
+    bool act = false;
+    if (valid) {
+        i = box.index(c);
+        j = box.step(c, d, 1);
+        act = label_edge_active<N>(a, fld, r, fld.proj(i, r), box.edge(c, d), j);
+    }
+    lab_unite_once(a.parent, act, i, j);
 }
 
 // label_delta with the 2D neighbours from the coordinates.
 template <int N, int D>
 __global__ __launch_bounds__(256) void box_delta(LabelArgs a, Box<D> box)
 {
-    const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (a.st->skip) return;
     double r[4];
     for (int k = 0; k < 4; ++k) r[k] = a.st->r[k];
     const std::uint32_t root = a.parent[a.st->seed];
     const Field<N> fld = label_field<N>(a);
-    unsigned long long cnt = 0;
-    double de = 0.0, dm[4] = {0.0, 0.0, 0.0, 0.0};
-    if (i < a.n && a.parent[i] == root) {
-        cnt = 1;
-        std::uint32_t c[3];
-        box.coords(i, c);
-        std::uint32_t nb[2 * D];
-        for (int d = 0; d < D; ++d) {   // the CSR row order does not matter to the sums below
-            nb[2 * d] = box.step(c, d, 1);
-            nb[2 * d + 1] = box.step(c, d, -1);
-        }
-        if constexpr (N == 1) {
-            const int si = fld.f[i];
-            int sum = 0;
-            for (int q = 0; q < 2 * D; ++q)
-                if (a.parent[nb[q]] != root) sum += fld.f[nb[q]];
-            de = 2.0 * a.J * static_cast<double>(si * sum);
-            dm[0] = -2.0 * si;
-        } else {
-            const double ai = fld.proj(i, r);
-            double h[N];
-            for (int k = 0; k < N; ++k) h[k] = 0.0;
-            for (int q = 0; q < 2 * D; ++q)
-                if (a.parent[nb[q]] != root)
-                    for (int k = 0; k < N; ++k) h[k] += fld.f[k * a.n + nb[q]];
-            for (int k = 0; k < N; ++k) {
-                const double s = fld.f[k * a.n + i];
-                const float sn = static_cast<float>(dadd(s, -dmul(dmul(2.0, ai), r[k])));
-                de += -a.J * (static_cast<double>(sn) - s) * h[k];
-                dm[k] = static_cast<double>(sn) - s;
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x)
+        delta_site<N>(a, fld, r, root, i, [&](auto visit) {
+            std::uint32_t c[3];
+            box.coords(i, c);
+            for (int d = 0; d < D; ++d) {   // the CSR row order does not matter to the sums
+                visit(box.step(c, d, 1));
+                visit(box.step(c, d, -1));
             }
-        }
-    }
-    __shared__ double part[4][6];
-    const std::uint32_t w = threadIdx.x >> 6;
-    const double vals[6] = {static_cast<double>(cnt), de, dm[0], dm[1], dm[2], dm[3]};
-    for (int k = 0; k < 2 + N; ++k) {
-        const double v = wave_sum(vals[k]);
-        if ((threadIdx.x & 63u) == 0) part[w][k] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < static_cast<unsigned>(2 + N)) {
-        const double v = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] + part[3][threadIdx.x];
-        if (threadIdx.x == 0 && v != 0.0) atomicAdd(&a.st->touched, static_cast<unsigned long long>(v));
-        if (v != 0.0) atomicAdd(&a.out[threadIdx.x], v);
-    }
+        }, acc);
+    block_partials<N>(a, __ballot(acc[0] != 0.0) != 0, acc);
 }
 
 template <int N, int D>
@@ -701,17 +732,22 @@ void box_flips(System& s, LabelArgs a, const ucmc_wolff_params& p)
     } else {
         box.L[0] = box.L[1] = box.L[2] = static_cast<std::uint32_t>(s.L);
     }
-    const dim3 grid((a.n + 255) / 256), tiles(a.n / kLabBlock);
+    const std::uint32_t blocks = (a.n + 255) / 256;
+    std::uint32_t face_max = 0;   // largest last-layer site count over the directions
+    for (int d = 0; d < D; ++d) face_max = std::max(face_max, a.n / BoxTile<D>::T[d]);
+    const dim3 grid(blocks), tiles(a.n / kLabBlock), faces((face_max + 255) / 256, D),
+        dgrid(std::min(blocks, kDeltaBlocks));
     for (std::uint64_t f = 0; f < p.nflips; ++f) {
         a.key = key_of(p.run_seed, kWolffCounterBase + p.flip0 + f);
         LabelArgs af = a;
         af.out = a.out + 6 * f;
         hipLaunchKernelGGL((label_begin<N>), dim3(1), dim3(64), 0, s.stream, af);
         hipLaunchKernelGGL((box_local<N, D>), tiles, dim3(kLabThreads), 0, s.stream, af, box);
-        hipLaunchKernelGGL((box_cross<N, D>), grid, dim3(256), 0, s.stream, af, box);
+        hipLaunchKernelGGL((box_cross<N, D>), faces, dim3(256), 0, s.stream, af, box);
         hipLaunchKernelGGL(lab_flatten, grid, dim3(256), 0, s.stream, af.parent, af.n,
                            static_cast<const ClusterState*>(af.st));
-        hipLaunchKernelGGL((box_delta<N, D>), grid, dim3(256), 0, s.stream, af, box);
+        hipLaunchKernelGGL((box_delta<N, D>), dgrid, dim3(256), 0, s.stream, af, box);
+        hipLaunchKernelGGL((label_reduce<N>), dim3(1), dim3(256), 0, s.stream, af, dgrid.x);
         hipLaunchKernelGGL((label_apply<N>), grid, dim3(256), 0, s.stream, af);
     }
     UCMC_HIP(hipGetLastError());
@@ -784,7 +820,8 @@ void label_flips(System& s, LabelArgs a, const ucmc_wolff_params& p)
 {
     if (s.box_dims == 2) return box_flips<N, 2>(s, a, p);
     if (s.box_dims == 3) return box_flips<N, 3>(s, a, p);
-    const dim3 grid((a.n + 255) / 256), blocks((a.n + kLabBlock - 1) / kLabBlock);
+    const dim3 grid((a.n + 255) / 256), blocks((a.n + kLabBlock - 1) / kLabBlock),
+        dgrid(std::min((a.n + 255) / 256, kDeltaBlocks));
     for (std::uint64_t f = 0; f < p.nflips; ++f) {
         a.key = key_of(p.run_seed, kWolffCounterBase + p.flip0 + f);
         LabelArgs af = a;
@@ -794,7 +831,8 @@ void label_flips(System& s, LabelArgs a, const ucmc_wolff_params& p)
         hipLaunchKernelGGL((label_cross<N>), grid, dim3(256), 0, s.stream, af);
         hipLaunchKernelGGL(lab_flatten, grid, dim3(256), 0, s.stream, af.parent, af.n,
                            static_cast<const ClusterState*>(af.st));
-        hipLaunchKernelGGL((label_delta<N>), grid, dim3(256), 0, s.stream, af);
+        hipLaunchKernelGGL((label_delta<N>), dgrid, dim3(256), 0, s.stream, af);
+        hipLaunchKernelGGL((label_reduce<N>), dim3(1), dim3(256), 0, s.stream, af, dgrid.x);
         hipLaunchKernelGGL((label_apply<N>), grid, dim3(256), 0, s.stream, af);
     }
     UCMC_HIP(hipGetLastError());
@@ -839,8 +877,10 @@ void wolff_run(System& s, const ucmc_wolff_params& p, std::uint64_t* sizes, ucmc
     if (labels) {
         if (!s.labels.p) s.labels.alloc(s.sites * 4);
         if (!s.cluster_state.p) s.cluster_state.alloc(sizeof(ClusterState));
+        if (!s.label_part.p) s.label_part.alloc(kDeltaBlocks * 6 * sizeof(double));
         UCMC_HIP(hipMemsetAsync(s.cluster_state.p, 0, sizeof(ClusterState), s.stream));
         LabelArgs la{};
+        la.part = s.label_part.as<double>();
         la.field = s.field[s.cur].p;
         la.n = static_cast<std::uint32_t>(s.sites);
         la.offsets = s.offsets.as<std::uint32_t>();
