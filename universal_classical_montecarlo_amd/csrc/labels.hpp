@@ -133,42 +133,62 @@ __device__ __forceinline__ LabSpan lab_span_of(std::uint32_t i, std::uint32_t n)
     return LabSpan{base, n - base < kLabBlock ? n - base : kLabBlock};
 }
 
-// Level one for this workgroup's block (blockDim.x a multiple of 64), with
-// local indices t:
-//   next_active(t): whether the edge (site(t), site(t) + 1) is activated (its
-//     first occurrence among the site's edges; false when there is none);
-//     called only where blk.run(t);
-//   inside_edges(t, skip_next, unite): calls unite(u) for every activated edge
-//     owned by site(t) whose other end is local u of the block, except that
-//     first (site(t), site(t) + 1) edge when skip_next.
+// What level one learns about local t's edges in one evaluation: whether the
+// edge (site(t), site(t) + 1) is activated (`run`, asked for only where it is
+// a run candidate), and a mask of its other activated internal edges, in the
+// caller's own numbering.
+struct LabAct {
+    bool run;
+    std::uint32_t mask;
+};
+
+// Level one for this workgroup's block (THREADS threads, a multiple of 64),
+// with local indices t:
+//   activated(t, run_ok) -> LabAct: evaluates each edge owned by site(t) with
+//     both ends in the block once; when run_ok, the first (site(t),
+//     site(t) + 1) edge goes into `run`, not into the mask;
+//   unites(t, mask, unite): calls unite(u) for the edges of `mask`, u the
+//     local index of the other end.
 // Runs of sites linked by (t, t + 1) edges inside a wave's 64 consecutive
 // local indices are labelled first, at once, from one ballot (each site points
 // at its run's first site); a union-find that united them pairwise would build
 // chains as long as the runs.  The remaining internal edges are then united in
 // LDS.
-template <class Block, class NextActive, class InsideEdges>
-__device__ __forceinline__ void lab_block(std::uint32_t* parent, const Block& blk, NextActive next_active,
-                                          InsideEdges inside_edges)
+template <std::uint32_t THREADS, class Block, class Activated, class Unites>
+__device__ __forceinline__ void lab_block(std::uint32_t* parent, const Block& blk, Activated activated,
+                                          Unites unites)
 {
+    static_assert(THREADS % 64 == 0 && kLabBlock % THREADS == 0, "whole waves, whole passes");
+    constexpr std::uint32_t R = kLabBlock / THREADS;
     __shared__ std::uint32_t lp[kLabBlock];
     const std::uint32_t lane = threadIdx.x & 63u;
-    for (std::uint32_t t0 = 0; t0 < blk.count; t0 += blockDim.x) {   // uniform trip count
-        const std::uint32_t t = t0 + threadIdx.x;
+    std::uint32_t masks[R];
+#pragma unroll
+    for (std::uint32_t r = 0; r < R; ++r) {
+        const std::uint32_t t = r * THREADS + threadIdx.x;
         const bool valid = t < blk.count;
-        const bool linked = valid && lane != 63u && blk.run(t) && next_active(t);
-        const unsigned long long m = __ballot(linked);
+        LabAct act{false, 0u};
+        if (valid) act = activated(t, lane != 63u && blk.run(t));
+        masks[r] = act.mask;
+        const unsigned long long m = __ballot(act.run);
         const unsigned long long starts = ~(m << 1);   // bit k: a run starts at lane k (bit 0 always)
         const unsigned long long upto = lane == 63u ? ~0ull : (2ull << lane) - 1ull;
         const std::uint32_t first = 63u - static_cast<std::uint32_t>(__clzll(starts & upto));
         if (valid) lp[t] = t - lane + first;
     }
     __syncthreads();
-    for (std::uint32_t t = threadIdx.x; t < blk.count; t += blockDim.x)
-        inside_edges(t, (t & 63u) != 63u && blk.run(t),
-                     [&](std::uint32_t u) { uf_unite<__HIP_MEMORY_SCOPE_WORKGROUP>(lp, t, u); });
+#pragma unroll
+    for (std::uint32_t r = 0; r < R; ++r) {
+        const std::uint32_t t = r * THREADS + threadIdx.x;
+        if (t < blk.count && masks[r])
+            unites(t, masks[r], [&](std::uint32_t u) { uf_unite<__HIP_MEMORY_SCOPE_WORKGROUP>(lp, t, u); });
+    }
     __syncthreads();
     // every unite has landed: plain walks to the roots
-    for (std::uint32_t t = threadIdx.x; t < blk.count; t += blockDim.x) {
+#pragma unroll
+    for (std::uint32_t r = 0; r < R; ++r) {
+        const std::uint32_t t = r * THREADS + threadIdx.x;
+        if (t >= blk.count) continue;
         std::uint32_t x = t;
         for (std::uint32_t q = lp[x]; q != x; q = lp[x]) x = q;
         parent[blk.site(t)] = blk.site(x);

@@ -571,22 +571,27 @@ __global__ __launch_bounds__(1024) void slmc_label_local(SlmcLabelArgs a)
         for (int k = 0; k < 8; ++k) a.st->count[k] = 0;
     }
     const LabSpan span = lab_span_of(blockIdx.x * kLabBlock, a.n);
-    // direction 0 is +x: the edge (i, i + 1) unless x = L - 1
-    auto next_active = [&](std::uint32_t t) {
-        const std::uint32_t i = span.site(t);
-        return i % a.L != a.L - 1u && slmc_edge_active(a, i, a.s[i], 0, i + 1u);
-    };
-    auto inside_edges = [&](std::uint32_t t, bool skip_next, auto unite) {
+    // mask bit d: forward direction d (direction 0 is +x: the edge (i, i + 1)
+    // unless x = L - 1)
+    auto activated = [&](std::uint32_t t, bool run_ok) {
         const std::uint32_t i = span.site(t);
         const int si = a.s[i];
+        LabAct act{false, 0u};
         for (int d = 0; d < 12; ++d) {
             if (!kForward[d]) continue;
             const std::uint32_t j = nbr(a.L, i, d);
-            if (!span.holds(j) || (d == 0 && skip_next && j == i + 1u)) continue;
-            if (slmc_edge_active(a, i, si, d, j)) unite(span.local(j));
+            if (!span.holds(j) || !slmc_edge_active(a, i, si, d, j)) continue;
+            if (d == 0 && run_ok && j == i + 1u) act.run = true;
+            else act.mask |= 1u << d;
         }
+        return act;
     };
-    lab_block(a.parent, span, next_active, inside_edges);
+    auto unites = [&](std::uint32_t t, std::uint32_t mask, auto unite) {
+        const std::uint32_t i = span.site(t);
+        for (int d = 0; d < 12; ++d)
+            if (mask >> d & 1u) unite(span.local(nbr(a.L, i, d)));
+    };
+    lab_block<1024>(a.parent, span, activated, unites);
 }
 
 // Level two: the activated edges between blocks.

@@ -346,27 +346,48 @@ __global__ __launch_bounds__(kLabThreads) void label_local(LabelArgs a)
     for (int k = 0; k < 4; ++k) r[k] = a.st->r[k];
     const Field<N> fld = label_field<N>(a);
     const LabSpan span = lab_span_of(blockIdx.x * kLabBlock, a.n);
-    auto next_active = [&](std::uint32_t t) {
-        const std::uint32_t i = span.site(t);
-        for (std::uint32_t q = a.offsets[i]; q < a.offsets[i + 1]; ++q)
-            if (a.bonds[q] == i + 1u) return label_edge_active<N>(a, fld, r, fld.proj(i, r), a.edge_id[q], i + 1u);
-        return false;
-    };
-    auto inside_edges = [&](std::uint32_t t, bool skip_next, auto unite) {
+    // mask bit k: the edge at position k of the site's CSR row (rows longer
+    // than 31 test their later edges when uniting; bit 31 flags them)
+    auto activated = [&](std::uint32_t t, bool run_ok) {
         const std::uint32_t i = span.site(t);
         const double ai = fld.proj(i, r);
-        bool skip = skip_next;
-        for (std::uint32_t q = a.offsets[i]; q < a.offsets[i + 1]; ++q) {
+        const std::uint32_t q0 = a.offsets[i], q1 = a.offsets[i + 1];
+        LabAct act{false, 0u};
+        bool run_seen = !run_ok;
+        for (std::uint32_t q = q0; q < q1 && q < q0 + 31u; ++q) {
             const std::uint32_t j = a.bonds[q];
             if (j <= i || !span.holds(j)) continue;
-            if (skip && j == i + 1u) {   // its first occurrence was labelled by the run pass
-                skip = false;
+            const bool on = label_edge_active<N>(a, fld, r, ai, a.edge_id[q], j);
+            if (!run_seen && j == i + 1u) {   // the first (i, i + 1) edge
+                run_seen = true;
+                act.run = on;
+            } else if (on) {
+                act.mask |= 1u << (q - q0);
+            }
+        }
+        if (q1 > q0 + 31u) act.mask |= 0x80000000u;   // the tail is looked at when uniting
+        return act;
+    };
+    auto unites = [&](std::uint32_t t, std::uint32_t mask, auto unite) {
+        const std::uint32_t i = span.site(t);
+        const std::uint32_t q0 = a.offsets[i], q1 = a.offsets[i + 1];
+        for (std::uint32_t q = q0; q < q1 && q < q0 + 31u; ++q)
+            if (mask >> (q - q0) & 1u) unite(span.local(a.bonds[q]));
+        if (q1 <= q0 + 31u) return;
+        const double ai = fld.proj(i, r);
+        bool run_seen = !((t & 63u) != 63u && span.run(t));
+        for (std::uint32_t q = q0; q < q0 + 31u; ++q) run_seen = run_seen || a.bonds[q] == i + 1u;
+        for (std::uint32_t q = q0 + 31u; q < q1; ++q) {
+            const std::uint32_t j = a.bonds[q];
+            if (j <= i || !span.holds(j)) continue;
+            if (!run_seen && j == i + 1u) {   // labelled by the run pass
+                run_seen = true;
                 continue;
             }
             if (label_edge_active<N>(a, fld, r, ai, a.edge_id[q], j)) unite(span.local(j));
         }
     };
-    lab_block(a.parent, span, next_active, inside_edges);
+    lab_block<kLabThreads>(a.parent, span, activated, unites);
 }
 
 // Level two: the activated edges between blocks, the wave's lanes stepping
@@ -634,32 +655,35 @@ __global__ __launch_bounds__(kLabThreads) void box_local(LabelArgs a, Box<D> box
     const Field<N> fld = label_field<N>(a);
     const BoxTile<D> blk(box, blockIdx.x);
     using T = BoxTile<D>;
-    // local t -> coordinates without divisions (the tile extents are constants)
-    auto coords = [&](std::uint32_t t, std::uint32_t* c, std::uint32_t* l) {
-        l[0] = t % T::T[0];
-        l[1] = (t / T::T[0]) % T::T[1];
-        l[2] = t / (T::T[0] * T::T[1]);
-        for (int d = 0; d < 3; ++d) c[d] = blk.o[d] + l[d];
-    };
-    auto next_active = [&](std::uint32_t t) {   // the +x edge, inside the tile
-        std::uint32_t c[3], l[3];
-        coords(t, c, l);
-        const std::uint32_t i = box.index(c);
-        return label_edge_active<N>(a, fld, r, fld.proj(i, r), box.edge(c, 0), i + 1u);
-    };
-    auto inside_edges = [&](std::uint32_t t, bool skip_next, auto unite) {
-        std::uint32_t c[3], l[3];
-        coords(t, c, l);
+    constexpr std::uint32_t stride[3] = {1u, T::T[0], T::T[0] * T::T[1]};
+    // mask bit d: the +d edge (inside the tile); local t -> coordinates by
+    // constant divisors
+    auto activated = [&](std::uint32_t t, bool run_ok) {
+        const std::uint32_t l[3] = {t % T::T[0], (t / T::T[0]) % T::T[1], t / (T::T[0] * T::T[1])};
+        const std::uint32_t c[3] = {blk.o[0] + l[0], blk.o[1] + l[1], blk.o[2] + l[2]};
         const std::uint32_t i = box.index(c);
         const double ai = fld.proj(i, r);
-        constexpr std::uint32_t stride[3] = {1u, T::T[0], T::T[0] * T::T[1]};
-        for (int d = skip_next ? 1 : 0; d < D; ++d) {
-            if (l[d] + 1u == T::T[d]) continue;   // leaves the tile: level two
-            const std::uint32_t j = box.step(c, d, 1);
-            if (label_edge_active<N>(a, fld, r, ai, box.edge(c, d), j)) unite(t + stride[d]);
+        // the +d edges of c that stay inside the tile are c's own (c_d < L_d - 1):
+        // ids prefix + (number of own +d' edges before d)
+        const std::uint32_t e0 = box.prefix(c);
+        std::uint32_t rank = 0, gstride = 1;
+        LabAct act{false, 0u};
+        for (int d = 0; d < D; ++d) {
+            if (l[d] + 1u < T::T[d]) {
+                const bool on = label_edge_active<N>(a, fld, r, ai, e0 + rank, i + gstride);
+                if (d == 0 && run_ok) act.run = on;
+                else if (on) act.mask |= 1u << d;
+            }
+            rank += c[d] + 1u < box.L[d] ? 1u : 0u;
+            gstride *= box.L[d];
         }
+        return act;
     };
-    lab_block(a.parent, blk, next_active, inside_edges);
+    auto unites = [&](std::uint32_t t, std::uint32_t mask, auto unite) {
+        for (int d = 0; d < D; ++d)
+            if (mask >> d & 1u) unite(t + stride[d]);
+    };
+    lab_block<kLabThreads>(a.parent, blk, activated, unites);
 }
 
 // Level two: the +d edges leaving a tile, from the sites of the tiles' last
