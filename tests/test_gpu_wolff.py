@@ -23,7 +23,8 @@ PATHS = [abi.WOLFF_BFS, abi.WOLFF_LABELS]
 
 
 @pytest.mark.parametrize("path", PATHS)
-@pytest.mark.parametrize("L,beta,J", [(16, 0.44, 1), (8, 0.3, 1), (32, 0.2, -1), (64, 0.45, 2)])
+@pytest.mark.parametrize("L,beta,J", [(16, 0.44, 1), (8, 0.3, 1), (32, 0.2, -1), (64, 0.45, 2),
+                                      (192, 0.44, 1), (192, 0.3, -1)])
 def test_ising_wolff_bit_exact(gpu, L, beta, J, path):
     print(f"ising wolff L={L} beta={beta} path={path}", flush=True)
     off, bonds = csr_square(L, L)

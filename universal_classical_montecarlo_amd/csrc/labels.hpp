@@ -196,14 +196,17 @@ __device__ __forceinline__ void lab_block(std::uint32_t* parent, const Block& bl
 }
 
 // Level two, called by every lane of the wave at once: unites i and j where
-// `active`, except where the previous lane unites the same two roots in this
-// call (edges between two blocks mostly join the same pair of components).
+// `active`, except where the previous lane's pair has the same two current
+// parents (then it is in the same two components, and that lane unites them):
+// edges between two blocks mostly join the same pair of components.  The
+// parents are read through the caches first; any value read, stale or not, is
+// an ancestor, so equal parents always mean one component.
 __device__ __forceinline__ void lab_unite_once(std::uint32_t* parent, bool active, std::uint32_t i, std::uint32_t j)
 {
     std::uint32_t a = 0, b = 0;
     if (active) {
-        a = lab_find(parent, i);
-        b = lab_find(parent, j);
+        a = uf_load<__HIP_MEMORY_SCOPE_WORKGROUP>(parent + i);
+        b = uf_load<__HIP_MEMORY_SCOPE_WORKGROUP>(parent + j);
         if (a > b) {
             const std::uint32_t t = a;
             a = b;
@@ -213,13 +216,20 @@ __device__ __forceinline__ void lab_unite_once(std::uint32_t* parent, bool activ
     const std::uint32_t pa = __shfl_up(a, 1, 64), pb = __shfl_up(b, 1, 64);
     const int pact = __shfl_up(active ? 1 : 0, 1, 64);
     const bool dup = (threadIdx.x & 63u) != 0u && pact && pa == a && pb == b;
-    if (active && a != b && !dup) {
-        // a < b were roots when found: try the hook at once, else the full unite
-        std::uint32_t expected = b;
-        if (!__hip_atomic_compare_exchange_strong(parent + b, &expected, a, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_AGENT))
-            lab_unite(parent, expected, a);
+    if (!active || a == b || dup) return;
+    a = lab_find(parent, a);
+    b = lab_find(parent, b);
+    if (a == b) return;
+    if (a > b) {
+        const std::uint32_t t = a;
+        a = b;
+        b = t;
     }
+    // a < b were roots when found: try the hook at once, else the full unite
+    std::uint32_t expected = b;
+    if (!__hip_atomic_compare_exchange_strong(parent + b, &expected, a, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT))
+        lab_unite(parent, expected, a);
 }
 
 namespace {
