@@ -8,11 +8,11 @@
 //  * a single system on any PairGraph: one launch per colour class;
 //  * a batch of replicas (parallel tempering): every replica is a periodic
 //    L x L square lattice; ONE launch per sweep updates both checkerboard
-//    colours of every replica tile by tile in LDS (tile + 3-site halo, colour 0
-//    on the tile and two rings, colour 1 on the tile and one ring, so the final
-//    values of the tile and its first ring give the tile's bond energies), and
-//    the last workgroup to finish performs the replica-exchange step for all
-//    ladders (no grid barrier).  Fields ping-pong between two buffers, so halos
+//    colours of every replica tile by tile in LDS (cos and sin of the angles
+//    of the tile + 3-site halo; colour 0 on the tile and two rings, colour 1
+//    on the tile and one ring, so the final values of the tile and its first
+//    ring give the tile's bond energies), and the last workgroup to finish
+//    performs the replica-exchange step for all ladders (no grid barrier).  Fields ping-pong between two buffers, so halos
 //    are always read unmodified; the counter-based per-site stream makes the
 //    redundant halo updates bit-identical to the owner's.
 #include <hip/hip_runtime.h>
@@ -145,17 +145,17 @@ struct PtArgs {
     std::uint64_t swap_seed;
 };
 
-__device__ __forceinline__ float xy_dE(float J, float tp, float ti, const float* nb)
-{
-    float d = 0.f;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) d += cosf(tp - nb[q]) - cosf(ti - nb[q]);
-    return -J * d;
-}
-
+// One sweep of every replica, tile by tile.  LDS holds cos t and sin t of the
+// tile and its halo; the energy change of t_i -> p against the neighbours'
+// sums C = sum_q cos t_q, S = sum_q sin t_q is -J [(cos p - cos t_i) C +
+// (sin p - sin t_i) S], the reference's sum_q [cos(p - t_q) - cos(t_i - t_q)]
+// by the angle-addition formula (one sincos per proposal instead of eight
+// cosines).  Each site is updated once per sweep, so its current angle is the
+// input buffer's; tile sites write their new angle to the output directly.
+// Dynamic LDS: 2 (T_max + 2 kHalo)^2 floats.
 __global__ __launch_bounds__(256) void pt_sweep(PtArgs a)
 {
-    __shared__ float tl[(kMaxTile + 2 * kHalo) * (kMaxTile + 2 * kHalo)];
+    extern __shared__ float pt_lds[];
     __shared__ double s_e;
     __shared__ unsigned long long s_acc;
     __shared__ bool s_last;
@@ -163,6 +163,8 @@ __global__ __launch_bounds__(256) void pt_sweep(PtArgs a)
     const std::uint32_t r = a.tile_rep[tile];
     const PtReplica rp = a.reps[r];
     const std::uint32_t L = rp.L, T = rp.tile, W = T + 2 * kHalo;
+    float* cs = pt_lds;
+    float* sn = pt_lds + W * W;
     const std::uint32_t per_row = L / T;
     const std::uint32_t local = tile - rp.first_tile;
     const std::uint32_t ty0 = (local / per_row) * T, tx0 = (local % per_row) * T;
@@ -175,46 +177,60 @@ __global__ __launch_bounds__(256) void pt_sweep(PtArgs a)
         s_e = 0.0;
         s_acc = 0;
     }
-    // load tile + halo (periodic)
+    // window row / column -> lattice row / column (periodic)
+    auto wy = [&](std::uint32_t ly) { return (ty0 + L - kHalo + ly) % L; };
+    auto wx = [&](std::uint32_t lx) { return (tx0 + L - kHalo + lx) % L; };
     for (std::uint32_t q = threadIdx.x; q < W * W; q += blockDim.x) {
         const std::uint32_t ly = q / W, lx = q % W;
-        const std::uint32_t y = (ty0 + L - kHalo + ly) % L, x = (tx0 + L - kHalo + lx) % L;
-        tl[q] = in[static_cast<std::size_t>(y) * L + x];
+        float sv, cv;
+        sincosf(in[static_cast<std::size_t>(wy(ly)) * L + wx(lx)], &sv, &cv);
+        cs[q] = cv;
+        sn[q] = sv;
     }
     __syncthreads();
-    // colour c over the window [lo, hi) of local coordinates
+    unsigned long long nacc = 0;
+    // colour c over the window [lo, hi)^2 (hi - lo even): the site parity is
+    // that of lx + ly (tile origins and L are even), so row ly holds colour c
+    // at lx = lo + 2k + ((c + lo + ly) & 1)
     auto pass = [&](std::uint32_t colour, std::uint32_t lo, std::uint32_t hi) {
-        const std::uint32_t w = hi - lo;
-        for (std::uint32_t q = threadIdx.x; q < w * w; q += blockDim.x) {
-            const std::uint32_t ly = lo + q / w, lx = lo + q % w;
-            const std::uint32_t y = (ty0 + L - kHalo + ly) % L, x = (tx0 + L - kHalo + lx) % L;
-            if (((x + y) & 1u) != colour) continue;
-            const float ti = tl[ly * W + lx];
-            const float nb[4] = {tl[ly * W + lx + 1], tl[ly * W + lx - 1], tl[(ly + 1) * W + lx],
-                                 tl[(ly - 1) * W + lx]};
+        const std::uint32_t half = (hi - lo) / 2;
+        for (std::uint32_t q = threadIdx.x; q < (hi - lo) * half; q += blockDim.x) {
+            const std::uint32_t ly = lo + q / half;
+            const std::uint32_t lx = lo + 2 * (q % half) + ((colour + lo + ly) & 1u);
+            const std::uint32_t y = wy(ly), x = wx(lx), w = ly * W + lx;
+            const std::size_t gi = static_cast<std::size_t>(y) * L + x;
+            const float ti = in[gi];
+            const float C = cs[w + 1] + cs[w - 1] + cs[w + W] + cs[w - W];
+            const float S = sn[w + 1] + sn[w - 1] + sn[w + W] + sn[w - W];
             Stream rg;
-            rg.init(key, static_cast<std::uint64_t>(y) * L + x, kTagSite);
+            rg.init(key, gi, kTagSite);
             const float tp = u1_shift(a.width, ti, rg.next());
-            const float dE = xy_dE(a.J, tp, ti, nb);
+            float sp, cp;
+            sincosf(tp, &sp, &cp);
+            const float dE = -a.J * ((cp - cs[w]) * C + (sp - sn[w]) * S);
             const double log_p = -beta * static_cast<double>(dE) + 0.0;
-            if (log_p >= 0.0 || accept_draw(rg.next(), log_p)) tl[ly * W + lx] = tp;
+            const bool acc = log_p >= 0.0 || accept_draw(rg.next(), log_p);
+            if (acc) {
+                cs[w] = cp;
+                sn[w] = sp;
+            }
+            if (ly - kHalo < T && lx - kHalo < T) {   // a tile site: its final value
+                out[gi] = acc ? tp : ti;
+                nacc += acc ? 1u : 0u;
+            }
         }
     };
     pass(0, 1, W - 1);  // colour 0 on tile + 2 rings
     __syncthreads();
     pass(1, 2, W - 2);  // colour 1 on tile + 1 ring
     __syncthreads();
-    // write the tile; count its changed sites; bonds to the right and down
-    // neighbours (both final within tile + 1 ring)
-    unsigned long long nacc = 0;
+    // bonds to the right and down neighbours (final within tile + 1 ring):
+    // cos(t - t') = cos t cos t' + sin t sin t'
     double e = 0.0;
     for (std::uint32_t q = threadIdx.x; q < T * T; q += blockDim.x) {
-        const std::uint32_t ly = kHalo + q / T, lx = kHalo + q % T;
-        const std::uint32_t y = ty0 + q / T, x = tx0 + q % T;
-        const float v = tl[ly * W + lx];
-        out[static_cast<std::size_t>(y) * L + x] = v;
-        nacc += (v != in[static_cast<std::size_t>(y) * L + x]) ? 1u : 0u;
-        e += cos(static_cast<double>(v) - tl[ly * W + lx + 1]) + cos(static_cast<double>(v) - tl[(ly + 1) * W + lx]);
+        const std::uint32_t w = (kHalo + q / T) * W + kHalo + q % T;
+        e += static_cast<double>(cs[w]) * cs[w + 1] + static_cast<double>(sn[w]) * sn[w + 1] +
+             static_cast<double>(cs[w]) * cs[w + W] + static_cast<double>(sn[w]) * sn[w + W];
     }
     e = wave_sum(e);
     nacc = wave_sum_u64(nacc);
@@ -433,6 +449,7 @@ void pt_layout(System& s, std::uint32_t tile, std::uint64_t run_seed)
     const std::size_t R = pt.L.size();
     std::vector<PtReplica> reps(R);
     pt.tile_rep_h.clear();
+    pt.max_tile = 0;
     for (std::size_t r = 0; r < R; ++r) {
         const std::uint32_t L = pt.L[r];
         std::uint32_t T = std::min(tile, L);
@@ -440,6 +457,7 @@ void pt_layout(System& s, std::uint32_t tile, std::uint64_t run_seed)
         reps[r].offset = pt.offset[r];
         reps[r].L = L;
         reps[r].tile = T;
+        pt.max_tile = std::max(pt.max_tile, T);
         reps[r].first_tile = static_cast<std::uint32_t>(pt.tile_rep_h.size());
         reps[r].ladder = pt.ladder[r];
         reps[r].run_seed = key_of(run_seed, (1ull << 61) + r);
@@ -488,7 +506,8 @@ void pt_launch(System& s, PtArgs a)
 {
     a.in = s.field[s.cur].as<float>();
     a.out = s.field[s.cur ^ 1].as<float>();
-    hipLaunchKernelGGL(pt_sweep, dim3(s.pt.ntiles), dim3(256), 0, s.stream, a);
+    const std::uint32_t W = s.pt.max_tile + 2 * kHalo;
+    hipLaunchKernelGGL(pt_sweep, dim3(s.pt.ntiles), dim3(256), 2 * W * W * sizeof(float), s.stream, a);
     UCMC_HIP(hipGetLastError());
     s.cur ^= 1;
 }
