@@ -104,7 +104,7 @@ struct System {
         std::uint32_t max_tile = 0;   // largest tile edge in the layout (sizes the LDS)
         std::uint32_t ntiles = 0;
         DevBuf reps, tile_rep, betas_d, temp_of, rep_at, ladder_start, energy, accepts, swap_acc,
-            swap_try, done, sweep_ctr;
+            swap_try, sweep_ctr;
     } pt;
 
     ~System();

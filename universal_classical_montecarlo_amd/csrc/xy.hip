@@ -11,8 +11,8 @@
 //    colours of every replica tile by tile in LDS (cos and sin of the angles
 //    of the tile + 3-site halo; colour 0 on the tile and two rings, colour 1
 //    on the tile and one ring, so the final values of the tile and its first
-//    ring give the tile's bond energies), and the last workgroup to finish
-//    performs the replica-exchange step for all ladders (no grid barrier).  Fields ping-pong between two buffers, so halos
+//    ring give the tile's bond energies), then a one-workgroup launch performs
+//    the replica-exchange step for all ladders.  Fields ping-pong between two buffers, so halos
 //    are always read unmodified; the counter-based per-site stream makes the
 //    redundant halo updates bit-identical to the owner's.
 #include <hip/hip_runtime.h>
@@ -136,7 +136,6 @@ struct PtArgs {
     unsigned long long* accepts;     // per replica, cumulative
     unsigned long long* swap_acc;    // per slot pair (ladder-major), cumulative
     unsigned long long* swap_try;
-    std::uint32_t* done;             // finished-block counter
     unsigned long long* sweep_ctr;   // device sweep counter (graph-replayable)
     double* energy_series;           // [sweep][replica] or null
     std::uint32_t series_cap;
@@ -158,7 +157,6 @@ __global__ __launch_bounds__(256) void pt_sweep(PtArgs a)
     extern __shared__ float pt_lds[];
     __shared__ double s_e;
     __shared__ unsigned long long s_acc;
-    __shared__ bool s_last;
     const std::uint32_t tile = blockIdx.x;
     const std::uint32_t r = a.tile_rep[tile];
     const PtReplica rp = a.reps[r];
@@ -177,9 +175,14 @@ __global__ __launch_bounds__(256) void pt_sweep(PtArgs a)
         s_e = 0.0;
         s_acc = 0;
     }
-    // window row / column -> lattice row / column (periodic)
-    auto wy = [&](std::uint32_t ly) { return (ty0 + L - kHalo + ly) % L; };
-    auto wx = [&](std::uint32_t lx) { return (tx0 + L - kHalo + lx) % L; };
+    // window row / column -> lattice row / column (periodic; the window
+    // reaches kHalo past either edge, one wrap when L >= kHalo)
+    auto wrap = [&](std::uint32_t v) {
+        if (L < kHalo) return (v + L * kHalo - kHalo) % L;
+        return v < kHalo ? v + L - kHalo : (v - kHalo >= L ? v - kHalo - L : v - kHalo);
+    };
+    auto wy = [&](std::uint32_t ly) { return wrap(ty0 + ly); };
+    auto wx = [&](std::uint32_t lx) { return wrap(tx0 + lx); };
     for (std::uint32_t q = threadIdx.x; q < W * W; q += blockDim.x) {
         const std::uint32_t ly = q / W, lx = q % W;
         float sv, cv;
@@ -194,9 +197,18 @@ __global__ __launch_bounds__(256) void pt_sweep(PtArgs a)
     // at lx = lo + 2k + ((c + lo + ly) & 1)
     auto pass = [&](std::uint32_t colour, std::uint32_t lo, std::uint32_t hi) {
         const std::uint32_t half = (hi - lo) / 2;
+        // (row, k) of position q, stepped by blockDim.x without dividing
+        std::uint32_t row = threadIdx.x / half, k = threadIdx.x - row * half;
+        const std::uint32_t drow = blockDim.x / half, dk = blockDim.x - drow * half;
         for (std::uint32_t q = threadIdx.x; q < (hi - lo) * half; q += blockDim.x) {
-            const std::uint32_t ly = lo + q / half;
-            const std::uint32_t lx = lo + 2 * (q % half) + ((colour + lo + ly) & 1u);
+            const std::uint32_t ly = lo + row;
+            const std::uint32_t lx = lo + 2 * k + ((colour + lo + ly) & 1u);
+            row += drow;
+            k += dk;
+            if (k >= half) {
+                k -= half;
+                ++row;
+            }
             const std::uint32_t y = wy(ly), x = wx(lx), w = ly * W + lx;
             const std::size_t gi = static_cast<std::size_t>(y) * L + x;
             const float ti = in[gi];
@@ -242,15 +254,16 @@ __global__ __launch_bounds__(256) void pt_sweep(PtArgs a)
     if (threadIdx.x == 0) {
         atomicAdd(&a.energy[r], -static_cast<double>(a.J) * s_e);
         atomicAdd(&a.accepts[r], s_acc);
-        __threadfence();
-        s_last = atomicAdd(a.done, 1u) == a.ntiles - 1;
     }
-    __syncthreads();
-    if (!s_last) return;
+}
 
-    // ---- the last block: record energies, then replica exchange on every
-    // ladder, pairs (k, k+1) with k of the sweep's parity ----
-    __threadfence();
+// After the sweep (its own launch, so every tile's energy has landed; a
+// last-block handoff would need a device-wide release fence in every
+// workgroup): record the energies, then replica exchange on every ladder,
+// pairs (k, k+1) with k of the sweep's parity.  One workgroup.
+__global__ __launch_bounds__(256) void pt_exchange(PtArgs a)
+{
+    const std::uint64_t sweep = *a.sweep_ctr;
     for (std::uint32_t q = threadIdx.x; q < a.nreps; q += blockDim.x) {
         const double ev = __hip_atomic_load(&a.energy[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (a.energy_series && sweep >= a.sweep_base && sweep - a.sweep_base < a.series_cap)
@@ -282,11 +295,7 @@ __global__ __launch_bounds__(256) void pt_sweep(PtArgs a)
     __syncthreads();
     for (std::uint32_t q = threadIdx.x; q < a.nreps; q += blockDim.x) a.energy[q] = 0.0;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        *a.done = 0;
-        *a.sweep_ctr = sweep + 1;
-    }
-    __threadfence();
+    if (threadIdx.x == 0) *a.sweep_ctr = sweep + 1;
 }
 
 // the ring / halo width needs tile + 2 kHalo <= kMaxTile + 2 kHalo
@@ -489,7 +498,6 @@ PtArgs pt_args(System& s, const ucmc_pt_params& p, double* series_d, std::uint32
     a.accepts = pt.accepts.as<unsigned long long>();
     a.swap_acc = pt.swap_acc.as<unsigned long long>();
     a.swap_try = pt.swap_try.as<unsigned long long>();
-    a.done = pt.done.as<std::uint32_t>();
     a.sweep_ctr = pt.sweep_ctr.as<unsigned long long>();
     a.energy_series = series_d;
     a.series_cap = cap;
@@ -508,6 +516,7 @@ void pt_launch(System& s, PtArgs a)
     a.out = s.field[s.cur ^ 1].as<float>();
     const std::uint32_t W = s.pt.max_tile + 2 * kHalo;
     hipLaunchKernelGGL(pt_sweep, dim3(s.pt.ntiles), dim3(256), 2 * W * W * sizeof(float), s.stream, a);
+    hipLaunchKernelGGL(pt_exchange, dim3(1), dim3(256), 0, s.stream, a);
     UCMC_HIP(hipGetLastError());
     s.cur ^= 1;
 }
@@ -560,8 +569,6 @@ void pt_setup(System& s, std::size_t R, const std::size_t* L, const std::uint32_
     pt.swap_acc.alloc(R * 8);
     pt.swap_try.alloc(R * 8);
     for (DevBuf* b : {&pt.energy, &pt.accepts, &pt.swap_acc, &pt.swap_try}) UCMC_HIP(hipMemset(b->p, 0, b->bytes));
-    pt.done.alloc(4);
-    UCMC_HIP(hipMemset(pt.done.p, 0, 4));
     pt.sweep_ctr.alloc(8);
     pt_layout(s, kDefaultTile, 0);
 }
