@@ -523,7 +523,8 @@ __global__ __launch_bounds__(1024) void slmc_flips(SlmcArgs a)
                 const std::uint32_t i = a.queue[idx];
                 a.s[i] = static_cast<std::int8_t>(-a.s[i]);
             }
-        __threadfence();
+        // one workgroup: the barrier's workgroup-scope fence publishes the
+        // flipped spins to the next step (no device-wide fence needed)
         __syncthreads();
     }
 }

@@ -257,7 +257,8 @@ __global__ __launch_bounds__(kThreads) void wolff_flips(WolffArgs a)
             o[1] = s_red[0];
             for (int k = 0; k < 4; ++k) o[2 + k] = k < N ? s_red[1 + k] : 0.0;
         }
-        __threadfence();
+        // one workgroup: the barrier's workgroup-scope fence publishes the
+        // flipped field to the next flip (no device-wide fence needed)
         __syncthreads();
     }
 }
