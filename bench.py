@@ -25,21 +25,6 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_SITE = 2.0  # int8 read + int8 write per site-update
 
 
-def square_csr(L):
-    n = L * L
-    idx = np.arange(n, dtype=np.int64)
-    x, y = idx % L, idx // L
-    right = y * L + (x + 1) % L
-    down = ((y + 1) % L) * L + x
-    left = y * L + (x - 1) % L
-    up = ((y - 1) % L) * L + x
-    # row order of square_lattice (lattice/square.hpp:30-38): right, down, then the
-    # reverse entries of the left and upper neighbours' edges
-    bonds = np.stack([right, down, left, up], axis=1).astype(np.uint32).reshape(-1)
-    off = (np.arange(n + 1, dtype=np.int64) * 4).astype(np.uint32)
-    return off, bonds
-
-
 def random_spins(n, seed):
     from oracle_lib import random_spins as rs
     return rs(n, seed, 0)
@@ -122,6 +107,18 @@ def suite(quick=False):
     out["heisenberg_128_wolff"] = {"flipped_sites_per_s": float(sizes.sum()) / dt,
                                    "flips_per_s": 500 / dt, "mean_cluster": float(sizes.mean())}
     v.close()
+    # Wolff on the 2D Ising lattice of the headline (tiled label path), beta = 0.44
+    L = 1024 if quick else 4096
+    w = abi.System.ising_square(L, L, 1)
+    w.upload(random_spins(L * L, 7))
+    w.metropolis(0.44, 200, 5)
+    w.wolff(0.44, 20, 6)
+    t0 = time.perf_counter()
+    sizes, _ = w.wolff(0.44, 100, 6, flip0=20)
+    dt = time.perf_counter() - t0
+    out[f"ising_{L}_wolff"] = {"flipped_sites_per_s": float(sizes.sum()) / dt, "flips_per_s": 100 / dt,
+                               "mean_cluster": float(sizes.mean())}
+    w.close()
     # C3: SU(3) 24^4 (16^4 quick), beta = 5.7 set once (model), SUShift 0.2, 4 hits
     for L in ((16,) if quick else (16, 24)):
         g = abi.System.sun_gauge(3, L, 5.7)
@@ -162,6 +159,46 @@ def suite(quick=False):
     return out
 
 
+def profile_others():
+    """Each of the other configurations at full size for a few steps, for
+    rocprofv3 --kernel-trace (kernel durations of K2-K9)."""
+    from universal_classical_montecarlo_amd import abi, slmc
+    from oracle_lib import csr_cubic
+    L = 128
+    off, bonds = csr_cubic(L)
+    v = abi.System.vector(3, off, bonds, 1.0)
+    x = np.random.default_rng(5).normal(size=(L ** 3, 3))
+    x /= np.linalg.norm(x, axis=1, keepdims=True)
+    v.upload(x.reshape(-1))
+    v.metropolis(0.693, 20, 1, proposal=abi.PROP_ROTATION, width=0.5)
+    v.wolff(0.693, 50, 3)
+    v.close()
+    w = abi.System.ising_square(4096, 4096, 1)
+    w.upload(random_spins(4096 * 4096, 7))
+    w.metropolis(0.44, 20, 5)
+    w.wolff(0.44, 20, 6)
+    w.close()
+    g = abi.System.sun_gauge(3, 24, 5.7)
+    g.metropolis(1.0, 2, 1, proposal=abi.PROP_SU_SHIFT, width=0.2, hits=4)
+    g.reunitarize()
+    g.close()
+    Ls, ladders, betas = [], [], []
+    grid = 0.8 + 0.6 * np.arange(64) / 63
+    for ld, (Lr, count) in enumerate([(32, 64), (64, 64), (128, 64), (256, 32), (512, 32)]):
+        Ls += [Lr] * count
+        ladders += [ld] * count
+        betas += list(grid if count == 64 else grid[::2])
+    pt = abi.System.pt(Ls, ladders, betas)
+    pt.pt_run(1.0, 20, 1, tile=64, use_graph=True)
+    pt.close()
+    f = abi.System.plaquette_ising(256, 1.0, 0.2)
+    _, J, _, _ = slmc.fit_effective_model(f, 0.40, burn=500, nconfigs=200, every=1, run_seed=3)
+    f.slmc_run(J, 0.40, 200, 4, record=False)
+    f.metropolis(0.40, 500, 5)
+    f.close()
+    print(json.dumps({"profile": "other_configs"}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,6 +210,8 @@ def main():
     ap.add_argument("--quick-suite", action="store_true")
     ap.add_argument("--profile", action="store_true",
                     help="only the headline kernel and a same-size copy kernel, for rocprofv3 --pmc")
+    ap.add_argument("--profile-others", action="store_true",
+                    help="the other configurations for a few steps each, for rocprofv3 --kernel-trace")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one MI355X, and the lattice "
@@ -181,6 +220,9 @@ def main():
 
     from universal_classical_montecarlo_amd import abi
     L = args.L
+    if args.profile_others:
+        profile_others()
+        return
     if args.profile:
         # 50 fused sweeps, then 50 copies of the same byte count (read L^2 + write L^2)
         s = abi.System.ising_square(L, L, 1)

@@ -164,8 +164,11 @@ __device__ __forceinline__ float norm_inf(const Mat<N>& a)
 }
 
 // dof/sun.hpp:192-220 in fp32: halve while the inf-norm exceeds 0.5, Taylor
-// series until the term norm drops below fp32 resolution (the reference stops
-// at 1e-16 in double), square back.
+// series, square back.  The reference sums terms until one drops below 1e-16
+// (double); in fp32 the series is cut at the order m whose remainder bound
+// n^(m+1) / (m+1)! stays below 2^-25 for the scaled norm n <= 0.5 (m = 8 at
+// n = 0.5, fewer for smaller n), and evaluated in Horner form,
+// I + b (I + b/2 (I + ... (I + b/m))), with no per-term norm.
 template <int N>
 __device__ __forceinline__ void expm(const Mat<N>& a, Mat<N>& out)
 {
@@ -175,26 +178,24 @@ __device__ __forceinline__ void expm(const Mat<N>& a, Mat<N>& out)
         n *= 0.5f;
         ++sq;
     }
+    const int m = n <= 0.08f ? 4 : n <= 0.16f ? 5 : n <= 0.28f ? 6 : n <= 0.43f ? 7 : 8;
     const float scale = ldexpf(1.0f, -sq);
-    Mat<N> b, term, tmp;
+    Mat<N> b, tmp;
 #pragma unroll
     for (int k = 0; k < N * N; ++k) {
         b.r[k] = a.r[k] * scale;
         b.i[k] = a.i[k] * scale;
-        out.r[k] = term.r[k] = (k % (N + 1) == 0) ? 1.f : 0.f;
-        out.i[k] = term.i[k] = 0.f;
+        out.r[k] = (k % (N + 1) == 0) ? 1.f : 0.f;
+        out.i[k] = 0.f;
     }
-    for (int k = 1; k <= 30; ++k) {
-        mul<N, false>(term, b, tmp);
+    for (int k = m; k >= 1; --k) {   // out = I + (b out) / k
+        mul<N, false>(b, out, tmp);
         const float inv = 1.0f / static_cast<float>(k);
 #pragma unroll
         for (int q = 0; q < N * N; ++q) {
-            term.r[q] = tmp.r[q] * inv;
-            term.i[q] = tmp.i[q] * inv;
-            out.r[q] += term.r[q];
-            out.i[q] += term.i[q];
+            out.r[q] = tmp.r[q] * inv + ((q % (N + 1) == 0) ? 1.f : 0.f);
+            out.i[q] = tmp.i[q] * inv;
         }
-        if (norm_inf(term) < 1e-8f) break;
     }
     for (int k = 0; k < sq; ++k) {
         mul<N, false>(out, out, tmp);

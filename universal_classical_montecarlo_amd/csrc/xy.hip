@@ -271,25 +271,28 @@ __global__ __launch_bounds__(256) void pt_exchange(PtArgs a)
     }
     __syncthreads();
     const std::uint64_t pkey = key_of(a.swap_seed, sweep);
-    for (std::uint32_t ld = threadIdx.x; ld < a.nladders; ld += blockDim.x) {
+    // the pairs (k, k+1) with k of the sweep's parity are disjoint: one thread
+    // per slot k (the slots of every ladder back to back)
+    for (std::uint32_t k = threadIdx.x; k < a.nreps; k += blockDim.x) {
+        std::uint32_t ld = 0;
+        while (a.ladder_start[ld + 1] <= k) ++ld;
         const std::uint32_t s0 = a.ladder_start[ld], s1 = a.ladder_start[ld + 1];
-        for (std::uint32_t k = s0 + static_cast<std::uint32_t>(sweep & 1u); k + 1 < s1; k += 2) {
-            const std::uint32_t ra = a.rep_at[k], rb = a.rep_at[k + 1];
-            const double ea = __hip_atomic_load(&a.energy[ra], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const double eb = __hip_atomic_load(&a.energy[rb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            // min(1, exp((beta_k - beta_k+1)(E_a - E_b)))
-            const double log_p = (a.betas[k] - a.betas[k + 1]) * (ea - eb);
-            Stream sw;
-            sw.init(pkey, k, kTagSwap);
-            const bool swap = log_p >= 0.0 || log(sw.uni()) < log_p;
-            a.swap_try[k] += 1;
-            if (swap) {
-                a.rep_at[k] = rb;
-                a.rep_at[k + 1] = ra;
-                a.temp_of[ra] = k + 1;
-                a.temp_of[rb] = k;
-                a.swap_acc[k] += 1;
-            }
+        if (((k - s0) & 1u) != (sweep & 1u) || k + 1 >= s1) continue;
+        const std::uint32_t ra = a.rep_at[k], rb = a.rep_at[k + 1];
+        const double ea = __hip_atomic_load(&a.energy[ra], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const double eb = __hip_atomic_load(&a.energy[rb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // min(1, exp((beta_k - beta_k+1)(E_a - E_b)))
+        const double log_p = (a.betas[k] - a.betas[k + 1]) * (ea - eb);
+        Stream sw;
+        sw.init(pkey, k, kTagSwap);
+        const bool swap = log_p >= 0.0 || log(sw.uni()) < log_p;
+        a.swap_try[k] += 1;
+        if (swap) {
+            a.rep_at[k] = rb;
+            a.rep_at[k + 1] = ra;
+            a.temp_of[ra] = k + 1;
+            a.temp_of[rb] = k;
+            a.swap_acc[k] += 1;
         }
     }
     __syncthreads();
