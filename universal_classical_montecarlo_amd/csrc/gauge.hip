@@ -79,22 +79,26 @@ struct Mat {
     float r[N * N], i[N * N];
 };
 
+// Byte offsets in 32 bits (the field is below 4 GiB, checked on the host), so
+// the loads take the uniform base and a 32-bit per-lane offset.
 template <int N>
 __device__ __forceinline__ void load(const float* f, std::uint32_t nl, std::uint32_t idx, Mat<N>& m)
 {
+    const char* fb = reinterpret_cast<const char*>(f);
 #pragma unroll
     for (int k = 0; k < N * N; ++k) {
-        m.r[k] = f[(2 * k) * static_cast<std::size_t>(nl) + idx];
-        m.i[k] = f[(2 * k + 1) * static_cast<std::size_t>(nl) + idx];
+        m.r[k] = *reinterpret_cast<const float*>(fb + ((2u * k) * nl + idx) * 4u);
+        m.i[k] = *reinterpret_cast<const float*>(fb + ((2u * k + 1u) * nl + idx) * 4u);
     }
 }
 template <int N>
 __device__ __forceinline__ void store(float* f, std::uint32_t nl, std::uint32_t idx, const Mat<N>& m)
 {
+    char* fb = reinterpret_cast<char*>(f);
 #pragma unroll
     for (int k = 0; k < N * N; ++k) {
-        f[(2 * k) * static_cast<std::size_t>(nl) + idx] = m.r[k];
-        f[(2 * k + 1) * static_cast<std::size_t>(nl) + idx] = m.i[k];
+        *reinterpret_cast<float*>(fb + ((2u * k) * nl + idx) * 4u) = m.r[k];
+        *reinterpret_cast<float*>(fb + ((2u * k + 1u) * nl + idx) * 4u) = m.i[k];
     }
 }
 // C = A op(B), op = identity or dagger
@@ -111,8 +115,8 @@ __device__ __forceinline__ void mul(const Mat<N>& a, const Mat<N>& b, Mat<N>& c)
                 const float ar = a.r[x * N + k], ai = a.i[x * N + k];
                 const float br = DAGB ? b.r[y * N + k] : b.r[k * N + y];
                 const float bi = DAGB ? -b.i[y * N + k] : b.i[k * N + y];
-                sr += ar * br - ai * bi;
-                si += ar * bi + ai * br;
+                sr = fmaf(ar, br, fmaf(-ai, bi, sr));   // four FMAs per complex product
+                si = fmaf(ar, bi, fmaf(ai, br, si));
             }
             c.r[x * N + y] = sr;
             c.i[x * N + y] = si;
@@ -131,8 +135,8 @@ __device__ __forceinline__ void mul_dag_a(const Mat<N>& a, const Mat<N>& b, Mat<
             for (int k = 0; k < N; ++k) {
                 const float ar = a.r[k * N + x], ai = -a.i[k * N + x];
                 const float br = b.r[k * N + y], bi = b.i[k * N + y];
-                sr += ar * br - ai * bi;
-                si += ar * bi + ai * br;
+                sr = fmaf(ar, br, fmaf(-ai, bi, sr));
+                si = fmaf(ar, bi, fmaf(ai, br, si));
             }
             c.r[x * N + y] = sr;
             c.i[x * N + y] = si;
@@ -146,7 +150,7 @@ __device__ __forceinline__ float retr_mul(const Mat<N>& a, const Mat<N>& b)
 #pragma unroll
     for (int x = 0; x < N; ++x)
 #pragma unroll
-        for (int k = 0; k < N; ++k) t += a.r[x * N + k] * b.r[k * N + x] - a.i[x * N + k] * b.i[k * N + x];
+        for (int k = 0; k < N; ++k) t = fmaf(a.r[x * N + k], b.r[k * N + x], fmaf(-a.i[x * N + k], b.i[k * N + x], t));
     return t;
 }
 template <int N>
@@ -193,7 +197,7 @@ __device__ __forceinline__ void expm(const Mat<N>& a, Mat<N>& out)
         const float inv = 1.0f / static_cast<float>(k);
 #pragma unroll
         for (int q = 0; q < N * N; ++q) {
-            out.r[q] = tmp.r[q] * inv + ((q % (N + 1) == 0) ? 1.f : 0.f);
+            out.r[q] = fmaf(tmp.r[q], inv, (q % (N + 1) == 0) ? 1.f : 0.f);
             out.i[q] = tmp.i[q] * inv;
         }
     }
@@ -639,6 +643,8 @@ void gauge_setup(System& s)
         s.colour[i] = static_cast<std::uint8_t>(2 * (i / L4) + (par & 1));
     }
     const std::size_t planes = s.kind == UCMC_U1_GAUGE ? 1 : 2 * s.N * s.N;
+    // the SU(N) kernels address the field with 32-bit byte offsets
+    if (s.sites * planes * sizeof(float) > 0xFFFFFFFFull) fail(UCMC_E_OVERFLOW, "gauge field exceeds 4 GiB");
     s.field[0].alloc(s.sites * planes * sizeof(float));
     std::vector<float> init(s.sites * planes, 0.f);
     if (s.kind == UCMC_SUN_GAUGE) // example/sun_gauge.hpp:26: identity (cold) start
