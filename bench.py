@@ -178,7 +178,7 @@ def suite(quick=False):
     from universal_classical_montecarlo_amd import slmc
     out["slmc"] = slmc.compare(L=64 if quick else 256, nconfigs=4000 if quick else 20000,
                                plain_sweeps=20000 if quick else 200000,
-                               slmc_steps=2000 if quick else 20000)
+                               slmc_steps=2000 if quick else 100000)
     return out
 
 
