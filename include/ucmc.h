@@ -298,6 +298,21 @@ int ucmc_slmc_run(ucmc_system* sys, const double* J_eff, double beta, uint64_t r
  * sweep-kernel launches in it. */
 int ucmc_metropolis_bench(ucmc_system* sys, const ucmc_metropolis_params* p, int repeats,
                           double* elapsed_ms, uint64_t* launches);
+/* ---- checkpoint / resume ---- */
+/* No counterpart in the reference: its Pcg32 state is private (rng.hpp:42-43),
+ * so a chain cannot be resumed there.  Here every draw is a function of
+ * (run_seed, sweep or flip index, site), so a chain's state is its field plus,
+ * for a PT batch, the replica <-> temperature assignment, the device sweep
+ * counter and the accept / swap counters.  A run of the same system type and
+ * size resumed from a checkpoint with the same run_seed and the next sweep0 /
+ * flip0 continues the chain exactly.  The buffer is opaque (a header with the
+ * type and sizes, then the device field as stored and the PT state);
+ * ucmc_checkpoint_load rejects a buffer of another type or size with
+ * UCMC_E_INVALID_ARGUMENT. */
+int ucmc_checkpoint_size(const ucmc_system* sys, size_t* bytes);
+int ucmc_checkpoint_save(ucmc_system* sys, void* buf, size_t bytes);
+int ucmc_checkpoint_load(ucmc_system* sys, const void* buf, size_t bytes);
+
 /* A plain device copy of `bytes` bytes (read + write), `repeats` times on
  * device 0: the calibration kernel for HBM byte counters.  gbps = read +
  * write bytes per second / 1e9. */

@@ -177,6 +177,33 @@ int main()
         std::printf("rotors Metropolis %.5f Wolff %.5f\n", em, ew);
         EXPECT(std::abs(em - ew) < 0.03);
     }
+    {   // checkpoint / resume: 30 + 30 sweeps through a saved chain equal 60 sweeps
+        RingIsing a(64, 1), b(64, 1), c(64, 1);
+        ucmc::gpu::Metropolis ma(a, Flip{}, Rng{9}, 0.5);
+        ma.run(30);
+        const auto ck = ma.checkpoint();
+        ma.run(30);
+        ma.sync();
+        ucmc::gpu::Metropolis mb(b, Flip{}, Rng{9}, 0.5);
+        mb.restore(ck);
+        EXPECT(mb.sweeps_done() == 30);
+        mb.run(30);
+        mb.sync();
+        bool same = true;
+        for (std::size_t i = 0; i < a.sites(); ++i) same = same && a.data()[i] == b.data()[i];
+        EXPECT(same);
+        ucmc::gpu::Wolff wa(c, FixedAxis{}, Rng{10}, 0.5);
+        wa.flip();
+        const auto wck = wa.checkpoint();
+        bool threw = false;
+        try {
+            mb.restore(std::vector<std::uint8_t>(wck.begin(), wck.end() - 1));
+        } catch (const std::invalid_argument&) {
+            threw = true;
+        }
+        EXPECT(threw);
+        std::printf("checkpoint resume %s\n", same ? "identical" : "DIFFERENT");
+    }
     {   // errors keep the reference's exception types
         std::vector<std::uint32_t> off{0, 1, 2}, b{0, 0};
         const ucmc_pair_graph g{2, off.data(), b.data()};

@@ -1,0 +1,101 @@
+"""Checkpoint / resume (ucmc_checkpoint_*): a chain stopped after k sweeps,
+saved, loaded into a fresh system of the same type and continued with the
+next sweep0 (flip0) ends exactly where the uninterrupted chain ends.  The
+reference has no counterpart (its Pcg32 state is private, rng.hpp:42-43); the
+counter-based streams make the field (plus the PT assignment and counters) the
+whole state."""
+import numpy as np
+import pytest
+
+from oracle_lib import csr_cubic, csr_square, random_spins
+from test_gpu_vector import random_unit
+from universal_classical_montecarlo_amd import abi
+
+pytestmark = pytest.mark.gpu
+
+
+def test_ising_metropolis_resume():
+    L = 64
+    a = abi.System.ising_square(L, L, 1)
+    a.upload(random_spins(L * L, 8, 1))
+    a.metropolis(0.44, 10, 21, sweep0=0)
+    ck = a.checkpoint()
+    a.metropolis(0.44, 10, 21, sweep0=10)
+    want = a.download()
+    b = abi.System.ising_square(L, L, 1)
+    b.restore(ck)
+    b.metropolis(0.44, 10, 21, sweep0=10)
+    assert np.array_equal(b.download(), want)
+    a.close()
+    b.close()
+
+
+def test_wolff_resume_and_mismatch():
+    L = 64
+    off, bonds = csr_square(L, L)
+    a = abi.System.ising(off, bonds, 1)
+    a.upload(random_spins(L * L, 9, 2))
+    a.wolff(0.44, 30, 5, flags=abi.WOLFF_LABELS)
+    ck = a.checkpoint()
+    sa, _ = a.wolff(0.44, 30, 5, flip0=30, flags=abi.WOLFF_LABELS)
+    b = abi.System.ising(off, bonds, 1)
+    b.restore(ck)
+    sb, _ = b.wolff(0.44, 30, 5, flip0=30, flags=abi.WOLFF_LABELS)
+    assert np.array_equal(sa, sb)
+    assert np.array_equal(a.download(), b.download())
+    # another size or another model is refused
+    c = abi.System.ising_square(32, 32, 1)
+    with pytest.raises(abi.UcmcError):
+        c.restore(ck)
+    off3, bonds3 = csr_cubic(16)
+    v = abi.System.vector(3, off3, bonds3, 1.0)
+    with pytest.raises(abi.UcmcError):
+        v.restore(ck)
+    with pytest.raises(abi.UcmcError):
+        b.restore(ck[:-1])
+    for s in (a, b, c, v):
+        s.close()
+
+
+def test_vector_resume():
+    L = 8
+    off, bonds = csr_cubic(L)
+    n = L ** 3
+    a = abi.System.vector(3, off, bonds, 1.0)
+    a.upload(random_unit(3, n, 31))
+    a.metropolis(0.693, 5, 4, proposal=abi.PROP_ROTATION, width=0.5)
+    ck = a.checkpoint()
+    a.metropolis(0.693, 5, 4, sweep0=5, proposal=abi.PROP_ROTATION, width=0.5)
+    b = abi.System.vector(3, off, bonds, 1.0)
+    b.restore(ck)
+    b.metropolis(0.693, 5, 4, sweep0=5, proposal=abi.PROP_ROTATION, width=0.5)
+    assert np.array_equal(a.download(), b.download())
+    a.close()
+    b.close()
+
+
+def test_pt_resume():
+    Ls = [8, 8, 16, 16]
+    ladders = [0, 0, 1, 1]
+    betas = [0.9, 1.2, 0.8, 1.3]
+    a = abi.System.pt(Ls, ladders, betas)
+    rng = np.random.default_rng(4)
+    a.upload(np.concatenate([rng.uniform(-np.pi, np.pi, L * L) for L in Ls]).astype(np.float32))
+    a.pt_run(1.0, 7, 3, sweep0=0, tile=8)
+    ck = a.checkpoint()
+    Ea, ta = a.pt_run(1.0, 6, 3, sweep0=7, tile=8, record=True)
+    b = abi.System.pt(Ls, ladders, betas)
+    b.restore(ck)
+    Eb, tb = b.pt_run(1.0, 6, 3, sweep0=7, tile=8, record=True)
+    # per-replica energies are sums of per-tile atomics (order not fixed)
+    assert np.allclose(Ea, Eb, rtol=1e-12, atol=1e-9)
+    assert np.array_equal(ta, tb)
+    assert np.array_equal(a.download(), b.download())
+    for x, y in zip(a.pt_stats(), b.pt_stats()):
+        assert np.array_equal(x, y)
+    # the same batch size with other replica edges is refused
+    c = abi.System.pt([16, 16, 8, 8], [0, 0, 1, 1], betas)
+    with pytest.raises(abi.UcmcError):
+        c.restore(ck)
+    for s in (a, b, c):
+        s.close()

@@ -39,7 +39,7 @@ EXPORTED = [
     "ucmc_observe", "ucmc_reunitarize", "ucmc_metropolis_bench", "ucmc_pt_create",
     "ucmc_pt_run", "ucmc_pt_stats", "ucmc_pt_autotune", "ucmc_slmc_collect",
     "ucmc_slmc_fit_normal", "ucmc_slmc_fit_local", "ucmc_slmc_loss_grad", "ucmc_slmc_fit_adam", "ucmc_slmc_run",
-    "ucmc_measure_copy",
+    "ucmc_checkpoint_size", "ucmc_checkpoint_save", "ucmc_checkpoint_load", "ucmc_measure_copy",
 ]
 
 
@@ -119,6 +119,9 @@ def lib():
     L.ucmc_wolff_run.argtypes = [vp, C.POINTER(WolffParams), C.c_void_p, C.c_void_p]
     L.ucmc_observe.argtypes = [vp, C.POINTER(Observables)]
     L.ucmc_reunitarize.argtypes = [vp]
+    L.ucmc_checkpoint_size.argtypes = [vp, C.POINTER(C.c_size_t)]
+    L.ucmc_checkpoint_save.argtypes = [vp, C.c_void_p, C.c_size_t]
+    L.ucmc_checkpoint_load.argtypes = [vp, C.c_void_p, C.c_size_t]
     L.ucmc_pt_create.argtypes = [C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
                                  C.c_int, C.POINTER(vp)]
     L.ucmc_pt_run.argtypes = [vp, C.POINTER(PtParams), C.c_void_p, C.c_void_p]
@@ -247,6 +250,19 @@ class System:
         check(lib().ucmc_pt_run(self.h, C.byref(p), None if series is None else _ptr(series),
                                 _ptr(temp)))
         return series, temp
+
+    def checkpoint(self):
+        """The chain state (ucmc_checkpoint_save) as a uint8 array."""
+        n = C.c_size_t()
+        check(lib().ucmc_checkpoint_size(self.h, C.byref(n)))
+        buf = np.zeros(n.value, np.uint8)
+        check(lib().ucmc_checkpoint_save(self.h, _ptr(buf), n.value))
+        return buf
+
+    def restore(self, buf):
+        """Loads a checkpoint of a system of the same type and size."""
+        buf = np.ascontiguousarray(buf, np.uint8)
+        check(lib().ucmc_checkpoint_load(self.h, _ptr(buf), buf.size))
 
     def pt_stats(self):
         a = np.zeros(self.nreps, np.uint64)

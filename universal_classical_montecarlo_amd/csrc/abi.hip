@@ -466,6 +466,105 @@ int ucmc_pt_stats(const ucmc_system* sys, uint64_t* accepts, uint64_t* swap_acce
     });
 }
 
+namespace {
+
+struct CheckpointHeader {
+    char magic[8];               // "UCMCCKP1"
+    std::uint32_t kind, N;
+    std::uint64_t sites, field_bytes, replicas;
+};
+constexpr char kCheckpointMagic[8] = {'U', 'C', 'M', 'C', 'C', 'K', 'P', '1'};
+
+// PT state after the field: the replica edges L, temp_of, rep_at (u32 each),
+// accepts, swap_acc, swap_try (u64 each) per replica, then the device sweep
+// counter.
+std::size_t pt_state_bytes(const System& s)
+{
+    return s.kind == UCMC_XY_PT ? s.pt.L.size() * (4 + 4 + 4 + 8 + 8 + 8) + 8 : 0;
+}
+
+std::size_t checkpoint_bytes(const System& s)
+{
+    return sizeof(CheckpointHeader) + s.field[s.cur].bytes + pt_state_bytes(s);
+}
+
+}  // namespace
+
+int ucmc_checkpoint_size(const ucmc_system* sys, size_t* bytes)
+{
+    return guard([&] {
+        if (!sys || !bytes) fail(UCMC_E_INVALID_ARGUMENT, "null argument");
+        *bytes = checkpoint_bytes(*sys);
+    });
+}
+
+int ucmc_checkpoint_save(ucmc_system* sys, void* buf, size_t bytes)
+{
+    return guard([&] {
+        System* s = checked(sys);
+        if (!buf || bytes != checkpoint_bytes(*s)) fail(UCMC_E_INVALID_ARGUMENT, "checkpoint buffer size");
+        auto* out = static_cast<std::uint8_t*>(buf);
+        CheckpointHeader h{};
+        std::memcpy(h.magic, kCheckpointMagic, 8);
+        h.kind = static_cast<std::uint32_t>(s->kind);
+        h.N = static_cast<std::uint32_t>(s->N);
+        h.sites = s->sites;
+        h.field_bytes = s->field[s->cur].bytes;
+        h.replicas = s->kind == UCMC_XY_PT ? s->pt.L.size() : 0;
+        std::memcpy(out, &h, sizeof h);
+        out += sizeof h;
+        UCMC_HIP(hipMemcpyAsync(out, s->field[s->cur].p, h.field_bytes, hipMemcpyDeviceToHost, s->stream));
+        out += h.field_bytes;
+        if (s->kind == UCMC_XY_PT) {
+            const std::size_t R = h.replicas;
+            auto& pt = s->pt;
+            std::memcpy(out, pt.L.data(), R * 4);
+            out += R * 4;
+            for (const auto& [buf_d, n] : {std::pair<DevBuf*, std::size_t>{&pt.temp_of, R * 4}, {&pt.rep_at, R * 4},
+                                          {&pt.accepts, R * 8}, {&pt.swap_acc, R * 8}, {&pt.swap_try, R * 8},
+                                          {&pt.sweep_ctr, 8}}) {
+                UCMC_HIP(hipMemcpyAsync(out, buf_d->p, n, hipMemcpyDeviceToHost, s->stream));
+                out += n;
+            }
+        }
+        UCMC_HIP(hipStreamSynchronize(s->stream));
+    });
+}
+
+int ucmc_checkpoint_load(ucmc_system* sys, const void* buf, size_t bytes)
+{
+    return guard([&] {
+        System* s = checked(sys);
+        if (!buf || bytes < sizeof(CheckpointHeader)) fail(UCMC_E_INVALID_ARGUMENT, "checkpoint buffer too small");
+        const auto* in = static_cast<const std::uint8_t*>(buf);
+        CheckpointHeader h;
+        std::memcpy(&h, in, sizeof h);
+        in += sizeof h;
+        if (std::memcmp(h.magic, kCheckpointMagic, 8) != 0) fail(UCMC_E_INVALID_ARGUMENT, "not a checkpoint");
+        if (h.kind != static_cast<std::uint32_t>(s->kind) || h.N != static_cast<std::uint32_t>(s->N) ||
+            h.sites != s->sites || h.field_bytes != s->field[s->cur].bytes ||
+            h.replicas != (s->kind == UCMC_XY_PT ? s->pt.L.size() : 0) || bytes != checkpoint_bytes(*s))
+            fail(UCMC_E_INVALID_ARGUMENT, "checkpoint of another system type or size");
+        if (s->kind == UCMC_XY_PT &&   // the same replica edges, in the same order
+            std::memcmp(in + h.field_bytes, s->pt.L.data(), h.replicas * 4) != 0)
+            fail(UCMC_E_INVALID_ARGUMENT, "checkpoint of another replica layout");
+        UCMC_HIP(hipMemcpyAsync(s->field[s->cur].p, in, h.field_bytes, hipMemcpyHostToDevice, s->stream));
+        in += h.field_bytes;
+        if (s->kind == UCMC_XY_PT) {
+            const std::size_t R = h.replicas;
+            auto& pt = s->pt;
+            in += R * 4;
+            for (const auto& [buf_d, n] : {std::pair<DevBuf*, std::size_t>{&pt.temp_of, R * 4}, {&pt.rep_at, R * 4},
+                                          {&pt.accepts, R * 8}, {&pt.swap_acc, R * 8}, {&pt.swap_try, R * 8},
+                                          {&pt.sweep_ctr, 8}}) {
+                UCMC_HIP(hipMemcpyAsync(buf_d->p, in, n, hipMemcpyHostToDevice, s->stream));
+                in += n;
+            }
+        }
+        UCMC_HIP(hipStreamSynchronize(s->stream));
+    });
+}
+
 int ucmc_pt_autotune(ucmc_system* sys, double width, uint32_t* tile, double* best_ms_per_sweep)
 {
     return guard([&] {

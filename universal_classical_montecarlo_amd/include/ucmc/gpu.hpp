@@ -324,6 +324,28 @@ double proposal_width(const Q& q)
 }
 
 // ----------------------------------------------------------- Metropolis --
+// Chain checkpoints (ucmc_checkpoint_*), prefixed with the driver's counter
+// (sweeps or flips done): restore() on a driver over a model of the same type
+// and size, built with the same rng, continues the chain exactly.  No
+// counterpart in the reference (its Pcg32 state is private, rng.hpp:42-43).
+inline std::vector<std::uint8_t> save_chain(ucmc_system* sys, std::uint64_t counter)
+{
+    std::size_t n = 0;
+    check(ucmc_checkpoint_size(sys, &n));
+    std::vector<std::uint8_t> buf(sizeof counter + n);
+    std::memcpy(buf.data(), &counter, sizeof counter);
+    check(ucmc_checkpoint_save(sys, buf.data() + sizeof counter, n));
+    return buf;
+}
+inline std::uint64_t load_chain(ucmc_system* sys, const std::vector<std::uint8_t>& buf)
+{
+    std::uint64_t counter = 0;
+    if (buf.size() < sizeof counter) throw std::invalid_argument("checkpoint too short");
+    check(ucmc_checkpoint_load(sys, buf.data() + sizeof counter, buf.size() - sizeof counter));
+    std::memcpy(&counter, buf.data(), sizeof counter);
+    return counter;
+}
+
 template <class M, class Q, class R>
 class Metropolis
 {
@@ -373,6 +395,11 @@ public:
 
     /// Re-reads the model's field (after the caller changed it).
     void reload() { Device<M>::upload(sys_, model_); }
+
+    /// The chain's state (device field and sweeps done), for resuming later.
+    std::vector<std::uint8_t> checkpoint() const { return save_chain(sys_.get(), sweep_); }
+    /// Resumes from checkpoint(); call sync() to see the field in the model.
+    void restore(const std::vector<std::uint8_t>& buf) { sweep_ = load_chain(sys_.get(), buf); }
 
     ucmc_info info() const { return sys_.info(); }
     std::uint64_t sweeps_done() const noexcept { return sweep_; }
@@ -482,6 +509,11 @@ public:
 
     void sync() { Device<M>::download(sys_, model_); }
     void reload() { Device<M>::upload(sys_, model_); }
+
+    /// The chain's state (device field and flips done), for resuming later; the
+    /// mean-cluster statistics are not part of it.
+    std::vector<std::uint8_t> checkpoint() const { return save_chain(sys_.get(), flip_); }
+    void restore(const std::vector<std::uint8_t>& buf) { flip_ = load_chain(sys_.get(), buf); }
 
 private:
     ucmc_wolff_params params(std::uint64_t n, std::uint32_t flags) const
