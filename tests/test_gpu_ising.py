@@ -133,6 +133,23 @@ def test_ising_random_graph(gpu):
     sysm.close()
 
 
+def test_ising_fused_16_row_blocks_equal_simple(gpu):
+    """Lattices with enough 16-row blocks to fill the device take the 16-row
+    variant of the fused kernel (smaller ones, like every other case here, the
+    8-row one); both equal the CSR colour-by-colour kernel bit for bit."""
+    Lx, Ly = 4096, 8192
+    a = abi.System.ising_square(Lx, Ly, 1)
+    spins = random_spins(Lx * Ly, 6, 2)
+    a.upload(spins)
+    a.metropolis(0.44, 2, 13, sweep0=40)
+    fused = a.download()
+    a.upload(spins)
+    a.metropolis(0.44, 2, 13, sweep0=40, flags=abi.FORCE_SIMPLE)
+    simple = a.download()
+    assert np.array_equal(fused, simple)
+    a.close()
+
+
 def test_ising_4096_full_size(gpu):
     """Full headline size: two sweeps bit-exact against the replay, then the
     device reduction against a host reduction of the downloaded field."""

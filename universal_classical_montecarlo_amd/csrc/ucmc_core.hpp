@@ -59,6 +59,7 @@ struct System {
     int device = 0;
     int N = 1;                 // components (VectorSpin N, SU(N) N)
     hipStream_t stream = nullptr;
+    int cus = 0;               // compute units of the device (0: not asked yet)
     std::size_t sites = 0;
     Geom geom = Geom::General;
     std::size_t Lx = 0, Ly = 0, L = 0;
