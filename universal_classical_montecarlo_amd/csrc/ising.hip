@@ -17,8 +17,10 @@
 // u_32 <= k*(dE) - 1, with k* computed on the host with the same double
 // expression (accept_threshold), so decisions are bit-identical to the CPU.
 // u_32 of site i is ising_u32(key, i) (Philox4x32-10, ucmc_device.hpp); the
-// fused kernel compares its high 16 bits first and computes the low 16 bits
-// only for a wave in which some site ties.
+// fused kernel compares its high 16 bits first, two sites at a time with packed
+// 16-bit differences against a pair table indexed by the sites' numbers of
+// anti-aligned neighbours, and computes the low 16 bits only for a wave in
+// which some site ties.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -130,56 +132,37 @@ __device__ __forceinline__ void store_row(std::uint32_t* out, std::uint32_t row_
     *reinterpret_cast<uint2*>(out + static_cast<std::size_t>(Y) * row_words + 2 * col) = w;
 }
 
-// The fast half of the exact decision u <= t for the four sites of one
-// 4b-form word, u = (h << 16) | l: byte k of C takes byte k of Cf when
-// h_k < T_k (T = t >> 16), with h_0, h_1 the halves of wa and h_2, h_3 those of
-// wb.  Per byte: one SDWA compare on the 16-bit half and one sub-dword
-// conditional move (dst_sel:BYTE_k, the other bytes preserved).  One asm block
-// for the word, so the table reads are issued before it.
-__device__ __forceinline__ std::uint32_t select_lt(std::uint32_t C, std::uint32_t Cf, std::uint32_t wa,
-                                                   std::uint32_t wb, const std::uint32_t (&t)[4])
-{
-#define UCMC_SEL(B, W, H, T)                                                                     \
-    "v_cmp_lt_u32_sdwa vcc, " W ", " T " src0_sel:WORD_" #H " src1_sel:DWORD\n\t"                 \
-    "v_cndmask_b32_sdwa %0, %0, %7, vcc dst_sel:BYTE_" #B                                       \
-    " dst_unused:UNUSED_PRESERVE src0_sel:BYTE_" #B " src1_sel:BYTE_" #B "\n\t"
-    asm(UCMC_SEL(0, "%1", 0, "%3") UCMC_SEL(1, "%1", 1, "%4") UCMC_SEL(2, "%2", 0, "%5")
-            UCMC_SEL(3, "%2", 1, "%6")
-        : "+v"(C)
-        : "v"(wa), "v"(wb), "v"(t[0]), "v"(t[1]), "v"(t[2]), "v"(t[3]), "v"(Cf)
-        : "vcc");
-#undef UCMC_SEL
-    return C;
-}
-
-// Lanes (bit mask) where some byte's high half equals T_k: those bytes need
-// the low halves.
-__device__ __forceinline__ std::uint64_t ties(std::uint32_t wa, std::uint32_t wb, const std::uint32_t (&t)[4])
-{
-    std::uint64_t m0, m1, m2, m3;
-    asm("v_cmp_eq_u32_sdwa %0, %4, %6 src0_sel:WORD_0 src1_sel:DWORD\n\t"
-        "v_cmp_eq_u32_sdwa %1, %4, %7 src0_sel:WORD_1 src1_sel:DWORD\n\t"
-        "v_cmp_eq_u32_sdwa %2, %5, %8 src0_sel:WORD_0 src1_sel:DWORD\n\t"
-        "v_cmp_eq_u32_sdwa %3, %5, %9 src0_sel:WORD_1 src1_sel:DWORD"
-        : "=s"(m0), "=s"(m1), "=s"(m2), "=s"(m3)
-        : "v"(wa), "v"(wb), "v"(t[0]), "v"(t[1]), "v"(t[2]), "v"(t[3]));
-    return m0 | m1 | m2 | m3;
-}
-
-// Thresholds of the fused kernel in LDS: T (high 16 bits, 0x10000 when every
-// u is accepted) at word S + 64 C4 (S = 4 * # of -1 neighbours, C4 = 0 or 4 for
-// a +1 / -1 site), the low 16 bits at the same index of the second table.
+// Thresholds of the fused kernel in LDS, by the number a of anti-aligned
+// neighbours of a site (dE = 2 J (4 - 2a)): hi[a] = t_a >> 16 and lo[a] =
+// t_a & 0xFFFF of the integer threshold t_a (0xFFFF, 0xFFFF when every u is
+// accepted), and pair[a0 + 8 a1] = (hi[a0] ^ 0x8000) | (hi[a1] ^ 0x8000) << 16,
+// the biased high thresholds of two neighbouring sites in one word.
 struct FusedTables {
-    std::uint32_t hi[261];
-    std::uint32_t lo[261];
+    std::uint32_t pair[37];
+    std::uint32_t hi[5], lo[5];
 };
 
+typedef short short2v __attribute__((ext_vector_type(2)));
+typedef unsigned short ushort2v __attribute__((ext_vector_type(2)));
+
+// v_pk_sub_i16 with clamp: saturating signed differences of the two halves
+__device__ __forceinline__ std::uint32_t pk_sub_sat(std::uint32_t x, std::uint32_t y)
+{
+    return __builtin_bit_cast(std::uint32_t, __builtin_elementwise_sub_sat(__builtin_bit_cast(short2v, x),
+                                                                           __builtin_bit_cast(short2v, y)));
+}
+// v_pk_min_u16
+__device__ __forceinline__ std::uint32_t pk_min_u16(std::uint32_t x, std::uint32_t y)
+{
+    return __builtin_bit_cast(std::uint32_t, __builtin_elementwise_min(__builtin_bit_cast(ushort2v, x),
+                                                                       __builtin_bit_cast(ushort2v, y)));
+}
+
 // The ties of one word (rare: probability 2^-16 per site): bytes whose high
-// half equals T_k flip when the low half is <= the low threshold.  Kept out of
-// line so the sweep loop around its call site stays small.
-__device__ __noinline__ std::uint32_t resolve_ties(std::uint32_t out, std::uint32_t C, std::uint32_t S,
-                                                   std::uint32_t wa, std::uint32_t wb,
-                                                   const FusedTables* tb, std::uint64_t key,
+// half equals hi[a] flip when the low half is <= lo[a].  Kept out of line so
+// the sweep loop around its call site stays small.
+__device__ __noinline__ std::uint32_t resolve_ties(std::uint32_t out, std::uint32_t A, std::uint32_t wa,
+                                                   std::uint32_t wb, const FusedTables* tb, std::uint64_t key,
                                                    std::uint32_t q, std::uint32_t r, bool odd)
 {
     const U4 lo = philox(U4{q, r, kTagIsingLo, 0u}, key);
@@ -188,36 +171,45 @@ __device__ __noinline__ std::uint32_t resolve_ties(std::uint32_t out, std::uint3
         const std::uint32_t w = k < 2 ? wa : wb, l = k < 2 ? la : lb;
         const std::uint32_t h16 = (k & 1) ? (w >> 16) : (w & 0xFFFFu);
         const std::uint32_t l16 = (k & 1) ? (l >> 16) : (l & 0xFFFFu);
-        const std::uint32_t idx = (((S >> (8 * k)) & 0xFFu) >> 2) + 64u * ((C >> (8 * k)) & 0xFFu);
-        if (h16 == tb->hi[idx] && l16 <= tb->lo[idx]) out ^= 4u << (8 * k);
+        const std::uint32_t an = ((A >> (8 * k)) & 0xFFu) >> 2;
+        if (h16 == tb->hi[an] && l16 <= tb->lo[an]) out ^= 4u << (8 * k);
     }
     return out;
 }
 
 // The fast half of one word's decision, and what resolving its ties needs.
 struct WordDecision {
-    std::uint32_t C, S, wa, wb, out;
-    std::uint64_t tie;   // lanes in which some byte's high half ties
+    std::uint32_t C, A, wa, wb, out;
+    std::uint32_t zmin;   // a zero half: some site's high half equals its threshold
 };
 
-// Decide the four sites of one 4b-form word C with neighbour counts S; wa, wb
-// are the high-half Philox words of these sites.
-__device__ __forceinline__ WordDecision decide_fast(std::uint32_t C, std::uint32_t S, std::uint32_t wa,
+// Decide the four sites of one 4b-form word C (0 / 4 per byte) whose bytes of
+// A hold 4 a (a: anti-aligned neighbours); wa, wb are the high-half Philox
+// words of sites 0, 1 and 2, 3.  u < t on the high halves is h < T, as a
+// signed comparison of h ^ 0x8000 and T ^ 0x8000: the saturating packed
+// difference D is negative exactly then and zero exactly on a tie.  Byte k of
+// A + (A >> 5) is 4 (a_k + 8 a_k+1) for k = 0, 2: the byte address of the
+// pair table entry of sites (0, 1) and (2, 3).  The signs of the four halves
+// of the two differences go to the four bytes with one v_perm (selectors
+// 8-11 replicate the sign bits of the half-words).
+__device__ __forceinline__ WordDecision decide_fast(std::uint32_t C, std::uint32_t A, std::uint32_t wa,
                                                     std::uint32_t wb, const FusedTables& tb)
 {
-    WordDecision d{C, S, wa, wb, 0u, 0u};
-    const std::uint32_t Cf = C ^ 0x04040404u;
-    std::uint32_t t[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        // table byte address S_k + 256 C_k: S_k in byte 0, C_k in byte 1, one v_perm
-        const std::uint32_t addr = __builtin_amdgcn_perm(C, S, 0x0C0C0000u | ((4u + k) << 8) | k);
-        t[k] = *reinterpret_cast<const std::uint32_t*>(reinterpret_cast<const char*>(tb.hi) + addr);
-    }
-    d.out = select_lt(C, Cf, wa, wb, t);
-    d.tie = ties(wa, wb, t);
+    WordDecision d{C, A, wa, wb, 0u, 0u};
+    const std::uint32_t Y = A + (A >> 5);
+    const char* base = reinterpret_cast<const char*>(tb.pair);
+    const std::uint32_t t01 = *reinterpret_cast<const std::uint32_t*>(base + (Y & 0xFFu));
+    const std::uint32_t t23 = *reinterpret_cast<const std::uint32_t*>(base + ((Y >> 16) & 0xFFu));
+    const std::uint32_t Da = pk_sub_sat(wa ^ 0x80008000u, t01);
+    const std::uint32_t Db = pk_sub_sat(wb ^ 0x80008000u, t23);
+    const std::uint32_t m = __builtin_amdgcn_perm(Db, Da, 0x0B0A0908u);   // 0xFF where h < T
+    d.out = C ^ (m & 0x04040404u);
+    d.zmin = pk_min_u16(Da, Db);
     return d;
 }
+
+// Whether some half of z is zero.
+__device__ __forceinline__ bool has_zero_half(std::uint32_t z) { return (z & 0xFFFFu) == 0u || (z >> 16) == 0u; }
 
 template <int H, bool COUNT>
 __global__ __launch_bounds__(256) void ising_fused(FusedArgs a)
@@ -310,14 +302,19 @@ __global__ __launch_bounds__(256) void ising_fused(FusedArgs a)
     for (int j = 1; j <= kEarly; ++j) philox_row(j);
 
     if (threadIdx.x == 0) {
+        // t_a: a +1 site with a anti-aligned (-1) neighbours
+        std::uint32_t th[5];
 #pragma unroll
-        for (int n = 0; n < 5; ++n)
+        for (int n = 0; n < 5; ++n) {
+            const std::uint32_t t = a.tm1[n];
+            th[n] = t >> 16;
+            tb.hi[n] = t >> 16;
+            tb.lo[n] = t & 0xFFFFu;
+        }
 #pragma unroll
-            for (int c = 0; c < 2; ++c) {
-                const std::uint32_t t = a.tm1[8 * c + n];
-                tb.hi[256 * c + n] = t == 0xFFFFFFFFu ? 0x10000u : t >> 16;
-                tb.lo[256 * c + n] = t == 0xFFFFFFFFu ? 0u : t & 0xFFFFu;
-            }
+        for (int a1 = 0; a1 < 5; ++a1)
+#pragma unroll
+            for (int a0 = 0; a0 < 5; ++a0) tb.pair[a0 + 8 * a1] = (th[a0] ^ 0x8000u) | ((th[a1] ^ 0x8000u) << 16);
     }
     __syncthreads();   // the loads stay in flight across it
 
@@ -333,13 +330,14 @@ __global__ __launch_bounds__(256) void ising_fused(FusedArgs a)
         const std::uint32_t Y = hY;
         auto b4 = [](std::uint32_t v) { return v & 4u; };   // +1 (0x01) -> 0, -1 (0xFF) -> 4
         auto update0 = [&](std::uint32_t x, std::uint32_t c, std::uint32_t s) -> std::uint32_t {
-            // colour-0 update of site (x, Y) from its unmodified neighbours
-            const std::uint32_t idx = (c << 6) + (s >> 2);
-            const std::uint32_t th = tb.hi[idx];
+            // colour-0 update of site (x, Y) from its unmodified neighbours; s =
+            // 4 * # of -1 neighbours, an = # of anti-aligned ones
+            const std::uint32_t an = (c ? 16u - s : s) >> 2;
+            const std::uint32_t th = tb.hi[an];
             const std::uint32_t i = Y * a.Lx + x, q = (i >> 3) & 63u, r = i >> 9, j = i & 7u;
             const std::uint32_t h16 = ising_half(philox(U4{q, r, kTagIsingHi, 0u}, a.key), j);
             bool acc = h16 < th;
-            if (h16 == th) acc = ising_half(philox(U4{q, r, kTagIsingLo, 0u}, a.key), j) <= tb.lo[idx];
+            if (h16 == th) acc = ising_half(philox(U4{q, r, kTagIsingLo, 0u}, a.key), j) <= tb.lo[an];
             return acc ? (c ^ 4u) : c;
         };
         const bool even = (Y & 1u) == 0;
@@ -355,31 +353,34 @@ __global__ __launch_bounds__(256) void ising_fused(FusedArgs a)
     if (!active) return;
 
     std::uint32_t acc = 0;
-    // neighbour counts of the even-x half of row j (odd half of the row, with the
-    // left halo for lane 0, even halves of rows j - 1, j + 1) and of the odd-x half
-    auto sumE = [&](const Row (&R)[H + 4], int j) {
+    // anti-aligned neighbour counts (4 a per byte) of the even-x half of row j
+    // (neighbours: the odd half of the row, with the left halo for lane 0, and
+    // the even halves of rows j - 1, j + 1) and of the odd-x half
+    auto antiE = [&](const Row (&R)[H + 4], int j) {
         const std::uint32_t Oprev = from_prev_lane(R[j].O, haloL[wid][j - 1]);
         const std::uint32_t Lft = __builtin_amdgcn_alignbit(R[j].O, Oprev, 24);
-        return Lft + R[j].O + R[j - 1].E + R[j + 1].E;
+        const std::uint32_t C = R[j].E;
+        return ((Lft ^ C) + (R[j].O ^ C)) + ((R[j - 1].E ^ C) + (R[j + 1].E ^ C));
     };
-    auto sumO = [&](const Row (&R)[H + 4], int j) {
+    auto antiO = [&](const Row (&R)[H + 4], int j) {
         const std::uint32_t Enext = from_next_lane(R[j].E, haloR[wid][j - 1]);
         const std::uint32_t Rgt = __builtin_amdgcn_alignbit(Enext, R[j].E, 8);
-        return R[j].E + Rgt + R[j - 1].O + R[j + 1].O;
+        const std::uint32_t C = R[j].O;
+        return ((R[j].E ^ C) + (Rgt ^ C)) + ((R[j - 1].O ^ C) + (R[j + 1].O ^ C));
     };
     // y0 is even, so window row j has the parity of j: colour 0 is the even-x
     // half on even j and the odd-x half on odd j.
     auto fast0 = [&](const Row (&R)[H + 4], int j) {
-        return (j & 1) ? decide_fast(R[j].O, sumO(R, j), P[j].z, P[j].w, tb)
-                       : decide_fast(R[j].E, sumE(R, j), P[j].x, P[j].y, tb);
+        return (j & 1) ? decide_fast(R[j].O, antiO(R, j), P[j].z, P[j].w, tb)
+                       : decide_fast(R[j].E, antiE(R, j), P[j].x, P[j].y, tb);
     };
     auto fast1 = [&](const Row (&R)[H + 4], int j) {
-        return (j & 1) ? decide_fast(R[j].E, sumE(R, j), P[j].x, P[j].y, tb)
-                       : decide_fast(R[j].O, sumO(R, j), P[j].z, P[j].w, tb);
+        return (j & 1) ? decide_fast(R[j].E, antiE(R, j), P[j].x, P[j].y, tb)
+                       : decide_fast(R[j].O, antiO(R, j), P[j].z, P[j].w, tb);
     };
     auto resolve = [&](WordDecision& d, int j, bool odd_half) {
-        d.out = resolve_ties(d.out, d.C, d.S, d.wa, d.wb, &tb, a.key, lane, rowY(j) * a.bands + band,
-                             odd_half);
+        if (has_zero_half(d.zmin))
+            d.out = resolve_ties(d.out, d.A, d.wa, d.wb, &tb, a.key, lane, rowY(j) * a.bands + band, odd_half);
     };
     // Rows 1 and H + 2 (y0 - 1 and y0 + H) get their colour-0 update only as
     // neighbours of the block's own rows; their owners update, store and count them.
@@ -408,7 +409,7 @@ __global__ __launch_bounds__(256) void ising_fused(FusedArgs a)
 #pragma unroll
     for (int j = 1; j <= 3; ++j) {
         WordDecision d = fast0(R, j);
-        if (d.tie) resolve(d, j, j & 1);
+        if (__ballot(has_zero_half(d.zmin))) resolve(d, j, j & 1);
         put0(R, j, d);
     }
 #pragma unroll
@@ -418,15 +419,15 @@ __global__ __launch_bounds__(256) void ising_fused(FusedArgs a)
             if (j + 3 <= H + 3 && j + 3 >= 5) R[j + 3] = convert_row(raw[j + 3]);
             WordDecision d0 = fast0(R, j + 2);
             WordDecision d1 = fast1(R, j);
-            if (d0.tie | d1.tie) {
-                if (d0.tie) resolve(d0, j + 2, (j + 2) & 1);
-                if (d1.tie) resolve(d1, j, !(j & 1));
+            if (__ballot(has_zero_half(pk_min_u16(d0.zmin, d1.zmin)))) {
+                resolve(d0, j + 2, (j + 2) & 1);
+                resolve(d1, j, !(j & 1));
             }
             put0(R, j + 2, d0);
             put1(R, j, d1);
         } else {
             WordDecision d1 = fast1(R, j);
-            if (d1.tie) resolve(d1, j, !(j & 1));
+            if (__ballot(has_zero_half(d1.zmin))) resolve(d1, j, !(j & 1));
             put1(R, j, d1);
         }
         store_row(a.out, row_words, rowY(j), col, R[j]);
