@@ -47,3 +47,18 @@ def test_no_device_is_reported_not_hidden():
         with pytest.raises(abi.UcmcError) as e:
             abi.System.ising(off, bonds, 1)
         assert e.value.code == abi.UCMC_E_NO_DEVICE
+
+
+def test_sincos_turn_error_bound(tmp_path):
+    """The fp32 sin / cos of 2 pi t the O(N) and XY proposals use (csrc/turns.hpp,
+    exact reduction in turns + Taylor polynomials) stays within 2e-7 of double
+    sin / cos over the draws t = k 2^-32 and over angles in [-2, 2] turns."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = tmp_path / "turns_check"
+    subprocess.run(["g++", "-std=c++17", "-O2", "-I" + os.path.join(root, "universal_classical_montecarlo_amd", "csrc"),
+                    os.path.join(root, "tests", "cpp", "turns_check.cpp"), "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout

@@ -1,0 +1,52 @@
+// turns.hpp -- sin and cos of 2 pi t in fp32 for the O(N) proposals, with the
+// reduction done exactly in turns: 4 t splits into a quadrant q and a fraction
+// f in [0, 1) without rounding (t is a float in [0, 1)), f folds to [0, 1/2]
+// by symmetry, and sin / cos of x = f pi / 2 <= pi / 4 come from their Taylor
+// series through x^9 and x^8 (remainders x^11 / 11! < 1.8e-9 and x^10 / 10! <
+// 2.5e-8 on that range) in Horner form.  Host-compilable so that the CPU test
+// suite can bound its error against double sin / cos.
+#pragma once
+
+#include <cmath>
+
+#if defined(__HIPCC__)
+#define UCMC_HD __host__ __device__
+#else
+#define UCMC_HD
+#endif
+
+namespace ucmc {
+
+// sin(2 pi t), cos(2 pi t) for t in [0, 1]
+UCMC_HD inline void sincos_turn(float t, float* s, float* c)
+{
+    const float t4 = 4.0f * t;                       // exact
+    const int q = static_cast<int>(t4);              // 0 .. 4
+    const float f = t4 - static_cast<float>(q);      // exact, [0, 1)
+    const bool hi = f > 0.5f;
+    const float r = hi ? 1.0f - f : f;               // exact, [0, 1/2]
+    const float x = r * 1.57079632679489661923f;     // [0, pi/4]
+    const float x2 = x * x;
+    const float sp = x + x * x2 * (-1.0f / 6.0f +
+                                   x2 * (1.0f / 120.0f + x2 * (-1.0f / 5040.0f + x2 * (1.0f / 362880.0f))));
+    const float cp = 1.0f + x2 * (-0.5f + x2 * (1.0f / 24.0f + x2 * (-1.0f / 720.0f + x2 * (1.0f / 40320.0f))));
+    // sin and cos of f pi / 2
+    const float sf = hi ? cp : sp, cf = hi ? sp : cp;
+    switch (q & 3) {
+    case 0: *s = sf; *c = cf; break;
+    case 1: *s = cf; *c = -sf; break;
+    case 2: *s = -sf; *c = -cf; break;
+    default: *s = -cf; *c = sf; break;
+    }
+}
+
+// sin(2 pi t), cos(2 pi t) for any finite t (the fraction of t first)
+UCMC_HD inline void sincos_turn_any(float t, float* s, float* c)
+{
+    // on |t|, where |t| - floor(|t|) is exact (Sterbenz), then sin(-x) = -sin x
+    const float a = fabsf(t);
+    sincos_turn(a - floorf(a), s, c);
+    if (t < 0.0f) *s = -*s;
+}
+
+}  // namespace ucmc

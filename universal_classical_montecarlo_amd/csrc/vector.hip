@@ -15,6 +15,7 @@
 #include <cstring>
 #include <vector>
 
+#include "turns.hpp"
 #include "ucmc_core.hpp"
 #include "ucmc_device.hpp"
 
@@ -37,12 +38,20 @@ struct VecArgs {
     std::uint64_t key;
     float J;
     double beta;
-    float delta;
+    float delta_turn;               // rotation width / (2 pi)
     int kind;                       // 0 sphere, 1 rotation
     unsigned long long* accepts;
     float* trace_dE;
     std::uint8_t* trace_acc;
 };
+
+// Component k of site i of the SoA field through a 32-bit byte offset (the
+// field is below 4 GiB, checked at setup), so the loads take the uniform base
+// and a per-lane offset.
+__device__ __forceinline__ float& fld(float* f, std::uint32_t n, int k, std::uint32_t i)
+{
+    return *reinterpret_cast<float*>(reinterpret_cast<char*>(f) + (static_cast<std::uint32_t>(k) * n + i) * 4u);
+}
 
 // u in [0, 1) as float, from a 32-bit draw k: (float)(k * 2^-32) may round to
 // 1.0f for k > 2^32 - 2^7; the affine maps below stay in range either way.
@@ -52,10 +61,9 @@ __device__ __forceinline__ float uf(std::uint32_t k) { return static_cast<float>
 template <int N>
 __device__ __forceinline__ void sphere(Stream& r, float* out)
 {
-    if constexpr (N == 2) {
-        const float phi = static_cast<float>(2.0 * kPi) * uf(r.next());
+    if constexpr (N == 2) {   // phi = 2 pi u
         float s, c;
-        sincosf(phi, &s, &c);
+        sincos_turn(uf(r.next()), &s, &c);
         out[0] = c;
         out[1] = s;
     } else {
@@ -101,10 +109,9 @@ struct Words8 {
 template <int N, int K0>
 __device__ __forceinline__ void sphere_words(const Words8& wd, float* out)
 {
-    if constexpr (N == 2) {
-        const float phi = static_cast<float>(2.0 * kPi) * uf(wd.template w<K0>());
+    if constexpr (N == 2) {   // phi = 2 pi u
         float s, c;
-        sincosf(phi, &s, &c);
+        sincos_turn(uf(wd.template w<K0>()), &s, &c);
         out[0] = c;
         out[1] = s;
     } else {
@@ -154,13 +161,14 @@ __device__ __forceinline__ bool orthonormalise(const float* u, float* v)
     return true;
 }
 
+// The rotation by theta = delta (2u - 1) in the (u, v) plane; delta_turn =
+// delta / (2 pi), so theta in turns is delta_turn (2u - 1).
 template <int N>
-__device__ __forceinline__ void rotate_by(float delta, std::uint32_t wth, const float* u, const float* v,
+__device__ __forceinline__ void rotate_by(float delta_turn, std::uint32_t wth, const float* u, const float* v,
                                           const float* cur, float* out)
 {
-    const float th = delta * (2.0f * uf(wth) - 1.0f);
     float sn, c;
-    sincosf(th, &sn, &c);
+    sincos_turn_any(delta_turn * (2.0f * uf(wth) - 1.0f), &sn, &c);
     float cu = 0.f, cv = 0.f;
 #pragma unroll
     for (int k = 0; k < N; ++k) {
@@ -177,7 +185,7 @@ __device__ __forceinline__ void rotate_by(float delta, std::uint32_t wth, const 
 // reference's draw order.  The common path reads the first 2N words from
 // registers; a rejected frame (probability ~1e-7) continues on the stream.
 template <int N>
-__device__ __forceinline__ std::uint32_t propose(int kind, float delta, const float* cur, std::uint64_t key,
+__device__ __forceinline__ std::uint32_t propose(int kind, float delta_turn, const float* cur, std::uint64_t key,
                                                  std::uint32_t i, float* p)
 {
     Words8 wd;
@@ -192,7 +200,7 @@ __device__ __forceinline__ std::uint32_t propose(int kind, float delta, const fl
     sphere_words<N, 0>(wd, u);
     sphere_words<N, N - 1>(wd, v);
     if (orthonormalise<N>(u, v)) {
-        rotate_by<N>(delta, wd.template w<2 * N - 2>(), u, v, cur, p);
+        rotate_by<N>(delta_turn, wd.template w<2 * N - 2>(), u, v, cur, p);
         return wd.template w<2 * N - 1>();
     }
     Stream r;   // the stream after the 2N - 2 words used so far
@@ -200,7 +208,7 @@ __device__ __forceinline__ std::uint32_t propose(int kind, float delta, const fl
     for (int k = 0; k < 2 * N - 2; ++k) (void)r.next();
     do sphere<N>(r, v);
     while (!orthonormalise<N>(u, v));
-    rotate_by<N>(delta, r.next(), u, v, cur, p);
+    rotate_by<N>(delta_turn, r.next(), u, v, cur, p);
     return r.next();
 }
 
@@ -241,23 +249,23 @@ __global__ __launch_bounds__(256) void vector_colour(VecArgs a)
         float s[N], p[N], h[N];
 #pragma unroll
         for (int k = 0; k < N; ++k) {
-            s[k] = a.f[k * a.n + i];
+            s[k] = fld(a.f, a.n, k, i);
             h[k] = 0.f;
         }
         if constexpr (CUBIC) {
 #pragma unroll
             for (int q = 0; q < 6; ++q)
 #pragma unroll
-                for (int k = 0; k < N; ++k) h[k] += a.f[k * a.n + nb[q]];
+                for (int k = 0; k < N; ++k) h[k] += fld(a.f, a.n, k, nb[q]);
         } else {
             for (std::uint32_t q = a.offsets[i]; q < a.offsets[i + 1]; ++q) {
                 const std::uint32_t j = a.bonds[q];
 #pragma unroll
-                for (int k = 0; k < N; ++k) h[k] += a.f[k * a.n + j];
+                for (int k = 0; k < N; ++k) h[k] += fld(a.f, a.n, k, j);
             }
         }
         (void)deg;
-        const std::uint32_t waccept = propose<N>(a.kind, a.delta, s, a.key, i, p);
+        const std::uint32_t waccept = propose<N>(a.kind, a.delta_turn, s, a.key, i, p);
         // example/vector_spin.hpp:50-60: -J sum_j (p.s_j - s.s_j) = -J (p - s).h
         float d = 0.f;
 #pragma unroll
@@ -272,7 +280,7 @@ __global__ __launch_bounds__(256) void vector_colour(VecArgs a)
             for (int k = 0; k < N; ++k) nn += p[k] * p[k];
             const float inv = rsqrtf(nn);
 #pragma unroll
-            for (int k = 0; k < N; ++k) a.f[k * a.n + i] = p[k] * inv;
+            for (int k = 0; k < N; ++k) fld(a.f, a.n, k, i) = p[k] * inv;
         }
         if constexpr (TRACE) {
             a.trace_dE[i] = dE;
@@ -342,7 +350,7 @@ void launch_colour(System& s, const ucmc_metropolis_params& p, std::uint64_t key
     a.key = key;
     a.J = static_cast<float>(s.J);
     a.beta = p.beta;
-    a.delta = static_cast<float>(p.width);
+    a.delta_turn = static_cast<float>(p.width / (2.0 * kPi));
     a.kind = p.proposal == UCMC_PROP_SPHERE ? 0 : 1;
     a.accepts = dacc;
     a.trace_dE = tdE;
@@ -440,6 +448,8 @@ void vector_setup(System& s)
         s.schedule = UCMC_SCHED_CSR;
     }
     upload_colour_lists(s);
+    // the Metropolis kernel addresses the field with 32-bit byte offsets
+    if (s.sites * s.N * sizeof(float) > 0xFFFFFFFFull) fail(UCMC_E_OVERFLOW, "O(N) field exceeds 4 GiB");
     s.field[0].alloc(s.sites * s.N * sizeof(float));
     // example/vector_spin.hpp:27-28: every spin along the last axis
     std::vector<float> init(s.sites * s.N, 0.f);

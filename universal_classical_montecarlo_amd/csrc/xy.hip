@@ -23,6 +23,7 @@
 #include <cstring>
 #include <vector>
 
+#include "turns.hpp"
 #include "ucmc_core.hpp"
 #include "ucmc_device.hpp"
 
@@ -31,6 +32,7 @@ namespace ucmc {
 namespace {
 
 constexpr float kTwoPiF = 6.283185307179586f;
+constexpr float kInvTwoPiF = 0.15915494309189535f;   // angles to turns for sincos_turn_any
 
 __device__ __forceinline__ float uf(std::uint32_t k) { return static_cast<float>(k) * 0x1.0p-32f; }
 
@@ -186,7 +188,7 @@ __global__ __launch_bounds__(256) void pt_sweep(PtArgs a)
     for (std::uint32_t q = threadIdx.x; q < W * W; q += blockDim.x) {
         const std::uint32_t ly = q / W, lx = q % W;
         float sv, cv;
-        sincosf(in[static_cast<std::size_t>(wy(ly)) * L + wx(lx)], &sv, &cv);
+        sincos_turn_any(in[static_cast<std::size_t>(wy(ly)) * L + wx(lx)] * kInvTwoPiF, &sv, &cv);
         cs[q] = cv;
         sn[q] = sv;
     }
@@ -218,7 +220,7 @@ __global__ __launch_bounds__(256) void pt_sweep(PtArgs a)
             rg.init(key, gi, kTagSite);
             const float tp = u1_shift(a.width, ti, rg.next());
             float sp, cp;
-            sincosf(tp, &sp, &cp);
+            sincos_turn_any(tp * kInvTwoPiF, &sp, &cp);
             const float dE = -a.J * ((cp - cs[w]) * C + (sp - sn[w]) * S);
             const double log_p = -beta * static_cast<double>(dE) + 0.0;
             const bool acc = log_p >= 0.0 || accept_draw(rg.next(), log_p);
