@@ -42,6 +42,14 @@
 #ifndef UCMC_PHILOX_EARLY
 #define UCMC_PHILOX_EARLY 0
 #endif
+// Steps between the load of a row and its first use, for 8-row and for
+// 16-row blocks (64: every row up front).
+#ifndef UCMC_ROWS_AHEAD_8
+#define UCMC_ROWS_AHEAD_8 1
+#endif
+#ifndef UCMC_ROWS_AHEAD_16
+#define UCMC_ROWS_AHEAD_16 64
+#endif
 
 namespace ucmc {
 
@@ -282,9 +290,14 @@ __global__ __launch_bounds__(256) void ising_fused(FusedArgs a)
         hv[9] = inb[rd + xr];
     }
 
+    // rows 0 .. kUpfront - 1 now; row j + 3 + kRowsAhead at step j of the sweep
+    // loop (row j + 3 is converted at step j), so the loads of later rows
+    // overlap the decisions of earlier ones
+    constexpr int kRowsAhead = H <= 8 ? UCMC_ROWS_AHEAD_8 : UCMC_ROWS_AHEAD_16;
+    constexpr int kUpfront = 5 + kRowsAhead < H + 4 ? 5 + kRowsAhead : H + 4;
     uint2 raw[H + 4];
 #pragma unroll
-    for (int j = 0; j < H + 4; ++j) raw[j] = load_raw(a.in, row_words, rowY(j), col);
+    for (int j = 0; j < kUpfront; ++j) raw[j] = load_raw(a.in, row_words, rowY(j), col);
 
     // high-half Philox words of window row j: (x, y) even-x sites, (z, w) odd-x
     U4 P[H + 4];
@@ -414,6 +427,8 @@ __global__ __launch_bounds__(256) void ising_fused(FusedArgs a)
     }
 #pragma unroll
     for (int j = 2; j <= H + 1; ++j) {
+        if (j + 3 + kRowsAhead >= kUpfront && j + 3 + kRowsAhead <= H + 3)
+            raw[j + 3 + kRowsAhead] = load_raw(a.in, row_words, rowY(j + 3 + kRowsAhead), col);
         if (j + 2 + kAhead > kFirst && j + 2 + kAhead <= H + 2) philox_row(j + 2 + kAhead);
         if (j + 2 <= H + 2) {
             if (j + 3 <= H + 3 && j + 3 >= 5) R[j + 3] = convert_row(raw[j + 3]);
