@@ -30,7 +30,7 @@
 #include "ucmc_core.hpp"
 #include "ucmc_device.hpp"
 
-// Rows per wave block of the fused kernel (even, <= 62), and how many rows
+// Rows per wave block of the fused kernel (even, <= 30), and how many rows
 // ahead of their use its Philox words are computed.
 #ifndef UCMC_FUSED_H
 #define UCMC_FUSED_H 16
@@ -49,6 +49,10 @@
 #endif
 #ifndef UCMC_ROWS_AHEAD_16
 #define UCMC_ROWS_AHEAD_16 64
+#endif
+// 16-row blocks from this many waves per CU on.
+#ifndef UCMC_FUSED_BIG_WAVES
+#define UCMC_FUSED_BIG_WAVES 16
 #endif
 
 namespace ucmc {
@@ -222,7 +226,7 @@ __device__ __forceinline__ bool has_zero_half(std::uint32_t z) { return (z & 0xF
 template <int H, bool COUNT>
 __global__ __launch_bounds__(256) void ising_fused(FusedArgs a)
 {
-    static_assert(H % 2 == 0 && H + 2 <= 64, "block height: even, halo rows fit one wave");
+    static_assert(H % 2 == 0 && H + 2 <= 32, "block height: even, halo rows fit half a wave");
     __shared__ FusedTables tb;
     __shared__ std::uint32_t haloL[4][H + 2];
     __shared__ std::uint32_t haloR[4][H + 2];
@@ -253,41 +257,39 @@ __global__ __launch_bounds__(256) void ising_fused(FusedArgs a)
         return static_cast<std::uint32_t>(t);
     };
 
-    // ---- halo pre-pass: one lane per window row 1 .. H + 2 (y0-1 .. y0+H) ----
-    // Lane l owns row Y = y0 - 1 + l and the two sites just outside the band,
-    // xl = x_band - 1 (odd) and xr = x_band + 512 (even).  Its ten bytes are
-    // requested first, then every row of the window: loads complete in order,
-    // so the pre-pass waits for its own bytes only and the sweep below for one
-    // row at a time.
+    // ---- halo pre-pass: one lane per window row 1 .. H + 2 (y0-1 .. y0+H) and side ----
+    // Lanes 0 .. H + 1 own the site just left of the band, x_l = x_band - 1
+    // (odd), of row Y = y0 - 1 + l; lanes 32 .. 33 + H the site just right of
+    // it, x_r = x_band + 512 (even), of row y0 - 1 + (l - 32).  Each lane's
+    // bytes are requested first, then every row of the window: loads complete
+    // in order, so the pre-pass waits for its own bytes only and the sweep
+    // below for one row at a time.
     const std::uint8_t* inb = reinterpret_cast<const std::uint8_t*>(a.in);
-    const bool halo_lane = active && lane < static_cast<std::uint32_t>(H + 2);
-    std::uint32_t hY = 0, hv[10] = {};
-    const std::uint32_t xb = band * 512u;
-    const std::uint32_t xl = xb == 0 ? a.Lx - 1u : xb - 1u;   // odd, >= 1
-    const std::uint32_t xr = xb + 512u == a.Lx ? 0u : xb + 512u;
+    const bool right_lane = lane >= 32u;
+    const std::uint32_t hrow = lane & 31u;
+    const bool halo_lane = active && hrow < static_cast<std::uint32_t>(H + 2);
+    std::uint32_t hY = 0, hx = 0, hpair = 0, hside = 0, hup = 0, hdn = 0;
     if (halo_lane) {
-        std::uint32_t Y = y0 + lane + a.Ly - 1u;               // < 3 Ly
+        const std::uint32_t xb = band * 512u;
+        // x_l = x_band - 1 or x_r = x_band + 512, periodic
+        const std::uint32_t x = right_lane ? (xb + 512u == a.Lx ? 0u : xb + 512u)
+                                           : (xb == 0 ? a.Lx - 1u : xb - 1u);
+        // the other horizontal neighbour: x_l + 1 or x_r - 1, periodic
+        const std::uint32_t xo = right_lane ? (x == 0 ? a.Lx - 1u : x - 1u) : (x + 1u == a.Lx ? 0u : x + 1u);
+        std::uint32_t Y = y0 + hrow + a.Ly - 1u;               // < 3 Ly
         if (Y >= a.Ly) Y -= a.Ly;
         if (Y >= a.Ly) Y -= a.Ly;
         hY = Y;
+        hx = x;
         const std::uint32_t Yu = Y == 0 ? a.Ly - 1u : Y - 1u;
         const std::uint32_t Yd = Y + 1u == a.Ly ? 0u : Y + 1u;
-        const std::uint32_t xlr = xl + 1u == a.Lx ? 0u : xl + 1u;
-        const std::uint32_t xrl = xr == 0 ? a.Lx - 1u : xr - 1u;
         const std::size_t r0 = static_cast<std::size_t>(Y) * a.Lx;
-        const std::size_t ru = static_cast<std::size_t>(Yu) * a.Lx;
-        const std::size_t rd = static_cast<std::size_t>(Yd) * a.Lx;
-        // left site: (left, centre) as one aligned 16-bit load, right, up, down;
-        // right site: (centre, right) as one 16-bit load, left, up, down.  They
-        // are split only after the row loads are issued (below).
-        hv[0] = *reinterpret_cast<const std::uint16_t*>(inb + r0 + xl - 1u);
-        hv[2] = inb[r0 + xlr];
-        hv[3] = inb[ru + xl];
-        hv[4] = inb[rd + xl];
-        hv[5] = *reinterpret_cast<const std::uint16_t*>(inb + r0 + xr);
-        hv[6] = inb[r0 + xrl];
-        hv[8] = inb[ru + xr];
-        hv[9] = inb[rd + xr];
+        // the aligned byte pair holding x: (x_l - 1, x_l) or (x_r, x_r + 1); it
+        // is split only after the row loads are issued (below)
+        hpair = *reinterpret_cast<const std::uint16_t*>(inb + r0 + (x & ~1u));
+        hside = inb[r0 + xo];
+        hup = inb[static_cast<std::size_t>(Yu) * a.Lx + x];
+        hdn = inb[static_cast<std::size_t>(Yd) * a.Lx + x];
     }
 
     // rows 0 .. kUpfront - 1 now; row j + 3 + kRowsAhead at step j of the sweep
@@ -334,33 +336,26 @@ __global__ __launch_bounds__(256) void ising_fused(FusedArgs a)
     if (halo_lane) {
         // an empty volatile asm that takes the bytes: nothing that reads them is
         // hoisted above the row loads (their wait here is vmcnt(H + 4))
-        asm volatile("" : "+v"(hv[0]), "+v"(hv[2]), "+v"(hv[3]), "+v"(hv[4]), "+v"(hv[5]), "+v"(hv[6]),
-                          "+v"(hv[8]), "+v"(hv[9]));
-        hv[1] = hv[0] & 0xFFu;   // x_l - 1
-        hv[0] >>= 8;             // x_l
-        hv[7] = hv[5] >> 8;      // x_r + 1
-        hv[5] &= 0xFFu;          // x_r
-        const std::uint32_t Y = hY;
+        asm volatile("" : "+v"(hpair), "+v"(hside), "+v"(hup), "+v"(hdn));
+        // x_l is the high byte of its pair, x_r the low one
+        const std::uint32_t cb = right_lane ? (hpair & 0xFFu) : (hpair >> 8);
+        const std::uint32_t nb = right_lane ? (hpair >> 8) : (hpair & 0xFFu);
         auto b4 = [](std::uint32_t v) { return v & 4u; };   // +1 (0x01) -> 0, -1 (0xFF) -> 4
-        auto update0 = [&](std::uint32_t x, std::uint32_t c, std::uint32_t s) -> std::uint32_t {
-            // colour-0 update of site (x, Y) from its unmodified neighbours; s =
-            // 4 * # of -1 neighbours, an = # of anti-aligned ones
-            const std::uint32_t an = (c ? 16u - s : s) >> 2;
-            const std::uint32_t th = tb.hi[an];
-            const std::uint32_t i = Y * a.Lx + x, q = (i >> 3) & 63u, r = i >> 9, j = i & 7u;
-            const std::uint32_t h16 = ising_half(philox(U4{q, r, kTagIsingHi, 0u}, a.key), j);
-            bool acc = h16 < th;
-            if (h16 == th) acc = ising_half(philox(U4{q, r, kTagIsingLo, 0u}, a.key), j) <= tb.lo[an];
-            return acc ? (c ^ 4u) : c;
-        };
-        const bool even = (Y & 1u) == 0;
-        const std::uint32_t lc = b4(hv[0]), rc = b4(hv[5]);
-        const std::uint32_t lnew = update0(xl, lc, b4(hv[1]) + b4(hv[2]) + b4(hv[3]) + b4(hv[4]));
-        const std::uint32_t rnew = update0(xr, rc, b4(hv[6]) + b4(hv[7]) + b4(hv[8]) + b4(hv[9]));
+        // colour-0 update of site (x, Y) from its unmodified neighbours: s =
+        // 4 * # of -1 neighbours, an = # of anti-aligned ones
+        const std::uint32_t c = b4(cb), sm = b4(nb) + b4(hside) + b4(hup) + b4(hdn);
+        const std::uint32_t an = (c ? 16u - sm : sm) >> 2;
+        const std::uint32_t th = tb.hi[an];
+        const std::uint32_t i = hY * a.Lx + hx, q = (i >> 3) & 63u, r = i >> 9, j = i & 7u;
+        const std::uint32_t h16 = ising_half(philox(U4{q, r, kTagIsingHi, 0u}, a.key), j);
+        bool acc = h16 < th;
+        if (h16 == th) acc = ising_half(philox(U4{q, r, kTagIsingLo, 0u}, a.key), j) <= tb.lo[an];
+        const std::uint32_t cnew = acc ? (c ^ 4u) : c;
         // left (odd x): even rows need the old value, odd rows the colour-0 value;
         // right (even x): even rows need the colour-0 value, odd rows the old one
-        haloL[wid][lane] = (even ? lc : lnew) << 24;
-        haloR[wid][lane] = even ? rnew : rc;
+        const bool even = (hY & 1u) == 0;
+        if (right_lane) haloR[wid][hrow] = even ? cnew : c;
+        else haloL[wid][hrow] = (even ? c : cnew) << 24;
     }
     __syncthreads();
     if (!active) return;
@@ -519,7 +514,7 @@ int fused_rows(System& s)
 {
     if (s.cus == 0) UCMC_HIP(hipDeviceGetAttribute(&s.cus, hipDeviceAttributeMultiprocessorCount, s.device));
     const std::size_t waves16 = (s.Lx / 512) * (s.Ly / kFusedH);
-    const bool big = s.Ly % kFusedH == 0 && waves16 >= static_cast<std::size_t>(s.cus) * 16;
+    const bool big = s.Ly % kFusedH == 0 && waves16 >= static_cast<std::size_t>(s.cus) * UCMC_FUSED_BIG_WAVES;
     return big ? kFusedH : kFusedHSmall;
 }
 
