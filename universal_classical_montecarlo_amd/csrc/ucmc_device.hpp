@@ -63,15 +63,26 @@ __device__ __forceinline__ U4 philox_round(U4 c, std::uint32_t k0, std::uint32_t
               xor3(static_cast<std::uint32_t>(p0 >> 32), c.w, k1), static_cast<std::uint32_t>(p0)};
 }
 
+// The first round XORs with plain operators: counters whose words y and z are
+// wave-uniform (a row or stream index and a tag) then give a uniform word x,
+// which the second round multiplies on the scalar unit; the bitop3 builtin
+// would pin both to vector registers (two quarter-rate multiplies per call).
+__device__ __forceinline__ U4 philox_first_round(U4 c, std::uint32_t k0, std::uint32_t k1)
+{
+    const std::uint64_t p0 = static_cast<std::uint64_t>(kPhiloxM0) * c.x;
+    const std::uint64_t p1 = static_cast<std::uint64_t>(kPhiloxM1) * c.z;
+    return U4{static_cast<std::uint32_t>(p1 >> 32) ^ c.y ^ k0, static_cast<std::uint32_t>(p1),
+              static_cast<std::uint32_t>(p0 >> 32) ^ c.w ^ k1, static_cast<std::uint32_t>(p0)};
+}
+
 __device__ __forceinline__ U4 philox(U4 c, std::uint64_t key)
 {
     std::uint32_t k0 = static_cast<std::uint32_t>(key), k1 = static_cast<std::uint32_t>(key >> 32);
+    c = philox_first_round(c, k0, k1);
 #pragma unroll
-    for (int r = 0; r < 10; ++r) {
-        if (r) {
-            k0 += kPhiloxW0;
-            k1 += kPhiloxW1;
-        }
+    for (int r = 1; r < 10; ++r) {
+        k0 += kPhiloxW0;
+        k1 += kPhiloxW1;
         c = philox_round(c, k0, k1);
     }
     return c;
