@@ -35,6 +35,8 @@ def _gpu_run(sysm, spins, beta, run_seed, sweep0, nsweeps, flags=0):
     (512, 32, 0.44, -1, abi.SCHED_FUSED),
     (512, 16, 0.0, 2, abi.SCHED_FUSED),
     (512, 16, 3.0, 1, abi.SCHED_FUSED),
+    (512, 64, 0.44, 1, abi.SCHED_FUSED),
+    (1024, 96, 0.40, -1, abi.SCHED_FUSED),
 ])
 def test_ising_square_bit_exact(gpu, Lx, Ly, beta, J, sched):
     print(f"ising square {Lx}x{Ly} beta={beta} J={J}", flush=True)
@@ -133,11 +135,13 @@ def test_ising_random_graph(gpu):
     sysm.close()
 
 
-def test_ising_fused_16_row_blocks_equal_simple(gpu):
+@pytest.mark.parametrize("Lx,Ly", [(4096, 8192), (16384, 2080)])
+def test_ising_fused_16_row_blocks_equal_simple(gpu, Lx, Ly):
     """Lattices with enough 16-row blocks to fill the device take the 16-row
     variant of the fused kernel (smaller ones, like every other case here, the
-    8-row one); both equal the CSR colour-by-colour kernel bit for bit."""
-    Lx, Ly = 4096, 8192
+    8-row one); both equal the CSR colour-by-colour kernel bit for bit.  2080
+    rows are 130 blocks of 16: band count 32, a lattice that is not square."""
+    print(f"ising fused 16-row {Lx}x{Ly}", flush=True)
     a = abi.System.ising_square(Lx, Ly, 1)
     spins = random_spins(Lx * Ly, 6, 2)
     a.upload(spins)

@@ -141,7 +141,12 @@ __device__ __forceinline__ void store_row(std::uint32_t* out, std::uint32_t row_
     uint2 w;
     w.x = __builtin_amdgcn_perm(0xFFFFFFFFu, 0x01010101u, lo);
     w.y = __builtin_amdgcn_perm(0xFFFFFFFFu, 0x01010101u, hi);
-    *reinterpret_cast<uint2*>(out + static_cast<std::size_t>(Y) * row_words + 2 * col) = w;
+    // A relaxed device-scope store (global_store ... sc1): written through the
+    // XCD's L2 while the sweep runs instead of left dirty there for the
+    // write-back at the end of the kernel (4096^2: 12.0 -> 11.1 us).
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(out + static_cast<std::size_t>(Y) * row_words + 2 * col),
+                       (static_cast<unsigned long long>(w.y) << 32) | w.x, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Thresholds of the fused kernel in LDS, by the number a of anti-aligned
