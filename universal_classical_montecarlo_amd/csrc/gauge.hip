@@ -95,10 +95,14 @@ template <int N>
 __device__ __forceinline__ void store(float* f, std::uint32_t nl, std::uint32_t idx, const Mat<N>& m)
 {
     char* fb = reinterpret_cast<char*>(f);
+    // relaxed device-scope stores: written through L2 while the colour class
+    // runs, rather than left dirty for the write-back at the end of the kernel
 #pragma unroll
     for (int k = 0; k < N * N; ++k) {
-        *reinterpret_cast<float*>(fb + ((2u * k) * nl + idx) * 4u) = m.r[k];
-        *reinterpret_cast<float*>(fb + ((2u * k + 1u) * nl + idx) * 4u) = m.i[k];
+        __hip_atomic_store(reinterpret_cast<float*>(fb + ((2u * k) * nl + idx) * 4u), m.r[k], __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(reinterpret_cast<float*>(fb + ((2u * k + 1u) * nl + idx) * 4u), m.i[k], __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 // C = A op(B), op = identity or dagger
