@@ -101,7 +101,8 @@ def expm_products(width, samples=200000, seed=1):
 
 
 def suite(quick=False):
-    """Throughput of the other BASELINE configurations (one GPU), each bounded
+    """Throughput of the other workloads (one GPU): Heisenberg 128^3, Wolff,
+    SU(3) 16^4 / 24^4, the 256-replica XY batch and SLMC, each bounded
     to a few seconds; reported beside the headline, not as `value`."""
     from universal_classical_montecarlo_amd import abi
     from oracle_lib import csr_cubic

@@ -98,8 +98,8 @@ def test_pt_batch_matches_replay(gpu, tile, use_graph):
     sysm.close()
 
 
-def test_pt_baseline_config_runs(gpu):
-    """256 replicas, L in {32..512}, 64 temperatures per
+def test_pt_256_replica_batch_runs(gpu):
+    """The 256-replica XY batch: L in {32..512}, 64 temperatures per
     ladder (the 32-replica ladders use every other beta); autotuned tiles."""
     Ls, ladders, betas = [], [], []
     grid = 0.8 + 0.6 * np.arange(64) / 63

@@ -4,7 +4,7 @@ colour-order replay.
 
 The device computes in fp32 (field, proposal, dE) and decides in double; the
 oracle replays the reference's double arithmetic on the same fp32 field.
-Tolerance (north_star): float dE within 1e-5 relative, with an absolute floor
+Tolerance: float dE within 1e-5 relative, with an absolute floor
 of 1e-5 for |dE| < 1; spins within 2e-6 per component."""
 import numpy as np
 import pytest
@@ -131,7 +131,8 @@ def test_vector_general_graph(gpu):
 
 
 def test_heisenberg_128_full_size(gpu):
-    """BASELINE config size: norm preservation and device vs host reductions."""
+    """Heisenberg 128^3 (2,097,152 sites): norm preservation and device vs host
+    reductions."""
     L = 128
     off, bonds = csr_cubic(L)
     sysm = abi.System.vector(3, off, bonds, 1.0)

@@ -18,7 +18,7 @@
 //    minimised with Adam by hand-written forward / backward kernels.
 //  * SLMC step: a Wolff cluster grown with the effective couplings over the 12
 //    neighbours (keyed per undirected edge as in wolff.hip), then accepted with
-//    min(1, exp(-beta [dE_true - dE_eff])), so the exact model stays the judge.
+//    min(1, exp(-beta [dE_true - dE_eff])), so the exact model has the last word.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
