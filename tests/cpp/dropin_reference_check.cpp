@@ -2,7 +2,9 @@
 // (test/model_check.cpp:171-222, test/graph_check.cpp:153-192,
 // test/u1_check.cpp:162-229, test/su_check.cpp:68-133) with the reference's
 // model, proposal and RNG types, driven by ucmc::gpu::Metropolis / Wolff
-// instead of the reference's drivers.  Built by oracle/Makefile (it needs the
+// instead of the reference's drivers; the same scenarios over the generic
+// models PairModel (model/pair.hpp) with IsingTerm / HeisenbergTerm and
+// CellModel (model/cell.hpp) with U1PlaquetteTerm / SUPlaquetteTerm.  Built by oracle/Makefile (it needs the
 // reference headers) into oracle/_ref/dropin_reference_check; run on a GPU by
 // tests/test_gpu_dropin.py.  Prints the measured values and "ok".
 #include <cmath>
@@ -18,8 +20,15 @@
 #include <example/sun_gauge.hpp>
 #include <example/u1_gauge.hpp>
 #include <example/vector_spin.hpp>
+#include <lattice/hypercubic.hpp>
 #include <lattice/square.hpp>
+#include <model/cell.hpp>
+#include <model/pair.hpp>
 #include <rng.hpp>
+#include <term/heisenberg.hpp>
+#include <term/ising.hpp>
+#include <term/plaquette.hpp>
+#include <term/sun_plaquette.hpp>
 
 #include <ucmc/gpu.hpp>
 
@@ -30,6 +39,12 @@
             std::exit(1);                                                           \
         }                                                                           \
     } while (0)
+
+template <class M>
+double cell_mean_plaquette(const M& model)   // 1 - S / (beta cells), S = sum beta (1 - P)
+{
+    return 1.0 - static_cast<double>(model.total_energy()) / (model.term().beta * double(model.cells()));
+}
 
 template <class M>
 double total_energy(const M& model)
@@ -118,6 +133,7 @@ int main()
         EXPECT(std::abs(hot05 - cold05) < 0.02);
         EXPECT(cold10 > 0.9);
     }
+    double su3_cold4 = 0.0;
     {   // su_check.cpp:109-133 (SU(2) L=4 and SU(3) L=3 -> device needs even L: SU(3) at L=4)
         auto chain3 = [](double beta, bool hot, std::uint64_t seed) {
             SUGauge<3> g(4, beta);
@@ -145,6 +161,93 @@ int main()
         std::printf("SU3 beta=0 hot=%.5f cold=%.5f, beta=4 hot=%.5f cold=%.5f\n", h0, c0, h4, c4);
         EXPECT(std::abs(h0) < 0.05 && std::abs(c0) < 0.05);
         EXPECT(std::abs(h4 - c4) < 0.05);
+        su3_cold4 = c4;
+    }
+    {   // PairModel<IsingDof, IsingTerm>: model_check.cpp:173-196 on the generic model
+        using PM = PairModel<IsingDof, IsingTerm, PairGraph<>>;
+        PM model(graph, IsingTerm{1});
+        const IsingSpin up = 1;
+        for (std::size_t i = 0; i < N; ++i) model.update(i, &up);   // PairModel starts at zero
+        ucmc::gpu::Metropolis metro(model, Flip{}, Pcg32(21, 1), 0.3);
+        metro.run(2000);
+        double sum = 0.0;
+        metro.sample(4000, [&](const PM& m) { sum += total_energy(m); });
+        const double avg = sum / (4000.0 * double(N));
+        PM wm(graph, IsingTerm{1});
+        for (std::size_t i = 0; i < N; ++i) wm.update(i, &up);
+        ucmc::gpu::Wolff wolf(wm, FixedAxis{}, Pcg32(22, 1), 0.3);
+        for (int k = 0; k < 2000; ++k) wolf.flip();
+        double wsum = 0.0;
+        wolf.sample(20000, [&](const PM& m) { wsum += total_energy(m); });
+        const double wavg = wsum / (20000.0 * double(N));
+        std::printf("PairModel<IsingTerm> Metropolis %.5f Wolff %.5f (exact %.5f)\n", avg, wavg, ising_exact);
+        EXPECT(std::abs(avg - ising_exact) < 0.03);
+        EXPECT(std::abs(wavg - ising_exact) < 0.05);
+        PM zero(graph, IsingTerm{1});   // left at zero: refused, not simulated
+        bool threw = false;
+        try {
+            ucmc::gpu::Metropolis bad(zero, Flip{}, Pcg32(23, 1), 0.3);
+        } catch (const std::invalid_argument&) {
+            threw = true;
+        }
+        EXPECT(threw);
+    }
+    {   // PairModel<VectorDof<3>, HeisenbergTerm<3>> against VectorSpin<3> above
+        using PM = PairModel<VectorDof<3>, HeisenbergTerm<3>, PairGraph<>>;
+        PM model(graph, HeisenbergTerm<3>{1.0});
+        const double z[3] = {0.0, 0.0, 1.0};
+        for (std::size_t i = 0; i < N; ++i) model.update(i, z);
+        ucmc::gpu::Metropolis metro(model, UniformSphereN<3>{}, Pcg32(24, 1), 0.3);
+        metro.run(2000);
+        double sum = 0.0;
+        metro.sample(4000, [&](const PM& m) { sum += total_energy(m); });
+        const double avg = sum / (4000.0 * double(N));
+        std::printf("PairModel<HeisenbergTerm<3>> Metropolis %.5f (VectorSpin %.5f)\n", avg, heis_metro);
+        EXPECT(std::abs(avg - heis_metro) < 0.05);
+    }
+    {   // CellModel<U1Dof, U1PlaquetteTerm> over hypercubic_plaquettes(4): u1_check.cpp as above
+        using CM = CellModel<U1Dof, U1PlaquetteTerm, CellComplex<4>>;
+        auto chain = [](double beta, bool hot, std::uint64_t seed) {
+            CM g(hypercubic_plaquettes(4), U1PlaquetteTerm{beta});
+            Pcg32 init(seed, 1);
+            if (hot)
+                for (std::size_t i = 0; i < g.sites(); ++i) {
+                    U1Link p;
+                    p.theta = 2.0 * std::numbers::pi_v<double> * init.uni();
+                    g.update(i, &p);
+                }
+            ucmc::gpu::Metropolis metro(g, U1Shift{0.7}, Pcg32(seed, 2), beta);
+            metro.run(100);
+            double sum = 0.0;
+            std::size_t n = 0;
+            metro.sample(400, [&](const CM& m) { sum += cell_mean_plaquette(m); ++n; });
+            return sum / double(n);
+        };
+        const double hot05 = chain(0.5, true, 51), cold05 = chain(0.5, false, 61);
+        const double cold10 = chain(10.0, false, 21);
+        std::printf("CellModel<U1PlaquetteTerm> beta=0.5 hot=%.5f cold=%.5f, beta=10 cold=%.5f\n", hot05,
+                    cold05, cold10);
+        EXPECT(std::abs(hot05 - cold05) < 0.02);
+        EXPECT(cold10 > 0.9);
+    }
+    {   // CellModel<SUDof<3>, SUPlaquetteTerm<3>>, cold start (identity links)
+        using CM = CellModel<SUDof<3>, SUPlaquetteTerm<3>, CellComplex<4>>;
+        auto chain3 = [](double beta, std::uint64_t seed) {
+            CM g(hypercubic_plaquettes(4), SUPlaquetteTerm<3>{beta});
+            const auto one = SULink<3>::identity();
+            for (std::size_t i = 0; i < g.sites(); ++i) g.update(i, &one);   // CellModel starts at zero
+            ucmc::gpu::Metropolis m(g, SUShift<3>{0.3}, Pcg32(seed, 2), beta);
+            m.run(100);
+            double sum = 0.0;
+            std::size_t n = 0;
+            m.sample(200, [&](const CM& model) { sum += cell_mean_plaquette(model); ++n; });
+            return sum / double(n);
+        };
+        const double c0 = chain3(0.0 + 1e-12, 71), c4 = chain3(4.0, 91);
+        std::printf("CellModel<SUPlaquetteTerm<3>> beta~0 cold=%.5f, beta=4 cold=%.5f (SUGauge %.5f)\n", c0,
+                    c4, su3_cold4);
+        EXPECT(std::abs(c0) < 0.05);
+        EXPECT(std::abs(c4 - su3_cold4) < 0.05);
     }
     {   // exception types at the boundary mirror the reference's
         bool threw = false;

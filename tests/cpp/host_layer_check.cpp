@@ -62,6 +62,43 @@ private:
     std::vector<Bond> rows_;
 };
 
+// Shaped like RingIsing, with a field: E = -J sum_i s_i s_{i+1} - h sum_i s_i.
+// No device kernel family implements it; the drivers must refuse it.
+class RingIsingField
+{
+public:
+    using value_type = std::int8_t;
+    using bond_type = Bond;
+    static constexpr std::size_t size = 1;
+    RingIsingField(std::size_t n, int J, int h) : J_(J), h_(h), s_(n, 1), rows_(2 * n)
+    {
+        for (std::size_t i = 0; i < n; ++i) {
+            rows_[2 * i].j = static_cast<std::uint32_t>((i + 1) % n);
+            rows_[2 * i + 1].j = static_cast<std::uint32_t>((i + n - 1) % n);
+            s_[i] = (i * 7 + 3) % 5 < 2 ? -1 : 1;
+        }
+    }
+    std::size_t sites() const noexcept { return s_.size(); }
+    const std::int8_t* data() const noexcept { return s_.data(); }
+    std::size_t degree(std::size_t) const noexcept { return 2; }
+    const Bond* bonds(std::size_t i) const noexcept { return rows_.data() + 2 * i; }
+    int energy_diff_at(std::size_t i, const std::int8_t* p) const noexcept
+    {
+        const int sum = s_[rows_[2 * i].j] + s_[rows_[2 * i + 1].j];
+        return -J_ * (*p - s_[i]) * sum - h_ * (*p - s_[i]);
+    }
+    int energy_diff_bond(std::size_t i, const Bond& b, const std::int8_t* p) const noexcept
+    {
+        return -J_ * (*p - s_[b.j]) * s_[i] - h_ * (*p - s_[b.j]) / 2;
+    }
+    void update(std::size_t i, const std::int8_t* p) noexcept { s_[i] = *p; }
+
+private:
+    int J_, h_;
+    std::vector<std::int8_t> s_;
+    std::vector<Bond> rows_;
+};
+
 // XY rotors as 2-vectors on a periodic L x L square lattice.
 class Rotors
 {
@@ -203,6 +240,17 @@ int main()
         }
         EXPECT(threw);
         std::printf("checkpoint resume %s\n", same ? "identical" : "DIFFERENT");
+    }
+    {   // a model shaped like Ising with another energy is refused, not mis-simulated
+        RingIsingField f(64, 1, 2);
+        bool threw = false;
+        try {
+            ucmc::gpu::Metropolis metro(f, Flip{}, Rng{5}, 0.5);
+        } catch (const std::invalid_argument& e) {
+            threw = true;
+            std::printf("field model refused: %s\n", e.what());
+        }
+        EXPECT(threw);
     }
     {   // errors keep the reference's exception types
         std::vector<std::uint32_t> off{0, 1, 2}, b{0, 0};

@@ -20,8 +20,10 @@
 // reference's own types (and user types shaped like them) work unchanged:
 //   Ising-like:      value_type int8_t, size 1, degree()/bonds()/energy_diff_bond
 //   VectorSpin-like: value_type double, size N in {2,3,4}, degree()/bonds()
-//   U1Gauge-like:    value_type with a double `theta`, length()
-//   SUGauge-like:    value_type with `dim` and an `m` array of complex<double>, length()
+//   U1Gauge-like:    value_type with a double `theta`; length(), or the cells of
+//                    a CellModel (cells(), cell_sites(c), term().beta)
+//   SUGauge-like:    value_type with `dim` and an `m` array of complex<double>;
+//                    length() or CellModel cells as above
 // with the proposal Flip / UniformSphereN / LocalRotation (has `delta`) /
 // U1Shift / SUShift (have `width`).  Couplings the models keep private (J,
 // beta) are read back through the models' own energy functions.
@@ -116,19 +118,75 @@ template <class M>
 concept VectorLike = PairRows<M> && std::same_as<typename M::value_type, double> &&
                      (M::size >= 2) && (M::size <= 4);
 
+// Gauge models: the example types keep L (length(), example/u1_gauge.hpp:30)
+// and their coupling private; CellModel (model/cell.hpp:16-99) exposes its
+// cells and its term, whose beta is the plaquette coupling
+// (term/plaquette.hpp:19, term/sun_plaquette.hpp:22).
 template <class M>
-concept U1GaugeLike = requires(const M cm, const typename M::value_type v) {
-    { v.theta } -> std::convertible_to<double>;
-    { cm.length() } -> std::convertible_to<std::size_t>;
-} && (sizeof(typename M::value_type) == sizeof(double));
+concept HasLength = requires(const M cm) { { cm.length() } -> std::convertible_to<std::size_t>; };
 
 template <class M>
-concept SUGaugeLike = requires(const M cm, const typename M::value_type v) {
-    M::value_type::dim;
+concept PlaquetteCells = requires(const M cm, std::size_t c) {
+    { cm.cells() } -> std::convertible_to<std::size_t>;
+    cm.cell_sites(c);
+    { cm.term().beta } -> std::convertible_to<double>;
+} && (M::arity == 4);
+
+template <class T>
+concept U1Value = requires(const T v) { { v.theta } -> std::convertible_to<double>; } &&
+                  (sizeof(T) == sizeof(double));
+
+template <class T>
+concept SUValue = requires(const T v) {
+    T::dim;
     v.m;
-    { cm.length() } -> std::convertible_to<std::size_t>;
-} && (sizeof(typename M::value_type) ==
-      2 * sizeof(double) * M::value_type::dim * M::value_type::dim);
+} && (sizeof(T) == 2 * sizeof(double) * T::dim * T::dim);
+
+template <class M>
+concept U1GaugeLike = U1Value<typename M::value_type> && (HasLength<M> || PlaquetteCells<M>);
+
+template <class M>
+concept SUGaugeLike = SUValue<typename M::value_type> && (HasLength<M> || PlaquetteCells<M>);
+
+// L of a CellModel whose cells are hypercubic_plaquettes(L) exactly
+// (lattice/hypercubic.hpp:15-64): 4 L^4 links, 6 L^4 plaquettes, planes mu < nu
+// in turn, sites c = (t, z, y, x) in row-major order within a plane, links
+// (mu, c), (nu, c + e_mu), (mu, c + e_nu), (nu, c) with link id mu L^4 + c.  The
+// device gauge kernels generate that complex; any other cell list is refused.
+template <class M>
+std::size_t plaquette_length(const M& m)
+{
+    const std::size_t links = m.sites();
+    std::size_t L = static_cast<std::size_t>(std::llround(std::pow(static_cast<double>(links) / 4.0, 0.25)));
+    const std::size_t L4 = L * L * L * L;
+    if (L == 0 || 4 * L4 != links || m.cells() != 6 * L4)
+        throw std::invalid_argument("ucmc::gpu: the cell list is not a 4D hypercubic plaquette complex");
+    // stride of axis a (0 = t, 1 = z, 2 = y, 3 = x) in the linear site index
+    const std::size_t stride[4] = {L * L * L, L * L, L, 1};
+    auto step = [&](std::size_t c, int a) {
+        const std::size_t x = (c / stride[a]) % L;
+        return x + 1 == L ? c + stride[a] - L * stride[a] : c + stride[a];
+    };
+    std::size_t cell = 0;
+    for (int mu = 0; mu < 4; ++mu)
+        for (int nu = mu + 1; nu < 4; ++nu)
+            for (std::size_t c = 0; c < L4; ++c, ++cell) {
+                const std::size_t want[4] = {mu * L4 + c, nu * L4 + step(c, mu), mu * L4 + step(c, nu), nu * L4 + c};
+                const auto got = m.cell_sites(cell);
+                for (int k = 0; k < 4; ++k)
+                    if (static_cast<std::size_t>(got[k]) != want[k])
+                        throw std::invalid_argument("ucmc::gpu: cell " + std::to_string(cell) +
+                                                    " differs from hypercubic_plaquettes(" + std::to_string(L) + ")");
+            }
+    return L;
+}
+
+template <class M>
+std::size_t gauge_length(const M& m)
+{
+    if constexpr (HasLength<M>) return m.length();
+    else return plaquette_length(m);
+}
 
 template <class Q>
 concept HasDelta = requires(const Q q) { { q.delta } -> std::convertible_to<double>; };
@@ -197,6 +255,36 @@ double probe_pair_coupling(const M& m)
     return 1.0; // no bonds: J does not enter any energy
 }
 
+// The pair kernels implement dE = -J (p - s_i).sum_j s_j over the model's
+// rows (example/ising.hpp:35-48, example/vector_spin.hpp:50-60, PairModel over
+// IsingTerm / HeisenbergTerm).  A model shaped like those types whose energy
+// has another form (a field, a different pair function) is rejected rather than
+// simulated as the wrong model: energy_diff_at of every site for the trial
+// value p = -s_i is compared with that form on the model's current field.
+template <class M>
+void verify_pair_form(const M& m, double J)
+{
+    using T = typename M::value_type;
+    constexpr std::size_t D = M::size;
+    for (std::size_t i = 0; i < m.sites(); ++i) {
+        const T* si = m.data() + i * D;
+        std::array<T, D> p{};
+        double want = 0.0;
+        for (std::size_t k = 0; k < D; ++k) {
+            p[k] = static_cast<T>(-si[k]);
+            double h = 0.0;
+            const auto* b = m.bonds(i);
+            for (std::size_t a = 0; a < m.degree(i); ++a) h += static_cast<double>(m.data()[b[a].j * D + k]);
+            want += -J * (static_cast<double>(p[k]) - static_cast<double>(si[k])) * h;
+        }
+        const double got = static_cast<double>(m.energy_diff_at(i, p.data()));
+        if (!(std::abs(got - want) <= 1e-9 * (1.0 + std::abs(want))))
+            throw std::invalid_argument("ucmc::gpu: the model's energy_diff_at is not -J (p - s_i).sum_j s_j "
+                                        "over its rows (site " + std::to_string(i) + "); no device kernel "
+                                        "family implements it");
+    }
+}
+
 // --------------------------------------------------------- device model --
 template <class M>
 struct Device
@@ -221,16 +309,22 @@ struct Device
             rows_of(m, off, b);
             const ucmc_pair_graph g{m.sites(), off.data(), b.data()};
             const double J = probe_pair_coupling(m);
+            // the Ising kernels decide from integer energies (IsingTerm::energy_type)
+            if (J != std::round(J) || std::abs(J) > 1e6)
+                throw std::invalid_argument("ucmc::gpu: Ising coupling " + std::to_string(J) + " is not an integer");
+            verify_pair_form(m, J);
             check(ucmc_ising_create(&g, static_cast<std::int32_t>(J), 0, &h));
         } else if constexpr (VectorLike<M>) {
             std::vector<std::uint32_t> off, b;
             rows_of(m, off, b);
             const ucmc_pair_graph g{m.sites(), off.data(), b.data()};
-            check(ucmc_vector_create(static_cast<int>(M::size), &g, probe_pair_coupling(m), 0, &h));
+            const double J = probe_pair_coupling(m);
+            verify_pair_form(m, J);
+            check(ucmc_vector_create(static_cast<int>(M::size), &g, J, 0, &h));
         } else if constexpr (U1GaugeLike<M>) {
-            check(ucmc_u1_gauge_create(m.length(), gauge_beta(m), 0, &h));
+            check(ucmc_u1_gauge_create(gauge_length(m), gauge_coupling(m), 0, &h));
         } else {
-            check(ucmc_sun_gauge_create(static_cast<int>(T::dim), m.length(), gauge_beta(m), 0, &h));
+            check(ucmc_sun_gauge_create(static_cast<int>(T::dim), gauge_length(m), gauge_coupling(m), 0, &h));
         }
         System s(h);
         upload(s, m);
@@ -239,10 +333,16 @@ struct Device
 
     static void upload(System& s, const M& m)
     {
-        if constexpr (IsingLike<M>)
+        if constexpr (IsingLike<M>) {
+            // PairModel starts every component at zero (model/pair.hpp:31-34):
+            // an Ising field must be set to +-1 before a driver takes it
+            for (std::size_t i = 0; i < m.sites(); ++i)
+                if (m.data()[i] != 1 && m.data()[i] != -1)
+                    throw std::invalid_argument("ucmc::gpu: Ising site " + std::to_string(i) + " is not +1 or -1");
             check(ucmc_field_upload(s.get(), m.data(), m.sites()));
-        else
+        } else {
             check(ucmc_field_upload(s.get(), m.data(), m.sites() * doubles_per_site() * sizeof(double)));
+        }
     }
 
     // Writes the device field into the model through ModelLike::update.
@@ -270,14 +370,20 @@ struct Device
         }
     }
 
+    static double gauge_coupling(const M& m)
+    {
+        if constexpr (PlaquetteCells<M>) return static_cast<double>(m.term().beta);
+        else return gauge_beta(m);
+    }
+
     // The gauge models keep beta private; energy_diff_at of one link is
     // beta * sum_p (before - after), and the same sum at beta = 1 comes from a
     // throw-away device system holding the same field.
     static double gauge_beta(const M& m)
     {
         ucmc_system* h = nullptr;
-        if constexpr (U1GaugeLike<M>) check(ucmc_u1_gauge_create(m.length(), 1.0, 0, &h));
-        else check(ucmc_sun_gauge_create(static_cast<int>(T::dim), m.length(), 1.0, 0, &h));
+        if constexpr (U1GaugeLike<M>) check(ucmc_u1_gauge_create(gauge_length(m), 1.0, 0, &h));
+        else check(ucmc_sun_gauge_create(static_cast<int>(T::dim), gauge_length(m), 1.0, 0, &h));
         System unit(h);
         upload(unit, m);
         ucmc_observables before{}, after{};
