@@ -711,11 +711,10 @@ int ucmc_field_upload(ucmc_system* sys, const void* host, size_t bytes)
             const std::size_t per = field_doubles_per_site(*s);
             if (bytes != s->sites * per * sizeof(double))
                 fail(UCMC_E_INVALID_ARGUMENT, "field size does not match the model");
-            // host AoS double -> device SoA fp32: component k of site i at [k * sites + i]
+            // host AoS double -> device AoS fp32: component k of site i at [i * per + k]
             std::vector<float> f(s->sites * per);
             const auto* v = static_cast<const double*>(host);
-            for (std::size_t i = 0; i < s->sites; ++i)
-                for (std::size_t k = 0; k < per; ++k) f[k * s->sites + i] = static_cast<float>(v[i * per + k]);
+            for (std::size_t i = 0; i < s->sites * per; ++i) f[i] = static_cast<float>(v[i]);
             UCMC_HIP(hipMemcpyAsync(s->field[s->cur].p, f.data(), f.size() * 4, hipMemcpyHostToDevice, s->stream));
             UCMC_HIP(hipStreamSynchronize(s->stream));
             break;
@@ -751,8 +750,7 @@ int ucmc_field_download(ucmc_system* sys, void* host, size_t bytes)
             UCMC_HIP(hipMemcpyAsync(f.data(), s->field[s->cur].p, f.size() * 4, hipMemcpyDeviceToHost, s->stream));
             UCMC_HIP(hipStreamSynchronize(s->stream));
             auto* v = static_cast<double*>(host);
-            for (std::size_t i = 0; i < s->sites; ++i)
-                for (std::size_t k = 0; k < per; ++k) v[i * per + k] = static_cast<double>(f[k * s->sites + i]);
+            for (std::size_t i = 0; i < s->sites * per; ++i) v[i] = static_cast<double>(f[i]);
             break;
         }
         }

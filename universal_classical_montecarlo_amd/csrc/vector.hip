@@ -2,8 +2,8 @@
 // Metropolis with UniformSphereN<N> or LocalRotation<N>{delta}
 // (dof/vector.hpp:35-122), driven as in driver.hpp:73-87.
 //
-// Field on the device: fp32 SoA, component k of site i at [k * sites + i]
-// (coalesced per component).  One launch per colour class: (x + y + z) & 1 on
+// Field on the device: fp32 AoS, component k of site i at [i * N + k]: the N
+// components of a site are one 4N-byte run, read with one multi-dword load.  One launch per colour class: (x + y + z) & 1 on
 // a periodic simple-cubic lattice with even L, (x + y) & 1 on an even square
 // lattice, a greedy colouring otherwise.  Each thread owns one site of the
 // class and draws its proposal and accept uniforms, in the reference's order,
@@ -26,7 +26,7 @@ namespace {
 constexpr double kPi = 3.14159265358979323846;
 
 struct VecArgs {
-    float* f;                       // SoA [N][n]
+    float* f;                       // AoS [n][N]: the N components of a site in one 4N-byte run
     std::uint32_t n;
     const std::uint32_t* offsets;   // CSR (general / square)
     const std::uint32_t* bonds;
@@ -45,12 +45,13 @@ struct VecArgs {
     std::uint8_t* trace_acc;
 };
 
-// Component k of site i of the SoA field through a 32-bit byte offset (the
-// field is below 4 GiB, checked at setup), so the loads take the uniform base
-// and a per-lane offset.
-__device__ __forceinline__ float& fld(float* f, std::uint32_t n, int k, std::uint32_t i)
+// The N components of site i of the AoS field, through a 32-bit byte offset
+// (the field is below 4 GiB, checked at setup): the loads take the uniform base
+// and one per-lane offset, and the N of them merge into one multi-dword load.
+template <int N>
+__device__ __forceinline__ float* site_of(float* f, std::uint32_t i)
 {
-    return *reinterpret_cast<float*>(reinterpret_cast<char*>(f) + (static_cast<std::uint32_t>(k) * n + i) * 4u);
+    return reinterpret_cast<float*>(reinterpret_cast<char*>(f) + i * static_cast<std::uint32_t>(4 * N));
 }
 
 // u in [0, 1) as float, from a 32-bit draw k: (float)(k * 2^-32) may round to
@@ -247,21 +248,24 @@ __global__ __launch_bounds__(256) void vector_colour(VecArgs a)
             i = a.sites[t];
         }
         float s[N], p[N], h[N];
+        float* const si = site_of<N>(a.f, i);
 #pragma unroll
         for (int k = 0; k < N; ++k) {
-            s[k] = fld(a.f, a.n, k, i);
+            s[k] = si[k];
             h[k] = 0.f;
         }
         if constexpr (CUBIC) {
 #pragma unroll
-            for (int q = 0; q < 6; ++q)
+            for (int q = 0; q < 6; ++q) {
+                const float* sj = site_of<N>(a.f, nb[q]);
 #pragma unroll
-                for (int k = 0; k < N; ++k) h[k] += fld(a.f, a.n, k, nb[q]);
+                for (int k = 0; k < N; ++k) h[k] += sj[k];
+            }
         } else {
             for (std::uint32_t q = a.offsets[i]; q < a.offsets[i + 1]; ++q) {
-                const std::uint32_t j = a.bonds[q];
+                const float* sj = site_of<N>(a.f, a.bonds[q]);
 #pragma unroll
-                for (int k = 0; k < N; ++k) h[k] += fld(a.f, a.n, k, j);
+                for (int k = 0; k < N; ++k) h[k] += sj[k];
             }
         }
         (void)deg;
@@ -280,7 +284,7 @@ __global__ __launch_bounds__(256) void vector_colour(VecArgs a)
             for (int k = 0; k < N; ++k) nn += p[k] * p[k];
             const float inv = rsqrtf(nn);
 #pragma unroll
-            for (int k = 0; k < N; ++k) fld(a.f, a.n, k, i) = p[k] * inv;
+            for (int k = 0; k < N; ++k) si[k] = p[k] * inv;
         }
         if constexpr (TRACE) {
             a.trace_dE[i] = dE;
@@ -307,7 +311,7 @@ __global__ __launch_bounds__(256) void vector_reduce(const float* f, std::uint32
         float s[N];
 #pragma unroll
         for (int k = 0; k < N; ++k) {
-            s[k] = f[k * n + i];
+            s[k] = f[i * N + k];
             m[k] += s[k];
         }
         if (L) { // cubic: +x, +y, +z bonds once each
@@ -317,14 +321,14 @@ __global__ __launch_bounds__(256) void vector_reduce(const float* f, std::uint32
             for (int q = 0; q < 3; ++q) {
                 double d = 0.0;
 #pragma unroll
-                for (int k = 0; k < N; ++k) d += static_cast<double>(s[k]) * f[k * n + nb[q]];
+                for (int k = 0; k < N; ++k) d += static_cast<double>(s[k]) * f[nb[q] * N + k];
                 e += d;
             }
         } else {
             double d = 0.0;
             for (std::uint32_t q = off[i]; q < off[i + 1]; ++q)
 #pragma unroll
-                for (int k = 0; k < N; ++k) d += static_cast<double>(s[k]) * f[k * n + bonds[q]];
+                for (int k = 0; k < N; ++k) d += static_cast<double>(s[k]) * f[bonds[q] * N + k];
             e += 0.5 * d;
         }
     }
@@ -453,7 +457,7 @@ void vector_setup(System& s)
     s.field[0].alloc(s.sites * s.N * sizeof(float));
     // example/vector_spin.hpp:27-28: every spin along the last axis
     std::vector<float> init(s.sites * s.N, 0.f);
-    for (std::size_t i = 0; i < s.sites; ++i) init[(s.N - 1) * s.sites + i] = 1.f;
+    for (std::size_t i = 0; i < s.sites; ++i) init[i * s.N + (s.N - 1)] = 1.f;
     UCMC_HIP(hipMemcpy(s.field[0].p, init.data(), init.size() * 4, hipMemcpyHostToDevice));
     s.scratch.alloc(64 * sizeof(double));
 }

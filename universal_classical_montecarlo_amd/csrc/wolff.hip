@@ -37,7 +37,7 @@ constexpr int kThreads = 1024;
 constexpr std::size_t kLabelMinSites = 32768;   // lattices from this size on use label propagation
 
 struct WolffArgs {
-    void* field;                   // int8 [n] (Ising) or fp32 SoA [N][n]
+    void* field;                   // int8 [n] (Ising) or fp32 AoS [n][N]
     std::uint32_t n;
     const std::uint32_t* offsets;
     const std::uint32_t* bonds;
@@ -79,7 +79,7 @@ __device__ void sphere_d(Stream& r, double* out)
     }
 }
 
-// N == 1: Ising (int8 field), else O(N) fp32 SoA.
+// N == 1: Ising (int8 field), else O(N) fp32 AoS.
 template <int N>
 struct Field;
 
@@ -96,7 +96,7 @@ struct Field {
     __device__ double proj(std::uint32_t i, const double* r) const
     {
         double a = 0.0;
-        for (int k = 0; k < N; ++k) a = dadd(a, dmul(static_cast<double>(f[k * n + i]), r[k]));
+        for (int k = 0; k < N; ++k) a = dadd(a, dmul(static_cast<double>(f[i * N + k]), r[k]));
         return a;
     }
 };
@@ -217,10 +217,10 @@ __global__ __launch_bounds__(kThreads) void wolff_flips(WolffArgs a)
                 for (std::uint32_t q = a.offsets[i]; q < a.offsets[i + 1]; ++q) {
                     const std::uint32_t j = a.bonds[q];
                     if (a.stamp[j] != mark)
-                        for (int k = 0; k < N; ++k) h[k] += fld.f[k * a.n + j];
+                        for (int k = 0; k < N; ++k) h[k] += fld.f[j * N + k];
                 }
                 for (int k = 0; k < N; ++k) {
-                    const double s = fld.f[k * a.n + i];
+                    const double s = fld.f[i * N + k];
                     const float sn = static_cast<float>(dadd(s, -dmul(dmul(2.0, ai), r[k])));
                     de += -a.J * (static_cast<double>(sn) - s) * h[k];
                     dm[k] += static_cast<double>(sn) - s;
@@ -245,8 +245,8 @@ __global__ __launch_bounds__(kThreads) void wolff_flips(WolffArgs a)
                 float* f = static_cast<float*>(a.field);
                 const double ai = fld.proj(i, r);
                 for (int k = 0; k < N; ++k) {
-                    const double s = f[k * a.n + i];
-                    f[k * a.n + i] = static_cast<float>(dadd(s, -dmul(dmul(2.0, ai), r[k])));
+                    const double s = f[i * N + k];
+                    f[i * N + k] = static_cast<float>(dadd(s, -dmul(dmul(2.0, ai), r[k])));
                 }
             }
         }
@@ -496,11 +496,11 @@ __device__ __forceinline__ void delta_site(const LabelArgs& a, const Field<N>& f
         for (int k = 0; k < N; ++k) h[k] = 0.0;
         nbrs([&](std::uint32_t j) {
             if (a.parent[j] != root)
-                for (int k = 0; k < N; ++k) h[k] += fld.f[k * a.n + j];
+                for (int k = 0; k < N; ++k) h[k] += fld.f[j * N + k];
         });
         double de = 0.0;
         for (int k = 0; k < N; ++k) {
-            const double s = fld.f[k * a.n + i];
+            const double s = fld.f[i * N + k];
             const float sn = static_cast<float>(dadd(s, -dmul(dmul(2.0, ai), r[k])));
             de += -a.J * (static_cast<double>(sn) - s) * h[k];
             acc[2 + k] += static_cast<double>(sn) - s;
@@ -544,10 +544,10 @@ __global__ __launch_bounds__(256) void label_apply(LabelArgs a)
         double r[4];
         for (int k = 0; k < 4; ++k) r[k] = a.st->r[k];
         double ai = 0.0;
-        for (int k = 0; k < N; ++k) ai = dadd(ai, dmul(static_cast<double>(f[k * a.n + i]), r[k]));
+        for (int k = 0; k < N; ++k) ai = dadd(ai, dmul(static_cast<double>(f[i * N + k]), r[k]));
         for (int k = 0; k < N; ++k) {
-            const double s = f[k * a.n + i];
-            f[k * a.n + i] = static_cast<float>(dadd(s, -dmul(dmul(2.0, ai), r[k])));
+            const double s = f[i * N + k];
+            f[i * N + k] = static_cast<float>(dadd(s, -dmul(dmul(2.0, ai), r[k])));
         }
     }
 }
