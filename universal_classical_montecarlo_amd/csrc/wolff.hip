@@ -89,14 +89,28 @@ struct Field<1> {
     __device__ double proj(std::uint32_t i, const double*) const { return f[i]; }
 };
 
+// The N components of site i of an AoS fp32 field (32-bit byte offset: the
+// field is below 4 GiB), so that their loads merge into one.
+template <int N>
+__device__ __forceinline__ const float* vsite(const float* f, std::uint32_t i)
+{
+    return reinterpret_cast<const float*>(reinterpret_cast<const char*>(f) + i * static_cast<std::uint32_t>(4 * N));
+}
+template <int N>
+__device__ __forceinline__ float* vsite(float* f, std::uint32_t i)
+{
+    return reinterpret_cast<float*>(reinterpret_cast<char*>(f) + i * static_cast<std::uint32_t>(4 * N));
+}
+
 template <int N>
 struct Field {
     const float* f;
     std::uint32_t n;
     __device__ double proj(std::uint32_t i, const double* r) const
     {
+        const float* s = vsite<N>(f, i);
         double a = 0.0;
-        for (int k = 0; k < N; ++k) a = dadd(a, dmul(static_cast<double>(f[i * N + k]), r[k]));
+        for (int k = 0; k < N; ++k) a = dadd(a, dmul(static_cast<double>(s[k]), r[k]));
         return a;
     }
 };
@@ -217,10 +231,10 @@ __global__ __launch_bounds__(kThreads) void wolff_flips(WolffArgs a)
                 for (std::uint32_t q = a.offsets[i]; q < a.offsets[i + 1]; ++q) {
                     const std::uint32_t j = a.bonds[q];
                     if (a.stamp[j] != mark)
-                        for (int k = 0; k < N; ++k) h[k] += fld.f[j * N + k];
+                        for (int k = 0; k < N; ++k) h[k] += vsite<N>(fld.f, j)[k];
                 }
                 for (int k = 0; k < N; ++k) {
-                    const double s = fld.f[i * N + k];
+                    const double s = vsite<N>(fld.f, i)[k];
                     const float sn = static_cast<float>(dadd(s, -dmul(dmul(2.0, ai), r[k])));
                     de += -a.J * (static_cast<double>(sn) - s) * h[k];
                     dm[k] += static_cast<double>(sn) - s;
@@ -245,8 +259,9 @@ __global__ __launch_bounds__(kThreads) void wolff_flips(WolffArgs a)
                 float* f = static_cast<float*>(a.field);
                 const double ai = fld.proj(i, r);
                 for (int k = 0; k < N; ++k) {
-                    const double s = f[i * N + k];
-                    f[i * N + k] = static_cast<float>(dadd(s, -dmul(dmul(2.0, ai), r[k])));
+                    float* si = vsite<N>(f, i);
+                    const double s = si[k];
+                    si[k] = static_cast<float>(dadd(s, -dmul(dmul(2.0, ai), r[k])));
                 }
             }
         }
@@ -496,11 +511,11 @@ __device__ __forceinline__ void delta_site(const LabelArgs& a, const Field<N>& f
         for (int k = 0; k < N; ++k) h[k] = 0.0;
         nbrs([&](std::uint32_t j) {
             if (a.parent[j] != root)
-                for (int k = 0; k < N; ++k) h[k] += fld.f[j * N + k];
+                for (int k = 0; k < N; ++k) h[k] += vsite<N>(fld.f, j)[k];
         });
         double de = 0.0;
         for (int k = 0; k < N; ++k) {
-            const double s = fld.f[i * N + k];
+            const double s = vsite<N>(fld.f, i)[k];
             const float sn = static_cast<float>(dadd(s, -dmul(dmul(2.0, ai), r[k])));
             de += -a.J * (static_cast<double>(sn) - s) * h[k];
             acc[2 + k] += static_cast<double>(sn) - s;
@@ -544,10 +559,11 @@ __global__ __launch_bounds__(256) void label_apply(LabelArgs a)
         double r[4];
         for (int k = 0; k < 4; ++k) r[k] = a.st->r[k];
         double ai = 0.0;
-        for (int k = 0; k < N; ++k) ai = dadd(ai, dmul(static_cast<double>(f[i * N + k]), r[k]));
+        float* si = vsite<N>(f, i);
+        for (int k = 0; k < N; ++k) ai = dadd(ai, dmul(static_cast<double>(si[k]), r[k]));
         for (int k = 0; k < N; ++k) {
-            const double s = f[i * N + k];
-            f[i * N + k] = static_cast<float>(dadd(s, -dmul(dmul(2.0, ai), r[k])));
+            const double s = si[k];
+            si[k] = static_cast<float>(dadd(s, -dmul(dmul(2.0, ai), r[k])));
         }
     }
 }
