@@ -14,9 +14,10 @@
 // so the 8 colour classes run as 8 launches, one thread per link, `hits`
 // Metropolis hits each drawing from the link's own Stream(key, link, kTagSite).
 //
-// Storage: fp32, 2 N^2 planes (re, im of the row-major entries) for SU(N), one
-// angle plane for U(1); within a plane, links are ordered
-// mu L^4 + parity L^4/2 + (linear(c) >> 1) so one colour class is contiguous.
+// Storage: fp32; SU(N) links as 2 N^2 consecutive floats (re, im of the
+// row-major entries: one link is a few multi-dword loads), U(1) links as one
+// angle; links are ordered mu L^4 + parity L^4/2 + (linear(c) >> 1) so one
+// colour class is contiguous.
 // FP64 is used only for the plaquette / action sums and the periodic
 // reunitarisation.
 #include <hip/hip_runtime.h>
@@ -79,30 +80,32 @@ struct Mat {
     float r[N * N], i[N * N];
 };
 
-// Byte offsets in 32 bits (the field is below 4 GiB, checked on the host), so
-// the loads take the uniform base and a 32-bit per-lane offset.
+// Link idx through a 32-bit byte offset (the field is below 4 GiB, checked on
+// the host): the loads take the uniform base and one per-lane offset, and the
+// 2 N^2 of them merge into multi-dword loads.
 template <int N>
-__device__ __forceinline__ void load(const float* f, std::uint32_t nl, std::uint32_t idx, Mat<N>& m)
+__device__ __forceinline__ const float* link_of(const float* f, std::uint32_t idx)
 {
-    const char* fb = reinterpret_cast<const char*>(f);
+    return reinterpret_cast<const float*>(reinterpret_cast<const char*>(f) + idx * static_cast<std::uint32_t>(8 * N * N));
+}
+template <int N>
+__device__ __forceinline__ void load(const float* f, std::uint32_t idx, Mat<N>& m)
+{
+    const float* lp = link_of<N>(f, idx);
 #pragma unroll
     for (int k = 0; k < N * N; ++k) {
-        m.r[k] = *reinterpret_cast<const float*>(fb + ((2u * k) * nl + idx) * 4u);
-        m.i[k] = *reinterpret_cast<const float*>(fb + ((2u * k + 1u) * nl + idx) * 4u);
+        m.r[k] = lp[2 * k];
+        m.i[k] = lp[2 * k + 1];
     }
 }
 template <int N>
-__device__ __forceinline__ void store(float* f, std::uint32_t nl, std::uint32_t idx, const Mat<N>& m)
+__device__ __forceinline__ void store(float* f, std::uint32_t idx, const Mat<N>& m)
 {
-    char* fb = reinterpret_cast<char*>(f);
-    // relaxed device-scope stores: written through L2 while the colour class
-    // runs, rather than left dirty for the write-back at the end of the kernel
+    float* lp = const_cast<float*>(link_of<N>(f, idx));
 #pragma unroll
     for (int k = 0; k < N * N; ++k) {
-        __hip_atomic_store(reinterpret_cast<float*>(fb + ((2u * k) * nl + idx) * 4u), m.r[k], __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(reinterpret_cast<float*>(fb + ((2u * k + 1u) * nl + idx) * 4u), m.i[k], __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+        lp[2 * k] = m.r[k];
+        lp[2 * k + 1] = m.i[k];
     }
 }
 // C = A op(B), op = identity or dagger
@@ -284,9 +287,9 @@ __global__ __launch_bounds__(256) void sun_colour(SunArgs a)
             shift(a.g, cn, nu, +1);
             // forward: U_nu(c+mu) U_mu(c+nu)^+ U_nu(c)^+
             Mat<N> x, y, z;
-            load<N>(a.f, a.nl, dev_index(a.g, nu, cm), x);
-            load<N>(a.f, a.nl, dev_index(a.g, mu, cn), y);
-            load<N>(a.f, a.nl, dev_index(a.g, nu, c), z);
+            load<N>(a.f, dev_index(a.g, nu, cm), x);
+            load<N>(a.f, dev_index(a.g, mu, cn), y);
+            load<N>(a.f, dev_index(a.g, nu, c), z);
             mul<N, true>(x, y, t1);
             mul<N, true>(t1, z, t2);
 #pragma unroll
@@ -299,9 +302,9 @@ __global__ __launch_bounds__(256) void sun_colour(SunArgs a)
             shift(a.g, cb, nu, -1);
             std::uint32_t cbm[4] = {cb[0], cb[1], cb[2], cb[3]};
             shift(a.g, cbm, mu, +1);
-            load<N>(a.f, a.nl, dev_index(a.g, nu, cbm), x);
-            load<N>(a.f, a.nl, dev_index(a.g, mu, cb), y);
-            load<N>(a.f, a.nl, dev_index(a.g, nu, cb), z);
+            load<N>(a.f, dev_index(a.g, nu, cbm), x);
+            load<N>(a.f, dev_index(a.g, mu, cb), y);
+            load<N>(a.f, dev_index(a.g, nu, cb), z);
             // (y x)^+ z = x^+ y^+ z
             mul<N, false>(y, x, t1);
             mul_dag_a<N>(t1, z, t2);
@@ -311,7 +314,7 @@ __global__ __launch_bounds__(256) void sun_colour(SunArgs a)
                 A.i[k] += t2.i[k];
             }
         }
-        load<N>(a.f, a.nl, self, U);
+        load<N>(a.f, self, U);
         Stream r;
         const std::uint32_t ref_id = a.mu * a.g.L4 + lin4(a.g, c);
         r.init(a.key, ref_id, kTagSite);
@@ -336,7 +339,7 @@ __global__ __launch_bounds__(256) void sun_colour(SunArgs a)
                 a.trace_acc[static_cast<std::size_t>(ref_id) * a.hits + hit] = acc ? 1 : 0;
             }
         }
-        if (nacc) store<N>(a.f, a.nl, self, U);
+        if (nacc) store<N>(a.f, self, U);
     }
     if (a.accepts) {
         unsigned long long v = nacc;
@@ -427,7 +430,7 @@ __global__ __launch_bounds__(256) void u1_colour(U1Args a)
 // sum over the 6 L^4 plaquettes (mu < nu, c) of Re Tr(U0 U1 U2^+ U3^+) / N
 // (products in fp32, sum in FP64).
 template <int N>
-__global__ __launch_bounds__(256) void sun_plaquette_sum(const float* f, Geo4 g, std::uint32_t nl, double* out)
+__global__ __launch_bounds__(256) void sun_plaquette_sum(const float* f, Geo4 g, double* out)
 {
     double acc = 0.0;
     for (std::uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < g.L4; s += gridDim.x * blockDim.x) {
@@ -442,10 +445,10 @@ __global__ __launch_bounds__(256) void sun_plaquette_sum(const float* f, Geo4 g,
                 shift(g, cm, mu, +1);
                 shift(g, cn, nu, +1);
                 Mat<N> u0, u1, u2, u3, t1, t2;
-                load<N>(f, nl, dev_index(g, mu, c), u0);
-                load<N>(f, nl, dev_index(g, nu, cm), u1);
-                load<N>(f, nl, dev_index(g, mu, cn), u2);
-                load<N>(f, nl, dev_index(g, nu, c), u3);
+                load<N>(f, dev_index(g, mu, c), u0);
+                load<N>(f, dev_index(g, nu, cm), u1);
+                load<N>(f, dev_index(g, mu, cn), u2);
+                load<N>(f, dev_index(g, nu, c), u3);
                 mul<N, false>(u0, u1, t1);
                 mul<N, true>(t1, u2, t2);
                 // Re Tr(t2 u3^+) = sum_xk Re(t2_xk conj(u3_xk))
@@ -490,8 +493,8 @@ __global__ __launch_bounds__(256) void su3_reunitarize(float* f, std::uint32_t n
     if (i >= nl) return;
     double re[9], im[9];
     for (int k = 0; k < 9; ++k) {
-        re[k] = f[(2 * k) * static_cast<std::size_t>(nl) + i];
-        im[k] = f[(2 * k + 1) * static_cast<std::size_t>(nl) + i];
+        re[k] = f[static_cast<std::size_t>(i) * 18 + 2 * k];
+        im[k] = f[static_cast<std::size_t>(i) * 18 + 2 * k + 1];
     }
     double n0 = 0.0;
     for (int j = 0; j < 3; ++j) n0 += re[j] * re[j] + im[j] * im[j];
@@ -525,8 +528,8 @@ __global__ __launch_bounds__(256) void su3_reunitarize(float* f, std::uint32_t n
         im[6 + k] = -ci;
     }
     for (int k = 0; k < 9; ++k) {
-        f[(2 * k) * static_cast<std::size_t>(nl) + i] = static_cast<float>(re[k]);
-        f[(2 * k + 1) * static_cast<std::size_t>(nl) + i] = static_cast<float>(im[k]);
+        f[static_cast<std::size_t>(i) * 18 + 2 * k] = static_cast<float>(re[k]);
+        f[static_cast<std::size_t>(i) * 18 + 2 * k + 1] = static_cast<float>(im[k]);
     }
 }
 
@@ -595,10 +598,10 @@ void plaquette_sum(System& s, double* out)
         hipLaunchKernelGGL(u1_plaquette_sum, dim3(blocks), dim3(256), 0, s.stream, s.field[s.cur].as<float>(), g, out);
     else if (s.N == 3)
         hipLaunchKernelGGL(sun_plaquette_sum<3>, dim3(blocks), dim3(256), 0, s.stream, s.field[s.cur].as<float>(), g,
-                           static_cast<std::uint32_t>(s.sites), out);
+                           out);
     else
         hipLaunchKernelGGL(sun_plaquette_sum<2>, dim3(blocks), dim3(256), 0, s.stream, s.field[s.cur].as<float>(), g,
-                           static_cast<std::uint32_t>(s.sites), out);
+                           out);
     UCMC_HIP(hipGetLastError());
 }
 
@@ -652,7 +655,8 @@ void gauge_setup(System& s)
     s.field[0].alloc(s.sites * planes * sizeof(float));
     std::vector<float> init(s.sites * planes, 0.f);
     if (s.kind == UCMC_SUN_GAUGE) // example/sun_gauge.hpp:26: identity (cold) start
-        for (int x = 0; x < s.N; ++x) std::fill_n(init.begin() + (2 * (x * s.N + x)) * s.sites, s.sites, 1.f);
+        for (std::size_t l = 0; l < s.sites; ++l)
+            for (int x = 0; x < s.N; ++x) init[l * planes + 2 * (x * s.N + x)] = 1.f;
     UCMC_HIP(hipMemcpy(s.field[0].p, init.data(), init.size() * 4, hipMemcpyHostToDevice));
     s.scratch.alloc(64);
 }
@@ -664,7 +668,7 @@ void gauge_upload(System& s, const double* host)
     std::vector<float> f(s.sites * planes);
     for (std::uint32_t l = 0; l < s.sites; ++l) {
         const std::uint32_t d = dev_of_ref(g, l);
-        for (std::size_t k = 0; k < planes; ++k) f[k * s.sites + d] = static_cast<float>(host[l * planes + k]);
+        for (std::size_t k = 0; k < planes; ++k) f[d * planes + k] = static_cast<float>(host[l * planes + k]);
     }
     UCMC_HIP(hipMemcpy(s.field[s.cur].p, f.data(), f.size() * 4, hipMemcpyHostToDevice));
 }
@@ -678,7 +682,7 @@ void gauge_download(System& s, double* host)
     UCMC_HIP(hipMemcpy(f.data(), s.field[s.cur].p, f.size() * 4, hipMemcpyDeviceToHost));
     for (std::uint32_t l = 0; l < s.sites; ++l) {
         const std::uint32_t d = dev_of_ref(g, l);
-        for (std::size_t k = 0; k < planes; ++k) host[l * planes + k] = f[k * s.sites + d];
+        for (std::size_t k = 0; k < planes; ++k) host[l * planes + k] = f[d * planes + k];
     }
 }
 
