@@ -84,6 +84,7 @@ typedef struct ucmc_system ucmc_system;
 #define UCMC_XY_ANGLE      5 /* XY spin stored as an angle (no reference model) */
 #define UCMC_PLAQ_ISING    6 /* J1 + 4-spin plaquette Ising (no reference model) */
 #define UCMC_XY_PT         7 /* batch of XY-angle replicas under parallel tempering */
+#define UCMC_ISING_CELLS   8 /* CellModel<IsingDof, Term, List>, tabulated term (model/cell.hpp) */
 
 /* proposals (dof/ising.hpp:39-62, dof/vector.hpp:35-122, dof/u1.hpp:35-50,
  * dof/sun.hpp:237-278) */
@@ -117,6 +118,28 @@ typedef struct {
     const uint32_t* offsets; /* sites + 1 entries */
     const uint32_t* bonds;   /* offsets[sites] entries */
 } ucmc_pair_graph;
+
+/* A CellModel over the Ising degree of freedom with any cell term of arity
+ * 1..4 (model/cell.hpp:16-99, term/cell.hpp:10-25), the term tabulated over the
+ * +-1 values of a cell's links.  Bit k of an index `bits` is set when link k
+ * of the cell holds -1.
+ *   cell_sites[c * arity + k]   site of link k of cell c (CellComplex::cell)
+ *   delta[(which << arity) | bits]
+ *                               term.delta(which, old, -old, links), links
+ *                               holding the current values (link `which`
+ *                               holds old), i.e. the Flip change of one cell
+ *   energy[bits]                term.energy(links)
+ * energy_diff_at(i, -s_i) is the sum of delta over the cells holding i in
+ * the order (cell, position) (CellComplex incidence, cell_complex.hpp:36-62),
+ * as CellModel::energy_diff_at sums it (model/cell.hpp:66-79). */
+typedef struct {
+    size_t          sites;
+    size_t          cells;
+    int             arity;
+    const uint32_t* cell_sites;  /* cells * arity entries */
+    const double*   delta;       /* arity << arity entries */
+    const double*   energy;      /* 1 << arity entries */
+} ucmc_ising_cells;
 
 typedef struct {
     int      kind;         /* UCMC_ISING ... */
@@ -198,6 +221,11 @@ int ucmc_xy_angle_create(const ucmc_pair_graph* graph, double J, int device,
                          ucmc_system** out);
 /* Periodic L x L Ising with nearest-neighbour J1 and 4-spin plaquette K. */
 int ucmc_plaquette_ising_create(size_t L, double J1, double K, int device, ucmc_system** out);
+/* CellModel<IsingDof, Term, List>(list, term) with the term tabulated as in
+ * ucmc_ising_cells (model/cell.hpp:31-35); the field starts all +1 (the
+ * reference's starts at zero, which is not an Ising state: upload one).
+ * Sites that share a cell get different colours (greedy, in site order). */
+int ucmc_ising_cells_create(const ucmc_ising_cells* model, int device, ucmc_system** out);
 int ucmc_system_destroy(ucmc_system* sys);
 int ucmc_system_info(const ucmc_system* sys, ucmc_info* out);
 /* The colour of every site in the Metropolis schedule (sites bytes). */

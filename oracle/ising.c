@@ -157,6 +157,74 @@ void orc_ising_metropolis_keyed_series(size_t n, const uint32_t* off, const uint
     }
 }
 
+static unsigned cell_bits(const uint32_t* cell, int A, const int8_t* s)
+{
+    unsigned bits = 0;
+    for (int k = 0; k < A; ++k)
+        if (s[cell[k]] < 0) bits |= 1u << k;
+    return bits;
+}
+
+int orc_ising_cells_metropolis(size_t n, size_t ncells, int A, const uint32_t* cells,
+                               const double* delta, int8_t* s, double beta,
+                               const uint8_t* colour, int ncolours, uint64_t run_seed,
+                               uint64_t sweep0, size_t nsweeps, uint64_t* accepts_out)
+{
+    /* incidence rows: counting sort of (cell, position) by site, stable in cell order */
+    size_t* off = calloc(n + 1, sizeof *off);
+    uint32_t* inc_cell = malloc((ncells * (size_t)A + 1) * sizeof *inc_cell);
+    uint8_t* inc_pos = malloc(ncells * (size_t)A + 1);
+    size_t* cur = malloc((n + 1) * sizeof *cur);
+    if (!off || !inc_cell || !inc_pos || !cur) {
+        free(off);
+        free(inc_cell);
+        free(inc_pos);
+        free(cur);
+        return -1;
+    }
+    for (size_t c = 0; c < ncells; ++c)
+        for (int k = 0; k < A; ++k) ++off[cells[c * A + k] + 1];
+    for (size_t i = 0; i < n; ++i) off[i + 1] += off[i];
+    memcpy(cur, off, n * sizeof *cur);
+    for (size_t c = 0; c < ncells; ++c)
+        for (int k = 0; k < A; ++k) {
+            const size_t q = cur[cells[c * A + k]]++;
+            inc_cell[q] = (uint32_t)c;
+            inc_pos[q] = (uint8_t)k;
+        }
+    for (size_t k = 0; k < nsweeps; ++k) {
+        const uint64_t key = orc_key(run_seed, sweep0 + k);
+        uint64_t acc = 0;
+        for (int col = 0; col < ncolours; ++col)
+            for (size_t i = 0; i < n; ++i) {
+                if (colour[i] != col) continue;
+                double dE = 0.0;
+                for (size_t q = off[i]; q < off[i + 1]; ++q)
+                    dE += delta[((unsigned)inc_pos[q] << A) | cell_bits(cells + (size_t)inc_cell[q] * A, A, s)];
+                const double log_p = -beta * dE + 0.0;
+                if (log_p >= 0.0 ||
+                    log((double)orc_ising_u32_inline(key, (uint32_t)i) * 0x1.0p-32) < log_p) {
+                    s[i] = (int8_t)(-s[i]);
+                    ++acc;
+                }
+            }
+        if (accepts_out) accepts_out[k] = acc;
+    }
+    free(off);
+    free(inc_cell);
+    free(inc_pos);
+    free(cur);
+    return 0;
+}
+
+double orc_ising_cells_energy(size_t ncells, int A, const uint32_t* cells, const double* energy,
+                              const int8_t* s)
+{
+    double e = 0.0;
+    for (size_t c = 0; c < ncells; ++c) e += energy[cell_bits(cells + c * A, A, s)];
+    return e;
+}
+
 /* driver.hpp:148-206 with FixedAxis (dof/ising.hpp:53-62): r = 1. */
 static uint64_t ising_wolff_flip_seq(size_t n, const uint32_t* off, const uint32_t* bonds,
                                      int8_t* s, int32_t J, double beta, orc_pcg32* rng,

@@ -127,6 +127,20 @@ void orc_ising_metropolis_keyed_series(size_t n, const uint32_t* off, const uint
                                        int8_t* s, int32_t J, double beta, const uint8_t* colour,
                                        int ncolours, uint64_t run_seed, uint64_t sweep0,
                                        size_t nsweeps, uint64_t* accepts_out, double* e_out);
+/* CellModel<IsingDof, Term, CellComplex<A>> with the term tabulated as in
+ * ucmc_ising_cells (include/ucmc.h): energy_diff_at (model/cell.hpp:66-79)
+ * sums delta[(which << A) | bits] over the site's (cell, position) incidence
+ * rows, built as CellComplex builds them (cell_complex.hpp:36-62); keyed
+ * Metropolis with Flip in colour order (driver.hpp:73-87), site i drawing
+ * orc_ising_u32(key, i).  accepts_out[k] = updates in sweep k (may be NULL).
+ * Returns -1 when out of memory. */
+int orc_ising_cells_metropolis(size_t n, size_t ncells, int A, const uint32_t* cells,
+                               const double* delta, int8_t* s, double beta,
+                               const uint8_t* colour, int ncolours, uint64_t run_seed,
+                               uint64_t sweep0, size_t nsweeps, uint64_t* accepts_out);
+/* CellModel::total_energy (model/cell.hpp:57-62) of the tabulated term. */
+double orc_ising_cells_energy(size_t ncells, int A, const uint32_t* cells, const double* energy,
+                              const int8_t* s);
 /* Reference chain: Wolff<Ising, FixedAxis, Pcg32>; burn flips then nflips flips
  * with the total energy recorded after each (driver.hpp:130-240). */
 void orc_ising_wolff_seq(size_t n, const uint32_t* off, const uint32_t* bonds, int8_t* s,

@@ -83,6 +83,9 @@ def oracle():
          u64, u64, sz, ou64, of64)
     _sig(L, "orc_ising_metropolis_keyed", None, sz, u32p, u32p, i8p, i32, f64, u8p, cint, u64,
          u64, sz, ou64)
+    _sig(L, "orc_ising_cells_metropolis", cint, sz, sz, cint, u32p, f64p, i8p, f64, u8p, cint, u64,
+         u64, sz, ou64)
+    _sig(L, "orc_ising_cells_energy", f64, sz, cint, u32p, f64p, i8p)
     _sig(L, "orc_ising_wolff_seq", None, sz, u32p, u32p, i8p, i32, f64, u64, u64, sz, sz, of64,
          ou64)
     _sig(L, "orc_ising_wolff_keyed", None, sz, u32p, u32p, u32p, i8p, i32, f64, u64, u64, sz,
@@ -295,3 +298,58 @@ def random_spins(n, seed, stream=0):
     out = np.empty(n, np.uint32)
     oracle().orc_pcg32_block(seed, stream, n, out)
     return np.where(out.astype(np.float64) * 2.0 ** -32 < 0.5, -1, 1).astype(np.int8)
+
+
+# ---- tabulated Ising cell terms (ucmc_ising_cells) ----
+def cell_values(bits, A):
+    """The +-1 link values of index `bits` (bit k set: link k holds -1)."""
+    return [(-1 if (bits >> k) & 1 else 1) for k in range(A)]
+
+
+def delta_from_energy(energy, A):
+    """delta[which, bits] = energy(link `which` flipped) - energy(bits): the
+    Flip change of one cell for a term whose delta is its energy difference."""
+    e = np.asarray(energy, np.float64)
+    d = np.empty((A, 1 << A), np.float64)
+    for w in range(A):
+        for b in range(1 << A):
+            d[w, b] = e[b ^ (1 << w)] - e[b]
+    return d
+
+
+def ising_pair_term(J):
+    """IsingTerm (term/ising.hpp): E = -J s0 s1, arity 2."""
+    e = np.array([-J * np.prod(cell_values(b, 2)) for b in range(4)], np.float64)
+    return delta_from_energy(e, 2), e
+
+
+def pair_cells(off, bonds):
+    """Each bond occurrence (i, j), i < j, of a CSR pair graph as one arity-2 cell."""
+    cells = [(i, int(j)) for i in range(len(off) - 1) for j in bonds[off[i]:off[i + 1]] if i < j]
+    return np.array(cells, np.uint32).reshape(-1, 2)
+
+
+def square_plaquettes(Lx, Ly):
+    """The Lx*Ly plaquettes (x, y), (x+1, y), (x+1, y+1), (x, y+1) of a periodic lattice."""
+    c = []
+    for y in range(Ly):
+        for x in range(Lx):
+            xp, yp = (x + 1) % Lx, (y + 1) % Ly
+            c.append((y * Lx + x, y * Lx + xp, yp * Lx + xp, yp * Lx + x))
+    return np.array(c, np.uint32)
+
+
+def cells_colouring(n, cells):
+    """Greedy colouring, in site order, of the graph of sites that share a cell."""
+    nbr = [set() for _ in range(n)]
+    for cell in cells:
+        for a in cell:
+            nbr[int(a)].update(int(b) for b in cell if b != a)
+    col = np.full(n, 255, np.uint8)
+    for i in range(n):
+        used = {int(col[j]) for j in nbr[i] if col[j] != 255}
+        c = 0
+        while c in used:
+            c += 1
+        col[i] = c
+    return col, int(col.max()) + 1 if n else 0

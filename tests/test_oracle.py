@@ -597,3 +597,55 @@ def test_oracle_against_compiled_reference_random_cases():
         o.orc_ising_metropolis_keyed(L * L, off, b, a1, 1, 0.37, col, 2, seed, 4, 3, acc1)
         r.ref_ising_keyed(L * L, len(ei), ei, ej, a2, 1, 0.37, col, 2, seed, 4, 3, acc2)
         assert np.array_equal(a1, a2) and np.array_equal(acc1, acc2)
+
+
+@pytest.mark.parametrize("Lx,Ly,beta,J", [(8, 8, 0.44, 1), (6, 10, 0.3, -1), (4, 4, 0.8, 2)])
+def test_cells_oracle_reduces_to_ising(Lx, Ly, beta, J):
+    """The tabulated cell-term replay with IsingTerm over the bonds (arity 2)
+    is the keyed Ising replay: same fields and accept counts, bit for bit, and
+    the same total energy."""
+    from oracle_lib import ising_pair_term, pair_cells
+    o = oracle()
+    off, b = csr_square(Lx, Ly)
+    n = Lx * Ly
+    cells = pair_cells(off, b)
+    delta, energy = ising_pair_term(J)
+    col = checkerboard((Lx, Ly))
+    s0 = random_spins(n, 31 + Lx, 2)
+    s1, s2 = s0.copy(), s0.copy()
+    a1, a2 = np.zeros(6, np.uint64), np.zeros(6, np.uint64)
+    o.orc_ising_metropolis_keyed(n, off, b, s1, J, beta, col, 2, 5, 3, 6, a1)
+    assert o.orc_ising_cells_metropolis(n, len(cells), 2, cells, delta.reshape(-1), s2, beta, col, 2, 5, 3, 6,
+                                        a2) == 0
+    assert np.array_equal(s1, s2) and np.array_equal(a1, a2)
+    e = sum(o.orc_ising_energy_at(off, b, s1, J, i) for i in range(n)) * 0.5
+    assert o.orc_ising_cells_energy(len(cells), 2, cells, energy, s2) == e
+
+
+def test_cells_oracle_plaquette_chain_samples_exact_distribution():
+    """A 4-spin plaquette term, E = -K s0 s1 s2 s3, on a periodic 3x3 lattice:
+    the keyed replay's energy average against exact enumeration of 2^9 states."""
+    from oracle_lib import cells_colouring, delta_from_energy, square_plaquettes, cell_values
+    o = oracle()
+    L, K, beta = 3, 1.0, 0.4
+    cells = square_plaquettes(L, L)
+    energy = np.array([-K * np.prod(cell_values(b, 4)) for b in range(16)])
+    delta = delta_from_energy(energy, 4)
+    n = L * L
+    col, ncol = cells_colouring(n, cells)
+    Z = Es = 0.0
+    for state in range(1 << n):
+        s = np.array([-1 if (state >> i) & 1 else 1 for i in range(n)], np.int8)
+        e = o.orc_ising_cells_energy(len(cells), 4, cells, energy, s)
+        w = math.exp(-beta * e)
+        Z += w
+        Es += w * e
+    exact = Es / Z
+    s = np.ones(n, np.int8)
+    es = []
+    for k in range(20000):
+        o.orc_ising_cells_metropolis(n, len(cells), 4, cells, delta.reshape(-1), s, beta, col, ncol, 11, k, 1, None)
+        es.append(o.orc_ising_cells_energy(len(cells), 4, cells, energy, s))
+    from universal_classical_montecarlo_amd import stats
+    mean, err, _ = stats.mean_error(np.array(es))
+    assert abs(mean - exact) < 4 * err + 1e-3, (mean, err, exact)

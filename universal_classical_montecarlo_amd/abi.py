@@ -24,6 +24,7 @@ UCMC_E_DEVICE = -5
 UCMC_E_NO_DEVICE = -6
 
 ISING, VECTOR, U1_GAUGE, SUN_GAUGE, XY_ANGLE, PLAQ_ISING = 1, 2, 3, 4, 5, 6
+ISING_CELLS = 8
 PROP_FLIP, PROP_SPHERE, PROP_ROTATION, PROP_U1_SHIFT, PROP_SU_SHIFT = 1, 2, 3, 4, 5
 SCHED_CSR, SCHED_CHECKERBOARD, SCHED_FUSED, SCHED_CUBIC, SCHED_LINK8, SCHED_FOUR = 1, 2, 3, 4, 5, 6
 RECORD_ENERGY, RECORD_ORDER, RECORD_ACCEPTS, FORCE_SIMPLE = 1, 2, 4, 8
@@ -34,6 +35,7 @@ EXPORTED = [
     "ucmc_abi_version", "ucmc_last_error", "ucmc_device_count", "ucmc_key",
     "ucmc_ising_create", "ucmc_ising_square_create", "ucmc_vector_create", "ucmc_u1_gauge_create",
     "ucmc_sun_gauge_create", "ucmc_xy_angle_create", "ucmc_plaquette_ising_create",
+    "ucmc_ising_cells_create",
     "ucmc_system_destroy", "ucmc_system_info", "ucmc_colouring", "ucmc_field_upload",
     "ucmc_field_download", "ucmc_metropolis_run", "ucmc_metropolis_trace", "ucmc_wolff_run",
     "ucmc_observe", "ucmc_reunitarize", "ucmc_metropolis_bench", "ucmc_pt_create",
@@ -46,6 +48,12 @@ EXPORTED = [
 class PairGraph(C.Structure):
     _fields_ = [("sites", C.c_size_t), ("offsets", C.POINTER(C.c_uint32)),
                 ("bonds", C.POINTER(C.c_uint32))]
+
+
+class IsingCells(C.Structure):
+    _fields_ = [("sites", C.c_size_t), ("cells", C.c_size_t), ("arity", C.c_int),
+                ("cell_sites", C.POINTER(C.c_uint32)), ("delta", C.POINTER(C.c_double)),
+                ("energy", C.POINTER(C.c_double))]
 
 
 class Info(C.Structure):
@@ -109,6 +117,7 @@ def lib():
     L.ucmc_xy_angle_create.argtypes = [C.POINTER(PairGraph), C.c_double, C.c_int, C.POINTER(vp)]
     L.ucmc_plaquette_ising_create.argtypes = [C.c_size_t, C.c_double, C.c_double, C.c_int,
                                               C.POINTER(vp)]
+    L.ucmc_ising_cells_create.argtypes = [C.POINTER(IsingCells), C.c_int, C.POINTER(vp)]
     L.ucmc_system_destroy.argtypes = [vp]
     L.ucmc_system_info.argtypes = [vp, C.POINTER(Info)]
     L.ucmc_colouring.argtypes = [vp, C.c_void_p]
@@ -223,6 +232,22 @@ class System:
         h = C.c_void_p()
         check(lib().ucmc_xy_angle_create(C.byref(g), float(J), device, C.byref(h)))
         return cls(h, np.float64, 1)
+
+    @classmethod
+    def ising_cells(cls, sites, cell_sites, delta, energy, device=0):
+        """CellModel<IsingDof, Term, List> with the term tabulated: cell_sites
+        (cells x arity), delta (arity x 2^arity: [which, bits]), energy (2^arity);
+        bit k of an index is set when link k holds -1."""
+        cs = np.ascontiguousarray(cell_sites, dtype=np.uint32)
+        arity = cs.shape[1] if cs.ndim == 2 else 1
+        d = np.ascontiguousarray(delta, dtype=np.float64).reshape(-1)
+        e = np.ascontiguousarray(energy, dtype=np.float64).reshape(-1)
+        m = IsingCells(int(sites), cs.shape[0], int(arity),
+                       cs.ctypes.data_as(C.POINTER(C.c_uint32)), d.ctypes.data_as(C.POINTER(C.c_double)),
+                       e.ctypes.data_as(C.POINTER(C.c_double)))
+        h = C.c_void_p()
+        check(lib().ucmc_ising_cells_create(C.byref(m), device, C.byref(h)))
+        return cls(h, np.int8, 1)
 
     @classmethod
     def plaquette_ising(cls, L, J1, K, device=0):

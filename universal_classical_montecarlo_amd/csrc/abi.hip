@@ -36,7 +36,7 @@ std::size_t System::device_bytes() const
 {
     return offsets.bytes + bonds.bytes + edge_id.bytes + colour_sites.bytes + field[0].bytes +
            field[1].bytes + scratch.bytes + scratch2.bytes + stamp.bytes + queue.bytes +
-           flip_out.bytes;
+           flip_out.bytes + cell_sites.bytes + cell_tab.bytes;
 }
 
 // square_lattice(Lx, Ly) (lattice/square.hpp:15-38) gives row 0 =
@@ -596,6 +596,41 @@ int ucmc_plaquette_ising_create(size_t L, double J1, double K, int device, ucmc_
     });
 }
 
+int ucmc_ising_cells_create(const ucmc_ising_cells* m, int device, ucmc_system** out)
+{
+    return guard([&] {
+        if (!out) fail(UCMC_E_INVALID_ARGUMENT, "null out");
+        *out = nullptr;
+        if (!m) fail(UCMC_E_INVALID_ARGUMENT, "null model");
+        if (m->arity < 1 || m->arity > 4) fail(UCMC_E_UNSUPPORTED, "cell arity must be 1 .. 4");
+        if (m->sites == 0) fail(UCMC_E_INVALID_ARGUMENT, "no sites");
+        if (m->sites > 0xFFFFFFFFull || m->cells >= (1ull << 30))
+            fail(UCMC_E_OVERFLOW, "CellComplex: too many sites or cells");
+        if ((m->cells && !m->cell_sites) || !m->delta || !m->energy) fail(UCMC_E_INVALID_ARGUMENT, "null table");
+        const std::size_t A = static_cast<std::size_t>(m->arity);
+        for (std::size_t k = 0; k < m->cells * A; ++k)
+            if (m->cell_sites[k] >= m->sites) fail(UCMC_E_OUT_OF_RANGE, "CellComplex: site out of range");
+        for (std::size_t k = 0; k < (A << A); ++k)
+            if (!std::isfinite(m->delta[k])) fail(UCMC_E_INVALID_ARGUMENT, "delta table is not finite");
+        for (std::size_t k = 0; k < (std::size_t{1} << A); ++k)
+            if (!std::isfinite(m->energy[k])) fail(UCMC_E_INVALID_ARGUMENT, "energy table is not finite");
+        auto* s = new ucmc_system();
+        try {
+            s->kind = UCMC_ISING_CELLS;
+            s->sites = m->sites;
+            s->arity = m->arity;
+            s->ncells = m->cells;
+            s->h_cells.assign(m->cell_sites, m->cell_sites + m->cells * A);
+            open_device(*s, device);
+            cells_setup(*s, m->delta, m->energy);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
 namespace {
 System* plaq(ucmc_system* sys)
 {
@@ -693,6 +728,7 @@ int ucmc_field_upload(ucmc_system* sys, const void* host, size_t bytes)
         if (!host) fail(UCMC_E_INVALID_ARGUMENT, "null host pointer");
         switch (s->kind) {
         case UCMC_ISING:
+        case UCMC_ISING_CELLS:
         case UCMC_PLAQ_ISING: {
             if (bytes != s->sites) fail(UCMC_E_INVALID_ARGUMENT, "Ising field is sites int8 values");
             const auto* v = static_cast<const std::int8_t*>(host);
@@ -731,6 +767,7 @@ int ucmc_field_download(ucmc_system* sys, void* host, size_t bytes)
         if (!host) fail(UCMC_E_INVALID_ARGUMENT, "null host pointer");
         switch (s->kind) {
         case UCMC_ISING:
+        case UCMC_ISING_CELLS:
         case UCMC_PLAQ_ISING:
             if (bytes != s->sites) fail(UCMC_E_INVALID_ARGUMENT, "Ising field is sites int8 values");
             UCMC_HIP(hipMemcpyAsync(host, s->field[s->cur].p, bytes, hipMemcpyDeviceToHost, s->stream));
@@ -772,6 +809,7 @@ int ucmc_metropolis_run(ucmc_system* sys, const ucmc_metropolis_params* p,
         case UCMC_SUN_GAUGE: gauge_metropolis(*s, *p, series, nullptr, nullptr); break;
         case UCMC_XY_ANGLE: xy_metropolis(*s, *p, series, nullptr, nullptr); break;
         case UCMC_PLAQ_ISING: fpi_metropolis(*s, *p, series, nullptr, nullptr); break;
+        case UCMC_ISING_CELLS: cells_metropolis(*s, *p, series, nullptr, nullptr); break;
         default: fail(UCMC_E_UNSUPPORTED, "model kind");
         }
     });
@@ -792,6 +830,7 @@ int ucmc_metropolis_trace(ucmc_system* sys, const ucmc_metropolis_params* p, flo
         case UCMC_SUN_GAUGE: gauge_metropolis(*s, *p, nullptr, accepted, dE); break;
         case UCMC_XY_ANGLE: xy_metropolis(*s, *p, nullptr, accepted, dE); break;
         case UCMC_PLAQ_ISING: fpi_metropolis(*s, *p, nullptr, accepted, dE); break;
+        case UCMC_ISING_CELLS: cells_metropolis(*s, *p, nullptr, accepted, dE); break;
         default: fail(UCMC_E_UNSUPPORTED, "model kind");
         }
     });
@@ -824,6 +863,7 @@ int ucmc_observe(ucmc_system* sys, ucmc_observables* out)
         case UCMC_SUN_GAUGE: gauge_observe(*s, *out); break;
         case UCMC_XY_ANGLE: xy_observe(*s, *out); break;
         case UCMC_PLAQ_ISING: fpi_observe(*s, *out); break;
+        case UCMC_ISING_CELLS: cells_observe(*s, *out); break;
         default: fail(UCMC_E_UNSUPPORTED, "model kind");
         }
     });
@@ -852,6 +892,7 @@ int ucmc_metropolis_bench(ucmc_system* sys, const ucmc_metropolis_params* p, int
         case UCMC_SUN_GAUGE: n = gauge_bench(*s, *p, repeats, ms); break;
         case UCMC_XY_ANGLE: n = xy_bench(*s, *p, repeats, ms); break;
         case UCMC_PLAQ_ISING: n = fpi_bench(*s, *p, repeats, ms); break;
+        case UCMC_ISING_CELLS: n = cells_bench(*s, *p, repeats, ms); break;
         default: fail(UCMC_E_UNSUPPORTED, "model kind");
         }
         *elapsed_ms = ms;

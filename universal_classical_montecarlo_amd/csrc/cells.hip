@@ -1,0 +1,288 @@
+// cells.hip -- CellModel over the Ising degree of freedom with a user cell term
+// (model/cell.hpp:16-99, term/cell.hpp:10-25), the term tabulated by the caller
+// over the +-1 values of a cell's links (ucmc_ising_cells in include/ucmc.h).
+//
+// Metropolis with Flip (dof/ising.hpp:27-38) as in driver.hpp:73-87, colour
+// class by colour class (sites that share a cell never share a colour), one
+// thread per site: dE = the sum over the site's cells, in incidence order
+// (cell, position) as CellModel::energy_diff_at sums it (model/cell.hpp:66-79),
+// of the tabulated Flip change, in FP64; accept when log_p = -beta dE >= 0 or
+// log(u) < log_p with the site's 32-bit Ising uniform u (the word the other
+// Ising kernels draw, ising_u32).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "ucmc_core.hpp"
+#include "ucmc_device.hpp"
+
+namespace ucmc {
+namespace {
+
+struct CellArgs {
+    std::int8_t* s;
+    const std::uint32_t* inc_off;   // sites + 1
+    const std::uint32_t* inc;       // cell << 2 | position, per site in (cell, position) order
+    const std::uint32_t* cells;     // cells * A site ids
+    const double* delta;            // A << A
+    const std::uint32_t* sites;     // the colour class
+    std::uint32_t count;
+    double beta;
+    std::uint64_t key;
+    unsigned long long* accepts;    // nullable
+    float* trace_dE;                // nullable
+    std::uint8_t* trace_acc;        // nullable
+};
+
+template <int A>
+__global__ __launch_bounds__(256) void cells_colour(CellArgs a)
+{
+    const std::uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    bool acc = false;
+    if (t < a.count) {
+        const std::uint32_t i = a.sites[t];
+        double dE = 0.0;
+        for (std::uint32_t q = a.inc_off[i]; q < a.inc_off[i + 1]; ++q) {
+            const std::uint32_t e = a.inc[q];
+            const std::uint32_t* c = a.cells + static_cast<std::size_t>(e >> 2) * A;
+            std::uint32_t bits = 0;
+#pragma unroll
+            for (int k = 0; k < A; ++k) bits |= (a.s[c[k]] < 0 ? 1u : 0u) << k;
+            dE += a.delta[((e & 3u) << A) | bits];
+        }
+        // driver.hpp:79-84, Flip's log_ratio is 0; the uniform is drawn only
+        // when log_p < 0
+        const double log_p = -a.beta * dE + 0.0;
+        acc = log_p >= 0.0 || accept_draw(ising_u32(a.key, i), log_p);
+        if (acc) a.s[i] = static_cast<std::int8_t>(-a.s[i]);
+        if (a.trace_dE) {
+            a.trace_dE[i] = static_cast<float>(dE);
+            a.trace_acc[i] = acc ? 1 : 0;
+        }
+    }
+    if (a.accepts) {
+        const unsigned long long n = __popcll(__ballot(acc));
+        if ((threadIdx.x & 63) == 0 && n) atomicAdd(a.accepts, n);
+    }
+}
+
+__device__ __forceinline__ double wave_sum_f64(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// out[0] (double): sum over cells of energy[bits]; out[1] (int64): sum of s.
+template <int A>
+__global__ __launch_bounds__(256) void cells_reduce(const std::int8_t* s, std::size_t sites, const std::uint32_t* cells,
+                                                    std::size_t ncells, const double* energy, double* out)
+{
+    double e = 0.0;
+    long long m = 0;
+    const std::size_t stride = static_cast<std::size_t>(gridDim.x) * blockDim.x;
+    for (std::size_t c = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; c < ncells; c += stride) {
+        std::uint32_t bits = 0;
+#pragma unroll
+        for (int k = 0; k < A; ++k) bits |= (s[cells[c * A + k]] < 0 ? 1u : 0u) << k;
+        e += energy[bits];
+    }
+    for (std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; i < sites; i += stride) m += s[i];
+    e = wave_sum_f64(e);
+    m = static_cast<long long>(wave_sum_u64(static_cast<unsigned long long>(m)));
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(out, e);
+        atomicAdd(reinterpret_cast<unsigned long long*>(out + 1), static_cast<unsigned long long>(m));
+    }
+}
+
+template <class F>
+void with_arity(int A, F&& f)
+{
+    switch (A) {
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 3: f(std::integral_constant<int, 3>{}); break;
+    default: f(std::integral_constant<int, 4>{}); break;
+    }
+}
+
+void reduce_launch(System& s, double* dev_out)
+{
+    const unsigned blocks = static_cast<unsigned>(
+        std::min<std::size_t>((std::max(s.sites, s.ncells) + 255) / 256, 1024));
+    with_arity(s.arity, [&](auto A) {
+        hipLaunchKernelGGL(cells_reduce<decltype(A)::value>, dim3(blocks), dim3(256), 0, s.stream,
+                           s.field[s.cur].as<std::int8_t>(), s.sites, s.cell_sites.as<std::uint32_t>(), s.ncells,
+                           s.cell_tab.as<double>() + (s.arity << s.arity), dev_out);
+    });
+    UCMC_HIP(hipGetLastError());
+}
+
+void finish(const double* h, ucmc_observables& o)
+{
+    o.energy = h[0];
+    long long m;
+    static_assert(sizeof m == sizeof h[1]);
+    std::memcpy(&m, &h[1], sizeof m);
+    o.order[0] = static_cast<double>(m);
+}
+
+void sweep(System& s, double beta, std::uint64_t key, unsigned long long* accepts, float* trace_dE,
+           std::uint8_t* trace_acc)
+{
+    for (int c = 0; c < s.ncolours; ++c) {
+        CellArgs a{};
+        a.s = s.field[s.cur].as<std::int8_t>();
+        a.inc_off = s.offsets.as<std::uint32_t>();
+        a.inc = s.bonds.as<std::uint32_t>();
+        a.cells = s.cell_sites.as<std::uint32_t>();
+        a.delta = s.cell_tab.as<double>();
+        a.sites = s.colour_sites.as<std::uint32_t>() + s.colour_start[c];
+        a.count = static_cast<std::uint32_t>(s.colour_start[c + 1] - s.colour_start[c]);
+        a.beta = beta;
+        a.key = key;
+        a.accepts = accepts;
+        a.trace_dE = trace_dE;
+        a.trace_acc = trace_acc;
+        if (a.count == 0) continue;
+        with_arity(s.arity, [&](auto A) {
+            hipLaunchKernelGGL(cells_colour<decltype(A)::value>, dim3((a.count + 255) / 256), dim3(256), 0,
+                               s.stream, a);
+        });
+        UCMC_HIP(hipGetLastError());
+    }
+}
+
+void check_flip(const ucmc_metropolis_params& p)
+{
+    if (p.proposal != UCMC_PROP_FLIP) fail(UCMC_E_UNSUPPORTED, "an Ising cell model takes the Flip proposal");
+    if (!(p.beta == p.beta)) fail(UCMC_E_INVALID_ARGUMENT, "beta is NaN");
+}
+
+}  // namespace
+
+// Incidence rows as CellComplex builds them (cell_complex.hpp:36-62): for each
+// site, its (cell, position) pairs in cell order; colours from the graph of
+// sites that share a cell.
+void cells_setup(System& s, const double* delta, const double* energy)
+{
+    const int A = s.arity;
+    const std::size_t n = s.sites;
+    std::vector<std::uint32_t> off(n + 1, 0), inc(s.ncells * A);
+    for (std::uint32_t v : s.h_cells) ++off[v + 1];
+    for (std::size_t i = 0; i < n; ++i) off[i + 1] += off[i];
+    std::vector<std::uint32_t> cursor(off.begin(), off.end() - 1);
+    for (std::size_t c = 0; c < s.ncells; ++c)
+        for (int k = 0; k < A; ++k) inc[cursor[s.h_cells[c * A + k]]++] = static_cast<std::uint32_t>(c << 2) | k;
+    // sites sharing a cell (duplicates and i itself are harmless to the colouring)
+    std::vector<std::uint32_t> aoff(n + 1, 0), adj;
+    adj.reserve(inc.size() * (A - 1));
+    for (std::size_t i = 0; i < n; ++i) {
+        for (std::uint32_t q = off[i]; q < off[i + 1]; ++q) {
+            const std::size_t c = inc[q] >> 2;
+            for (int k = 0; k < A; ++k) {
+                const std::uint32_t j = s.h_cells[c * A + k];
+                if (j != i) adj.push_back(j);
+            }
+        }
+        aoff[i + 1] = static_cast<std::uint32_t>(adj.size());
+    }
+    s.ncolours = greedy_colouring(aoff.data(), adj.data(), n, s.colour);
+    s.schedule = UCMC_SCHED_CSR;
+    upload_colour_lists(s);
+    s.offsets.alloc(off.size() * 4);
+    UCMC_HIP(hipMemcpy(s.offsets.p, off.data(), off.size() * 4, hipMemcpyHostToDevice));
+    s.bonds.alloc(std::max<std::size_t>(inc.size(), 1) * 4);
+    if (!inc.empty()) UCMC_HIP(hipMemcpy(s.bonds.p, inc.data(), inc.size() * 4, hipMemcpyHostToDevice));
+    s.cell_sites.alloc(std::max<std::size_t>(s.h_cells.size(), 1) * 4);
+    if (!s.h_cells.empty())
+        UCMC_HIP(hipMemcpy(s.cell_sites.p, s.h_cells.data(), s.h_cells.size() * 4, hipMemcpyHostToDevice));
+    const std::size_t nd = static_cast<std::size_t>(A) << A, ne = std::size_t{1} << A;
+    std::vector<double> tab(delta, delta + nd);
+    tab.insert(tab.end(), energy, energy + ne);
+    s.cell_tab.alloc(tab.size() * 8);
+    UCMC_HIP(hipMemcpy(s.cell_tab.p, tab.data(), tab.size() * 8, hipMemcpyHostToDevice));
+    s.field[0].alloc(n);
+    UCMC_HIP(hipMemset(s.field[0].p, 1, n));   // all +1
+    s.scratch.alloc(64);
+}
+
+void cells_metropolis(System& s, const ucmc_metropolis_params& p, ucmc_observables* series,
+                      std::uint8_t* trace_acc, float* trace_dE)
+{
+    check_flip(p);
+    const bool rec = series && (p.flags & (UCMC_RECORD_ENERGY | UCMC_RECORD_ORDER | UCMC_RECORD_ACCEPTS));
+    // per sweep: [accepts (u64), energy (f64), order (i64)]
+    DevBuf rec_buf;
+    if (rec) {
+        rec_buf.alloc(p.nsweeps * 3 * 8);
+        UCMC_HIP(hipMemsetAsync(rec_buf.p, 0, rec_buf.bytes, s.stream));
+    }
+    DevBuf trace_buf;
+    float* d_dE = nullptr;
+    std::uint8_t* d_acc = nullptr;
+    if (trace_acc) {
+        trace_buf.alloc(s.sites * 5);
+        d_dE = trace_buf.as<float>();
+        d_acc = reinterpret_cast<std::uint8_t*>(d_dE + s.sites);
+    }
+    for (std::uint64_t k = 0; k < p.nsweeps; ++k) {
+        unsigned long long* slot = rec ? rec_buf.as<unsigned long long>() + 3 * k : nullptr;
+        sweep(s, p.beta, key_of(p.run_seed, p.sweep0 + k), (rec && (p.flags & UCMC_RECORD_ACCEPTS)) ? slot : nullptr,
+              d_dE, d_acc);
+        if (rec && (p.flags & (UCMC_RECORD_ENERGY | UCMC_RECORD_ORDER)))
+            reduce_launch(s, reinterpret_cast<double*>(slot + 1));
+    }
+    if (rec) {
+        std::vector<unsigned long long> h(p.nsweeps * 3);
+        UCMC_HIP(hipMemcpyAsync(h.data(), rec_buf.p, h.size() * 8, hipMemcpyDeviceToHost, s.stream));
+        UCMC_HIP(hipStreamSynchronize(s.stream));
+        for (std::uint64_t k = 0; k < p.nsweeps; ++k) {
+            ucmc_observables o{};
+            double d[2];
+            std::memcpy(d, &h[3 * k + 1], sizeof d);
+            finish(d, o);
+            o.accepted = h[3 * k];
+            series[k] = o;
+        }
+    }
+    if (trace_acc) {
+        UCMC_HIP(hipMemcpyAsync(trace_dE, d_dE, s.sites * 4, hipMemcpyDeviceToHost, s.stream));
+        UCMC_HIP(hipMemcpyAsync(trace_acc, d_acc, s.sites, hipMemcpyDeviceToHost, s.stream));
+    }
+    UCMC_HIP(hipStreamSynchronize(s.stream));
+}
+
+void cells_observe(System& s, ucmc_observables& o)
+{
+    UCMC_HIP(hipMemsetAsync(s.scratch.p, 0, 16, s.stream));
+    reduce_launch(s, s.scratch.as<double>());
+    double h[2];
+    UCMC_HIP(hipMemcpyAsync(h, s.scratch.p, 16, hipMemcpyDeviceToHost, s.stream));
+    UCMC_HIP(hipStreamSynchronize(s.stream));
+    finish(h, o);
+}
+
+std::uint64_t cells_bench(System& s, const ucmc_metropolis_params& p, int repeats, float& ms)
+{
+    check_flip(p);
+    hipEvent_t e0, e1;
+    UCMC_HIP(hipEventCreate(&e0));
+    UCMC_HIP(hipEventCreate(&e1));
+    UCMC_HIP(hipStreamSynchronize(s.stream));
+    UCMC_HIP(hipEventRecord(e0, s.stream));
+    for (int r = 0; r < repeats; ++r)
+        for (std::uint64_t k = 0; k < p.nsweeps; ++k)
+            sweep(s, p.beta, key_of(p.run_seed, p.sweep0 + r * p.nsweeps + k), nullptr, nullptr, nullptr);
+    UCMC_HIP(hipEventRecord(e1, s.stream));
+    UCMC_HIP(hipEventSynchronize(e1));
+    UCMC_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return static_cast<std::uint64_t>(repeats) * p.nsweeps * s.ncolours;
+}
+
+}  // namespace ucmc

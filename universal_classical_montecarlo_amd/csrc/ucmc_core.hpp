@@ -75,6 +75,11 @@ struct System {
     double J = 1.0;           // vector / XY J
     double model_beta = 1.0;  // gauge models: beta inside energy_diff_at
     double K = 0.0;           // plaquette Ising K
+    // tabulated Ising cell term: arity, cells, cell_sites [cells * arity],
+    // cell_tab = delta [arity << arity] then energy [1 << arity] (doubles)
+    int arity = 0;
+    std::size_t ncells = 0;
+    std::vector<std::uint32_t> h_cells;
 
     // device tables
     DevBuf offsets, bonds, edge_id;        // CSR + undirected edge ids (Wolff)
@@ -95,6 +100,7 @@ struct System {
     // label-propagation clusters: parent array and the per-flip control block
     DevBuf labels, cluster_state, label_part;   // + per-block partial sums of a flip
     DevBuf local_gram;   // SLMC: FP64 local-field Gram matrix of the training set
+    DevBuf cell_sites, cell_tab;   // tabulated Ising cell term
     std::uint32_t wolff_mark = 0;
     // parallel tempering batch (XY replicas)
     struct Pt {
@@ -145,6 +151,11 @@ void pt_setup(System& s, std::size_t R, const std::size_t* L, const std::uint32_
 void pt_run(System& s, const ucmc_pt_params& p, double* series, std::uint32_t* temp_of);
 void pt_stats(const System& s, std::uint64_t* acc, std::uint64_t* sw_acc, std::uint64_t* sw_try);
 void pt_autotune(System& s, double width, std::uint32_t* tile, double* best_ms);
+void cells_setup(System& s, const double* delta, const double* energy);
+void cells_metropolis(System& s, const ucmc_metropolis_params& p, ucmc_observables* series,
+                      std::uint8_t* trace_acc, float* trace_dE);
+void cells_observe(System& s, ucmc_observables& o);
+std::uint64_t cells_bench(System& s, const ucmc_metropolis_params& p, int repeats, float& ms);
 void fpi_setup(System& s);
 void fpi_metropolis(System& s, const ucmc_metropolis_params& p, ucmc_observables* series,
                     std::uint8_t* trace_acc, float* trace_dE);
