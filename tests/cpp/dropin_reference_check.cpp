@@ -40,6 +40,23 @@
         }                                                                           \
     } while (0)
 
+// E = -K s0 s1 s2 s3: a user cell term the reference does not ship
+struct FourSpin {
+    using dtype = IsingSpin;
+    using energy_type = double;
+    static constexpr std::size_t arity = 4;
+    double K = 1.0;
+    energy_type energy(const dtype* const* l) const noexcept { return -K * (*l[0]) * (*l[1]) * (*l[2]) * (*l[3]); }
+    energy_type delta(std::size_t which, const dtype* old, const dtype* nw, const dtype* const* l) const noexcept
+    {
+        double rest = 1.0;
+        for (std::size_t k = 0; k < arity; ++k)
+            if (k != which) rest *= *l[k];
+        return -K * (double(*nw) - double(*old)) * rest;
+    }
+};
+static_assert(CellTerm<FourSpin, IsingDof>);
+
 template <class M>
 double cell_mean_plaquette(const M& model)   // 1 - S / (beta cells), S = sum beta (1 - P)
 {
@@ -248,6 +265,57 @@ int main()
                     c4, su3_cold4);
         EXPECT(std::abs(c0) < 0.05);
         EXPECT(std::abs(c4 - su3_cold4) < 0.05);
+    }
+    {   // CellModel<IsingDof, Term, CellComplex<A>> with user terms (tabulated on the device):
+        // IsingTerm over the bonds as arity-2 cells (model_check.cpp:172 exact value), and a
+        // 4-spin plaquette term on the 4 x 4 plaquettes against exact enumeration (2^16 states)
+        std::vector<Cell<2>> bonds;
+        for (std::size_t i = 0; i < N; ++i)
+            for (std::size_t a = 0; a < graph.degree(i); ++a)
+                if (i < graph.bonds(i)[a].j)
+                    bonds.push_back(Cell<2>{{static_cast<std::uint32_t>(i), static_cast<std::uint32_t>(graph.bonds(i)[a].j)}});
+        using PM = CellModel<IsingDof, IsingTerm, CellComplex<2>>;
+        PM pm(CellComplex<2>(N, bonds), IsingTerm{1});
+        const IsingSpin up = 1;
+        for (std::size_t i = 0; i < N; ++i) pm.update(i, &up);
+        ucmc::gpu::Metropolis mp(pm, Flip{}, Pcg32(31, 1), 0.3);
+        mp.run(2000);
+        double sum = 0.0;
+        mp.sample(4000, [&](const PM& m) { sum += double(m.total_energy()); });
+        const double pavg = sum / (4000.0 * double(N));
+        std::printf("CellModel<IsingTerm> over bonds: %.5f (exact %.5f)\n", pavg, ising_exact);
+        EXPECT(std::abs(pavg - ising_exact) < 0.03);
+
+        std::vector<Cell<4>> plaq;
+        for (std::uint32_t y = 0; y < L; ++y)
+            for (std::uint32_t x = 0; x < L; ++x) {
+                const std::uint32_t xp = (x + 1) % L, yp = (y + 1) % L;
+                plaq.push_back(Cell<4>{{y * std::uint32_t(L) + x, y * std::uint32_t(L) + xp,
+                                        yp * std::uint32_t(L) + xp, yp * std::uint32_t(L) + x}});
+            }
+        const double K = 0.8, beta = 0.5;
+        double Z = 0.0, EZ = 0.0;
+        for (std::uint32_t st = 0; st < (1u << N); ++st) {
+            double e = 0.0;
+            for (const auto& c : plaq) {
+                double p = 1.0;
+                for (auto v : c.sites) p *= ((st >> v) & 1) ? -1.0 : 1.0;
+                e += -K * p;
+            }
+            const double w = std::exp(-beta * e);
+            Z += w;
+            EZ += w * e;
+        }
+        using QM = CellModel<IsingDof, FourSpin, CellComplex<4>>;
+        QM qm(CellComplex<4>(N, plaq), FourSpin{K});
+        for (std::size_t i = 0; i < N; ++i) qm.update(i, &up);
+        ucmc::gpu::Metropolis mq(qm, Flip{}, Pcg32(32, 1), beta);
+        mq.run(2000);
+        double qsum = 0.0;
+        mq.sample(20000, [&](const QM& m) { qsum += m.total_energy(); });
+        const double qavg = qsum / 20000.0;
+        std::printf("CellModel<4-spin term>: %.4f (exact %.4f)\n", qavg, EZ / Z);
+        EXPECT(std::abs(qavg - EZ / Z) < 0.15);
     }
     {   // exception types at the boundary mirror the reference's
         bool threw = false;

@@ -24,6 +24,9 @@
 //                    a CellModel (cells(), cell_sites(c), term().beta)
 //   SUGauge-like:    value_type with `dim` and an `m` array of complex<double>;
 //                    length() or CellModel cells as above
+//   Ising cells:     value_type int8_t, size 1, cells(), cell_sites(c) and any
+//                    term() of arity 1..4 (CellModel<IsingDof, Term, List>):
+//                    the term is tabulated and run by a table-driven kernel
 // with the proposal Flip / UniformSphereN / LocalRotation (has `delta`) /
 // U1Shift / SUShift (have `width`).  Couplings the models keep private (J,
 // beta) are read back through the models' own energy functions.
@@ -181,6 +184,44 @@ std::size_t plaquette_length(const M& m)
     return L;
 }
 
+// CellModel over the Ising degree of freedom (model/cell.hpp:16-99) with any
+// cell term of arity 1..4: the term is tabulated over the +-1 values of a
+// cell's links (energy and the Flip change at each position, as
+// CellModel::energy_diff_at calls it: links hold the current values and
+// position `which` changes from old to -old), and one table-driven kernel runs
+// it (ucmc_ising_cells_create).
+template <class M>
+concept IsingCellTerm = requires(const M cm, std::size_t c) {
+    { cm.cells() } -> std::convertible_to<std::size_t>;
+    cm.cell_sites(c);
+    cm.term();
+} && std::same_as<typename M::value_type, std::int8_t> && (M::size == 1) && (M::arity >= 1) &&
+                        (M::arity <= 4);
+
+template <class M>
+void ising_cell_tables(const M& m, std::vector<double>& delta, std::vector<double>& energy)
+{
+    using T = typename M::value_type;
+    constexpr std::size_t A = M::arity;
+    const auto& term = m.term();
+    delta.assign(A << A, 0.0);
+    energy.assign(std::size_t{1} << A, 0.0);
+    for (std::size_t bits = 0; bits < (std::size_t{1} << A); ++bits) {
+        std::array<T, A> v{};
+        std::array<const T*, A> links{};
+        for (std::size_t k = 0; k < A; ++k) {
+            v[k] = static_cast<T>(((bits >> k) & 1) ? -1 : 1);
+            links[k] = &v[k];
+        }
+        energy[bits] = static_cast<double>(term.energy(links.data()));
+        for (std::size_t which = 0; which < A; ++which) {
+            const T old = v[which];
+            const T nw = static_cast<T>(-old);
+            delta[(which << A) | bits] = static_cast<double>(term.delta(which, &old, &nw, links.data()));
+        }
+    }
+}
+
 template <class M>
 std::size_t gauge_length(const M& m)
 {
@@ -289,13 +330,13 @@ void verify_pair_form(const M& m, double J)
 template <class M>
 struct Device
 {
-    static_assert(IsingLike<M> || VectorLike<M> || U1GaugeLike<M> || SUGaugeLike<M>,
+    static_assert(IsingLike<M> || IsingCellTerm<M> || VectorLike<M> || U1GaugeLike<M> || SUGaugeLike<M>,
                   "ucmc::gpu: no device kernel family for this model type");
     using T = typename M::value_type;
 
     static constexpr std::size_t doubles_per_site()
     {
-        if constexpr (IsingLike<M>) return 0;
+        if constexpr (IsingLike<M> || IsingCellTerm<M>) return 0;
         else if constexpr (VectorLike<M>) return M::size;
         else if constexpr (U1GaugeLike<M>) return 1;
         else return 2 * T::dim * T::dim;
@@ -314,6 +355,18 @@ struct Device
                 throw std::invalid_argument("ucmc::gpu: Ising coupling " + std::to_string(J) + " is not an integer");
             verify_pair_form(m, J);
             check(ucmc_ising_create(&g, static_cast<std::int32_t>(J), 0, &h));
+        } else if constexpr (IsingCellTerm<M>) {
+            constexpr std::size_t A = M::arity;
+            std::vector<std::uint32_t> cells(m.cells() * A);
+            for (std::size_t c = 0; c < m.cells(); ++c) {
+                const auto cs = m.cell_sites(c);
+                for (std::size_t k = 0; k < A; ++k) cells[c * A + k] = static_cast<std::uint32_t>(cs[k]);
+            }
+            std::vector<double> delta, energy;
+            ising_cell_tables(m, delta, energy);
+            const ucmc_ising_cells t{m.sites(), m.cells(), static_cast<int>(A), cells.data(), delta.data(),
+                                     energy.data()};
+            check(ucmc_ising_cells_create(&t, 0, &h));
         } else if constexpr (VectorLike<M>) {
             std::vector<std::uint32_t> off, b;
             rows_of(m, off, b);
@@ -333,7 +386,7 @@ struct Device
 
     static void upload(System& s, const M& m)
     {
-        if constexpr (IsingLike<M>) {
+        if constexpr (IsingLike<M> || IsingCellTerm<M>) {
             // PairModel starts every component at zero (model/pair.hpp:31-34):
             // an Ising field must be set to +-1 before a driver takes it
             for (std::size_t i = 0; i < m.sites(); ++i)
@@ -349,7 +402,7 @@ struct Device
     static void download(System& s, M& m)
     {
         const std::size_t n = m.sites();
-        if constexpr (IsingLike<M>) {
+        if constexpr (IsingLike<M> || IsingCellTerm<M>) {
             std::vector<std::int8_t> v(n);
             check(ucmc_field_download(s.get(), v.data(), n));
             for (std::size_t i = 0; i < n; ++i) m.update(i, &v[i]);
@@ -416,7 +469,7 @@ struct Device
 template <class M, class Q>
 constexpr int proposal_kind()
 {
-    if constexpr (IsingLike<M>) return UCMC_PROP_FLIP;
+    if constexpr (IsingLike<M> || IsingCellTerm<M>) return UCMC_PROP_FLIP;
     else if constexpr (VectorLike<M>) return HasDelta<Q> ? UCMC_PROP_ROTATION : UCMC_PROP_SPHERE;
     else if constexpr (U1GaugeLike<M>) return UCMC_PROP_U1_SHIFT;
     else return UCMC_PROP_SU_SHIFT;
