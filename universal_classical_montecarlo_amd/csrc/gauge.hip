@@ -491,10 +491,11 @@ __global__ __launch_bounds__(256) void su3_reunitarize(float* f, std::uint32_t n
 {
     const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nl) return;
+    float* lp = f + static_cast<std::size_t>(i) * 18;
     double re[9], im[9];
     for (int k = 0; k < 9; ++k) {
-        re[k] = f[static_cast<std::size_t>(i) * 18 + 2 * k];
-        im[k] = f[static_cast<std::size_t>(i) * 18 + 2 * k + 1];
+        re[k] = lp[2 * k];
+        im[k] = lp[2 * k + 1];
     }
     double n0 = 0.0;
     for (int j = 0; j < 3; ++j) n0 += re[j] * re[j] + im[j] * im[j];
@@ -528,8 +529,8 @@ __global__ __launch_bounds__(256) void su3_reunitarize(float* f, std::uint32_t n
         im[6 + k] = -ci;
     }
     for (int k = 0; k < 9; ++k) {
-        f[static_cast<std::size_t>(i) * 18 + 2 * k] = static_cast<float>(re[k]);
-        f[static_cast<std::size_t>(i) * 18 + 2 * k + 1] = static_cast<float>(im[k]);
+        lp[2 * k] = static_cast<float>(re[k]);
+        lp[2 * k + 1] = static_cast<float>(im[k]);
     }
 }
 
