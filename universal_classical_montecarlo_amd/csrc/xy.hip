@@ -185,12 +185,22 @@ __global__ __launch_bounds__(256) void pt_sweep(PtArgs a)
     };
     auto wy = [&](std::uint32_t ly) { return wrap(ty0 + ly); };
     auto wx = [&](std::uint32_t lx) { return wrap(tx0 + lx); };
-    for (std::uint32_t q = threadIdx.x; q < W * W; q += blockDim.x) {
-        const std::uint32_t ly = q / W, lx = q % W;
-        float sv, cv;
-        sincos_turn_any(in[static_cast<std::size_t>(wy(ly)) * L + wx(lx)] * kInvTwoPiF, &sv, &cv);
-        cs[q] = cv;
-        sn[q] = sv;
+    {
+        // (ly, lx) of position q, stepped by blockDim.x without dividing
+        std::uint32_t ly = threadIdx.x / W, lx = threadIdx.x - ly * W;
+        const std::uint32_t dly = blockDim.x / W, dlx = blockDim.x - dly * W;
+        for (std::uint32_t q = threadIdx.x; q < W * W; q += blockDim.x) {
+            float sv, cv;
+            sincos_turn_any(in[static_cast<std::size_t>(wy(ly)) * L + wx(lx)] * kInvTwoPiF, &sv, &cv);
+            cs[q] = cv;
+            sn[q] = sv;
+            ly += dly;
+            lx += dlx;
+            if (lx >= W) {
+                lx -= W;
+                ++ly;
+            }
+        }
     }
     __syncthreads();
     unsigned long long nacc = 0;
