@@ -37,6 +37,9 @@ def _gpu_run(sysm, spins, beta, run_seed, sweep0, nsweeps, flags=0):
     (512, 16, 3.0, 1, abi.SCHED_FUSED),
     (512, 64, 0.44, 1, abi.SCHED_FUSED),
     (1024, 96, 0.40, -1, abi.SCHED_FUSED),
+    # one 8-row block: the window's halo rows wrap onto the block's own rows
+    (512, 8, 0.44, 1, abi.SCHED_FUSED),
+    (1536, 8, 0.3, -1, abi.SCHED_FUSED),
 ])
 def test_ising_square_bit_exact(gpu, Lx, Ly, beta, J, sched):
     print(f"ising square {Lx}x{Ly} beta={beta} J={J}", flush=True)
