@@ -25,11 +25,6 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_SITE = 2.0  # int8 read + int8 write per site-update
 
 
-def random_spins(n, seed):
-    from oracle_lib import random_spins as rs
-    return rs(n, seed, 0)
-
-
 def traffic_from_profile():
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
     scaled by the copy-kernel calibration), if the summary has been committed."""
@@ -47,8 +42,9 @@ def cpu_baseline(L, seconds_target=12.0):
     own headers into oracle/_ref) on one host core, over a bounded number of
     sweeps of the same L x L lattice.  Falls back to the oracle restatement."""
     from oracle_lib import REF_SO, oracle, ref, square_edges, csr_square
+    from universal_classical_montecarlo_amd import lattices
     n = L * L
-    spins = random_spins(n, 7)
+    spins = lattices.random_ising(n, 7)
     kind = "reference"
     sweeps = 1
     if os.path.exists(REF_SO):
@@ -104,17 +100,13 @@ def suite(quick=False):
     """Throughput of the other workloads (one GPU): Heisenberg 128^3, Wolff,
     SU(3) 16^4 / 24^4, the 256-replica XY batch and SLMC, each bounded
     to a few seconds; reported beside the headline, not as `value`."""
-    from universal_classical_montecarlo_amd import abi
-    from oracle_lib import csr_cubic
+    from universal_classical_montecarlo_amd import abi, lattices
     out = {}
     # C2b: 3D Heisenberg 128^3, LocalRotation(0.5), beta = 0.693, 2-colour
     L = 64 if quick else 128
-    off, bonds = csr_cubic(L)
+    off, bonds = lattices.cubic_csr(L)
     v = abi.System.vector(3, off, bonds, 1.0)
-    rng = np.random.default_rng(5)
-    x = rng.normal(size=(L ** 3, 3))
-    x /= np.linalg.norm(x, axis=1, keepdims=True)
-    v.upload(x.reshape(-1))
+    v.upload(lattices.random_unit_vectors(L ** 3, 3, 5))
     v.metropolis(0.693, 5, 1, proposal=abi.PROP_ROTATION, width=0.5)
     ms, n = v.bench(0.693, 50, 2, proposal=abi.PROP_ROTATION, width=0.5)
     rate = L ** 3 * 50 / (ms * 1e-3)
@@ -175,6 +167,16 @@ def suite(quick=False):
                                  "hbm_frac_8B": 8 * pt.sites * 100 / dt / 1e9 / HBM_PEAK_GBS,
                                  "swap_acceptance": float(sacc.sum() / max(stry.sum(), 1))}
     pt.close()
+    # A user cell term through the generic table-driven path: the 4-spin plaquette
+    # term E = -K s0 s1 s2 s3 (K = 1) as CellModel<IsingDof, Term, CellComplex<4>>
+    L = 256 if quick else 1024
+    delta, energy = lattices.four_spin_term(1.0)
+    c = abi.System.ising_cells(L * L, lattices.square_plaquettes(L, L), delta, energy)
+    c.metropolis(0.5, 5, 1)
+    ms, n = c.bench(0.5, 50, 2)
+    out[f"ising_cells_plaquette_{L}"] = {"site_updates_per_s": L * L * 50 / (ms * 1e-3),
+                                         "colours": int(c.info.ncolours)}
+    c.close()
     # C5: SLMC vs plain GPU Metropolis, independent samples / s
     from universal_classical_montecarlo_amd import slmc
     out["slmc"] = slmc.compare(L=64 if quick else 256, nconfigs=4000 if quick else 20000,
@@ -186,19 +188,16 @@ def suite(quick=False):
 def profile_others():
     """Each of the other configurations at full size for a few steps, for
     rocprofv3 --kernel-trace (kernel durations of K2-K9)."""
-    from universal_classical_montecarlo_amd import abi, slmc
-    from oracle_lib import csr_cubic
+    from universal_classical_montecarlo_amd import abi, lattices, slmc
     L = 128
-    off, bonds = csr_cubic(L)
+    off, bonds = lattices.cubic_csr(L)
     v = abi.System.vector(3, off, bonds, 1.0)
-    x = np.random.default_rng(5).normal(size=(L ** 3, 3))
-    x /= np.linalg.norm(x, axis=1, keepdims=True)
-    v.upload(x.reshape(-1))
+    v.upload(lattices.random_unit_vectors(L ** 3, 3, 5))
     v.metropolis(0.693, 20, 1, proposal=abi.PROP_ROTATION, width=0.5)
     v.wolff(0.693, 50, 3)
     v.close()
     w = abi.System.ising_square(4096, 4096, 1)
-    w.upload(random_spins(4096 * 4096, 7))
+    w.upload(lattices.random_ising(4096 * 4096, 7))
     w.metropolis(0.44, 20, 5)
     w.wolff(0.44, 20, 6)
     w.close()
@@ -242,7 +241,7 @@ def main():
               "of this workload is one GPU's share; run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
-    from universal_classical_montecarlo_amd import abi
+    from universal_classical_montecarlo_amd import abi, lattices
     L = args.L
     if args.profile_others:
         profile_others()
@@ -250,7 +249,7 @@ def main():
     if args.profile:
         # 50 fused sweeps, then 50 copies of the same byte count (read L^2 + write L^2)
         s = abi.System.ising_square(L, L, 1)
-        s.upload(random_spins(L * L, 7))
+        s.upload(lattices.random_ising(L * L, 7))
         s.metropolis(0.44, 50, 2026)
         s.close()
         print(json.dumps({"profile": "ising_fused", "L": L,
@@ -258,7 +257,7 @@ def main():
         return
     sysm = abi.System.ising_square(L, L, 1)
     assert sysm.info.schedule == abi.SCHED_FUSED
-    sysm.upload(random_spins(L * L, 7))
+    sysm.upload(lattices.random_ising(L * L, 7))
     beta = 0.44
     # warmup (untimed)
     sysm.metropolis(beta, args.warmup, 2026, sweep0=0)

@@ -62,3 +62,24 @@ def test_sincos_turn_error_bound(tmp_path):
     r = subprocess.run([str(exe)], capture_output=True, text=True)
     print(r.stdout)
     assert r.returncode == 0, r.stdout
+
+
+def test_benchmark_lattices_match_the_oracle_tables():
+    """The numpy inputs bench.py builds (universal_classical_montecarlo_amd.lattices)
+    against the oracle's tables: cubic rows as multisets, the plaquette cells and
+    the tabulated 4-spin term."""
+    import numpy as np
+    from oracle_lib import cell_values, csr_cubic, delta_from_energy, square_plaquettes
+    from universal_classical_montecarlo_amd import lattices
+    for L in (3, 4, 6):
+        off, b = lattices.cubic_csr(L)
+        woff, wb = csr_cubic(L)
+        assert np.array_equal(off, woff)
+        for i in range(L ** 3):
+            assert sorted(b[off[i]:off[i + 1]]) == sorted(wb[woff[i]:woff[i + 1]])
+        assert b[1] == L   # the cubic detection reads L from row 0
+    assert np.array_equal(lattices.square_plaquettes(5, 3), square_plaquettes(5, 3))
+    delta, energy = lattices.four_spin_term(0.7)
+    want_e = np.array([-0.7 * np.prod(cell_values(x, 4)) for x in range(16)])
+    assert np.array_equal(energy, want_e)
+    assert np.array_equal(delta, delta_from_energy(want_e, 4))
