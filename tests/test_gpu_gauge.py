@@ -119,3 +119,31 @@ def test_su3_reunitarize_and_chain(gpu):
     acc = ser[-1].accepted / (4 * 4 * L ** 4)
     assert 0.3 < acc < 0.95
     sysm.close()
+
+
+def test_su3_plaquette_against_fp64_oracle_chain(gpu):
+    """The fp32 device chain against the FP64 oracle chain with the same keyed
+    uniforms (SU(3), L = 4, beta = 5.7, 4 hits, hot start): the same accept
+    counts sweep by sweep, and the FP64 plaquette of the fp32 chain within 1e-5
+    of the FP64 chain's (the stated precision of the FP32 link path)."""
+    N, L, hits, sweeps = 3, 4, 4, 4
+    links = hot_sun(N, L, 5)
+    cells, *_ = hypercubic(L)
+    o = oracle()
+    sysm = abi.System.sun_gauge(N, L, 5.7)
+    sysm.upload(links)
+    ser = sysm.metropolis(1.0, sweeps, 41, sweep0=0, proposal=abi.PROP_SU_SHIFT, width=0.2, hits=hits,
+                          flags=abi.RECORD_ENERGY | abi.RECORD_ACCEPTS, record=True)
+    want = links.copy()
+    worst = 0.0
+    for k in range(sweeps):
+        acc = np.zeros(1, np.uint64)
+        o.orc_sun_metropolis_keyed(N, L, want, 5.7, 1.0, 0.2, hits, 41, k, 1, acc, None)
+        p64 = o.orc_sun_mean_plaquette(N, 6 * L ** 4, cells, want)
+        worst = max(worst, abs(ser[k].plaquette - p64))
+        print(f"sweep {k}: accepts device {ser[k].accepted} oracle {int(acc[0])}, "
+              f"plaquette fp32 chain {ser[k].plaquette:.9f} FP64 chain {p64:.9f}", flush=True)
+        assert ser[k].accepted == int(acc[0])
+    print(f"max plaquette error against the FP64 chain: {worst:.3e}")
+    assert worst < 1e-5
+    sysm.close()
