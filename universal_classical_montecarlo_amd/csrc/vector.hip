@@ -282,7 +282,7 @@ __global__ __launch_bounds__(256) void vector_colour(VecArgs a)
             float nn = 0.f;
 #pragma unroll
             for (int k = 0; k < N; ++k) nn += p[k] * p[k];
-            const float inv = rsqrtf(nn);
+            const float inv = nn > 0.f ? rsqrtf(nn) : 1.f;   // a zero vector stays zero
 #pragma unroll
             for (int k = 0; k < N; ++k) si[k] = p[k] * inv;
         }
