@@ -8,7 +8,8 @@
 // (cell, position) as CellModel::energy_diff_at sums it (model/cell.hpp:66-79),
 // of the tabulated Flip change, in FP64; accept when log_p = -beta dE >= 0 or
 // log(u) < log_p with the site's 32-bit Ising uniform u (the word the other
-// Ising kernels draw, ising_u32).
+// Ising kernels draw, ising_u32; its low half only where the high half leaves
+// the test open).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -56,7 +57,7 @@ __global__ __launch_bounds__(256) void cells_colour(CellArgs a)
         // driver.hpp:79-84, Flip's log_ratio is 0; the uniform is drawn only
         // when log_p < 0
         const double log_p = -a.beta * dE + 0.0;
-        acc = log_p >= 0.0 || accept_draw(ising_u32(a.key, i), log_p);
+        acc = log_p >= 0.0 || ising_accept(a.key, i, log_p);
         if (acc) a.s[i] = static_cast<std::int8_t>(-a.s[i]);
         if (a.trace_dE) {
             a.trace_dE[i] = static_cast<float>(dE);

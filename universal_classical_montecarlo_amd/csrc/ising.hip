@@ -89,8 +89,7 @@ __global__ __launch_bounds__(256) void ising_csr_colour(CsrArgs a)
         int sum = 0;
         for (std::uint32_t q = a.offsets[i]; q < a.offsets[i + 1]; ++q) sum += a.field[a.bonds[q]];
         const int h = s * sum;
-        const std::uint32_t u = ising_u32(a.key, i);
-        acc = u <= a.tm1[h + static_cast<int>(a.max_degree)];
+        acc = ising_below(a.key, i, a.tm1[h + static_cast<int>(a.max_degree)]);
         if (acc) a.field[i] = static_cast<std::int8_t>(-s);
         if (a.trace_dE) {
             a.trace_dE[i] = static_cast<float>(a.twoJ * h);

@@ -73,8 +73,7 @@ __global__ __launch_bounds__(256) void fpi_colour(FpiArgs a)
         const int c_mp = at(a.s, L, xm, yp), c_0p = at(a.s, L, x, yp), c_pp = at(a.s, L, xp, yp);
         const int pq = si * (c_mm * c_0m * c_m0 + c_0m * c_pm * c_p0 + c_m0 * c_mp * c_0p + c_p0 * c_0p * c_pp);
         const int hn = si * nn;
-        const std::uint32_t u = ising_u32(a.key, i);
-        acc = u <= a.tm1[((hn + 4) / 2) * 5 + (pq + 4) / 2];
+        acc = ising_below(a.key, static_cast<std::uint32_t>(i), a.tm1[((hn + 4) / 2) * 5 + (pq + 4) / 2]);
         if (acc) a.s[i] = static_cast<std::int8_t>(-si);
         if constexpr (TRACE) {
             a.trace_dE[i] = static_cast<float>(a.twoJ1 * hn + a.twoK * pq);

@@ -148,6 +148,43 @@ __device__ __forceinline__ std::uint32_t ising_u32(std::uint64_t key, std::uint3
     return (ising_half(hi, j) << 16) | ising_half(lo, j);
 }
 
+// The two 16-bit halves of the same uniform separately, so that the low-half
+// Philox call runs only where the high half does not decide.
+__device__ __forceinline__ std::uint32_t ising_hi16(std::uint64_t key, std::uint32_t i)
+{
+    return ising_half(philox(U4{(i >> 3) & 63u, i >> 9, kTagIsingHi, 0u}, key), i & 7u);
+}
+__device__ __forceinline__ std::uint32_t ising_lo16(std::uint64_t key, std::uint32_t i)
+{
+    return ising_half(philox(U4{(i >> 3) & 63u, i >> 9, kTagIsingLo, 0u}, key), i & 7u);
+}
+
+// u <= t for site i's uniform u: the high halves decide unless they are equal.
+__device__ __forceinline__ bool ising_below(std::uint64_t key, std::uint32_t i, std::uint32_t t)
+{
+    const std::uint32_t hi = ising_hi16(key, i), th = t >> 16;
+    if (hi != th) return hi < th;
+    return ising_lo16(key, i) <= (t & 0xFFFFu);
+}
+
+// Forward declaration; defined below.
+__device__ __forceinline__ bool accept_draw(std::uint32_t k, double log_p);
+
+// driver.hpp:84's log(u) < log_p for site i's uniform u = k 2^-32.  The high
+// half bounds k to [hi 2^16, (hi + 1) 2^16): when log of both ends lies on one
+// side of log_p by more than accept_draw's fp32 margin the range decides;
+// otherwise (about one site in a thousand) the full word goes to accept_draw.
+__device__ __forceinline__ bool ising_accept(std::uint64_t key, std::uint32_t i, double log_p)
+{
+    const std::uint32_t hi = ising_hi16(key, i);
+    const float lp = static_cast<float>(log_p);
+    // above log of every k of the range (0 for hi = 0xFFFF)
+    if (__logf(static_cast<float>(hi + 1u) * 0x1.0p-16f) < lp - 1e-3f) return true;
+    // at most log of every k of the range (-inf for hi = 0)
+    if (__logf(static_cast<float>(hi) * 0x1.0p-16f) > lp + 1e-3f) return false;
+    return accept_draw((hi << 16) | ising_lo16(key, i), log_p);
+}
+
 // The Wolff activation uniform of edge e: word e & 3 of philox((e >> 2, 0, tag, 0)).
 __device__ __forceinline__ std::uint32_t edge_u32(std::uint64_t key, std::uint32_t e,
                                                            std::uint32_t tag = kTagWolffEdge)
