@@ -179,6 +179,8 @@ void cells_setup(System& s, const double* delta, const double* energy)
     for (std::size_t c = 0; c < s.ncells; ++c)
         for (int k = 0; k < A; ++k) inc[cursor[s.h_cells[c * A + k]]++] = static_cast<std::uint32_t>(c << 2) | k;
     // sites sharing a cell (duplicates and i itself are harmless to the colouring)
+    if (inc.size() * static_cast<std::size_t>(A - 1) > 0xFFFFFFFFull)
+        fail(UCMC_E_OVERFLOW, "CellComplex: too many incidences to colour");
     std::vector<std::uint32_t> aoff(n + 1, 0), adj;
     adj.reserve(inc.size() * (A - 1));
     for (std::size_t i = 0; i < n; ++i) {
