@@ -222,9 +222,11 @@ int ucmc_xy_angle_create(const ucmc_pair_graph* graph, double J, int device,
 /* Periodic L x L Ising with nearest-neighbour J1 and 4-spin plaquette K. */
 int ucmc_plaquette_ising_create(size_t L, double J1, double K, int device, ucmc_system** out);
 /* CellModel<IsingDof, Term, List>(list, term) with the term tabulated as in
- * ucmc_ising_cells (model/cell.hpp:31-35); the field starts all +1 (the
- * reference's starts at zero, which is not an Ising state: upload one).
- * Sites that share a cell get different colours (greedy, in site order). */
+ * ucmc_ising_cells (model/cell.hpp:31-35).  The device field starts all +1
+ * (CellModel's own starts at zero, which is not an Ising state): upload the
+ * model's field before running.  Sites that share a cell get different
+ * colours (greedy, in site order).  ucmc_observe reports energy = the sum of
+ * the tabulated cell energies (CellModel::total_energy) and order[0] = sum s_i. */
 int ucmc_ising_cells_create(const ucmc_ising_cells* model, int device, ucmc_system** out);
 int ucmc_system_destroy(ucmc_system* sys);
 int ucmc_system_info(const ucmc_system* sys, ucmc_info* out);
