@@ -46,6 +46,27 @@ def test_ising_wolff_bit_exact(gpu, L, beta, J, path):
     sysm.close()
 
 
+@pytest.mark.parametrize("path", PATHS)
+@pytest.mark.parametrize("shape", [(2, 2), (1, 4), (3, 5), (2, 6), (5, 7)])
+def test_ising_wolff_small_and_odd_graphs(gpu, shape, path):
+    """square_lattice tables with duplicate bonds (a side of 2), self-bonds (a side of
+    1) and odd sides (no checkerboard): the pair-graph path, against the replay."""
+    Lx, Ly = shape
+    print(f"ising wolff {Lx}x{Ly} path={path}", flush=True)
+    off, bonds = csr_square(Lx, Ly)
+    n = Lx * Ly
+    sysm = abi.System.ising(off, bonds, 1)
+    spins = random_spins(n, 23 + Lx, 1)
+    sysm.upload(spins)
+    sizes, _ = sysm.wolff(0.5, 200, 9, flip0=3, record=True, flags=path)
+    want = spins.copy()
+    wsz = np.zeros(200, np.uint64)
+    oracle().orc_ising_wolff_keyed(n, off, bonds, edge_ids(off, bonds), want, 1, 0.5, 9, 3, 200, wsz, None)
+    assert np.array_equal(sizes, wsz)
+    assert np.array_equal(sysm.download(), want)
+    sysm.close()
+
+
 @pytest.mark.parametrize("L", [64, 128])
 def test_ising_wolff_reordered_csr(gpu, L):
     """A square lattice whose CSR rows list the neighbours in another order
