@@ -54,12 +54,17 @@ CASES = [
     (2, ("square", 16), abi.PROP_ROTATION, 1.0, 1.1),
     (2, ("square", 12), abi.PROP_SPHERE, 0.0, 0.5),
     (4, ("square", 8), abi.PROP_ROTATION, 0.3, 0.7),
+    # odd sides and duplicate bonds: the general pair-graph path (greedy colours, CSR rows)
+    (3, ("square", (5, 7)), abi.PROP_ROTATION, 0.5, 0.6),
+    (2, ("square", (2, 6)), abi.PROP_SPHERE, 0.0, 0.4),
 ]
 
 
 def graph(geom):
     kind, L = geom
-    return csr_cubic(L) if kind == "cubic" else csr_square(L, L)
+    if kind == "cubic":
+        return csr_cubic(L)
+    return csr_square(*L) if isinstance(L, tuple) else csr_square(L, L)
 
 
 @pytest.mark.parametrize("N,geom,prop,width,beta", CASES)
