@@ -99,3 +99,23 @@ def test_pt_resume():
         c.restore(ck)
     for s in (a, b, c):
         s.close()
+
+
+def test_ising_cells_resume():
+    """A tabulated 4-spin plaquette term (CellModel over the Ising DoF)."""
+    from universal_classical_montecarlo_amd import lattices
+    L = 32
+    delta, energy = lattices.four_spin_term(0.9)
+    cells = lattices.square_plaquettes(L, L)
+    a = abi.System.ising_cells(L * L, cells, delta, energy)
+    a.upload(random_spins(L * L, 12, 3))
+    a.metropolis(0.6, 7, 4, sweep0=0)
+    ck = a.checkpoint()
+    a.metropolis(0.6, 7, 4, sweep0=7)
+    want = a.download()
+    b = abi.System.ising_cells(L * L, cells, delta, energy)
+    b.restore(ck)
+    b.metropolis(0.6, 7, 4, sweep0=7)
+    assert np.array_equal(b.download(), want)
+    a.close()
+    b.close()
