@@ -17,7 +17,7 @@ def f32(a):
 
 
 # ------------------------------------------------------------ XY angles --
-@pytest.mark.parametrize("L,beta,width", [(16, 1.1, 1.0), (10, 0.8, 2.0)])
+@pytest.mark.parametrize("L,beta,width", [(16, 1.1, 1.0), (10, 0.8, 2.0), (7, 0.9, 1.5)])  # odd: greedy colours
 def test_xy_angle_trace(gpu, L, beta, width):
     off, bonds = csr_square(L, L)
     sysm = abi.System.xy_angle(off, bonds, 1.0)
