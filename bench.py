@@ -73,6 +73,38 @@ def cpu_baseline(L, seconds_target=12.0):
             "seconds": dt}
 
 
+def cpu_baseline_all_cores(L, threads=None, seconds_target=6.0):
+    """The same compiled reference chain on every host core of this GPU's share
+    (at most 16): independent chains, one per thread, each on its own copy of
+    an L x L lattice with its own stream, aggregated (the ctypes call releases
+    the GIL).  Reported beside the one-core baseline."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle_lib import REF_SO, ref, square_edges
+    from universal_classical_montecarlo_amd import lattices
+    if not os.path.exists(REF_SO):
+        return None
+    threads = threads or max(1, min(16, os.cpu_count() or 1))
+    n = L * L
+    r = ref()
+    ei, ej = square_edges(L, L)
+    spins = lattices.random_ising(n, 7)
+    def run(args):
+        stream, k = args
+        s = spins.copy()
+        r.ref_ising_metropolis_seq(n, len(ei), ei, ej, s, 1, 0.44, 2026, stream, k, 0, None, None, None)
+    t0 = time.perf_counter()
+    run((1, 1))
+    sweeps = max(1, int(seconds_target / max(time.perf_counter() - t0, 1e-6)))
+    with ThreadPoolExecutor(threads) as ex:
+        t0 = time.perf_counter()
+        list(ex.map(run, [(1 + t, sweeps) for t in range(threads)]))
+        dt = time.perf_counter() - t0
+    return {"value": threads * n * sweeps / dt, "unit": "site-updates/s", "cores": threads,
+            "kind": "reference",
+            "sample": f"{threads} independent chains x {sweeps} sweeps of the {L}x{L} lattice, "
+                      "one thread each"}
+
+
 def expm_products(width, samples=200000, seed=1):
     """Mean number of 3x3 complex products in the device expm of an SUShift<3>
     generator (Horner terms m plus squarings s, with the kernel's rule: halve
@@ -283,6 +315,7 @@ def main():
         s2.close()
 
     cpu = None if args.no_cpu else cpu_baseline(L)
+    cpu_all = None if args.no_cpu else cpu_baseline_all_cores(1024)
     extra = None if args.no_suite else suite(quick=args.quick_suite)
     line = {
         "metric": METRIC,
@@ -306,6 +339,7 @@ def main():
                      "bytes_per_site": BYTES_PER_SITE},
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in
                                                   ("value", "unit", "cores", "kind", "sample")},
+        "cpu_baseline_all_cores": cpu_all,
         "out_of_cache": ooc,
         "other_configs": extra,
     }
