@@ -236,13 +236,18 @@ __global__ __launch_bounds__(256) void vector_colour(VecArgs a)
                 z = yz / L;
             }
             const std::uint32_t x = 2 * xh + ((a.colour + y + z) & 1u);
-            i = (z * L + y) * L + x;
-            nb[0] = (z * L + y) * L + (x + 1 == L ? 0 : x + 1);
-            nb[1] = (z * L + y) * L + (x == 0 ? L - 1 : x - 1);
-            nb[2] = (z * L + (y + 1 == L ? 0 : y + 1)) * L + x;
-            nb[3] = (z * L + (y == 0 ? L - 1 : y - 1)) * L + x;
-            nb[4] = ((z + 1 == L ? 0 : z + 1) * L + y) * L + x;
-            nb[5] = ((z == 0 ? L - 1 : z - 1) * L + y) * L + x;
+            // L^3 < 2^32 keeps L and z L + y below 2^24: full-rate 24-bit
+            // multiplies instead of quarter-rate 32-bit ones
+            auto lin = [L](std::uint32_t xx, std::uint32_t yy, std::uint32_t zz) {
+                return __umul24(__umul24(zz, L) + yy, L) + xx;
+            };
+            i = lin(x, y, z);
+            nb[0] = lin(x + 1 == L ? 0 : x + 1, y, z);
+            nb[1] = lin(x == 0 ? L - 1 : x - 1, y, z);
+            nb[2] = lin(x, y + 1 == L ? 0 : y + 1, z);
+            nb[3] = lin(x, y == 0 ? L - 1 : y - 1, z);
+            nb[4] = lin(x, y, z + 1 == L ? 0 : z + 1);
+            nb[5] = lin(x, y, z == 0 ? L - 1 : z - 1);
             deg = 6;
         } else {
             i = a.sites[t];
