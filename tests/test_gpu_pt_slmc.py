@@ -212,7 +212,7 @@ def test_slmc_local_fit(gpu):
     sysm.close()
 
 
-@pytest.mark.parametrize("L", [16, 64])   # breadth-first growth / label propagation
+@pytest.mark.parametrize("L", [16, 32, 64])   # breadth-first growth / label propagation (2)
 def test_slmc_step_matches_replay(gpu, L):
     sysm = abi.System.plaquette_ising(L, 1.0, 0.2)
     s = random_spins(L * L, 6, 2)

@@ -7,9 +7,13 @@
 //    lattices, and as the simple reference-shaped path.
 //  * ising_fused: periodic square lattice with Lx % 512 == 0 and Ly % H == 0.
 //    Both checkerboard colours of a sweep in one pass: each wave owns a 512-byte
-//    column band and an H-row block, loads the block and its two halo rows
-//    above and below into registers at once, then walks down the rows (colour 0
-//    of row j+1, then colour 1 of row j) and stores each row once (2 B/site).
+//    column band and an H-row block, keeps the block and its two halo rows
+//    above and below in registers (each row loaded a step or more before its
+//    first use), walks down the rows (colour 0 of row j+1, then colour 1 of
+//    row j) and stores each row once (2 B/site).  The sites just left and
+//    right of the band (owned by the neighbouring bands) get their colour-0
+//    update recomputed first by a pre-pass on the two half-waves, so colour 1
+//    at the band's edges sees the values their owners will store.
 //    Rows are ping-ponged between two buffers so halo rows are always read
 //    unmodified.
 //

@@ -35,7 +35,10 @@ namespace ucmc {
 namespace {
 
 constexpr std::uint64_t kSlmcCounterBase = 3ull << 61;
-constexpr std::size_t kSlmcLabelMinSites = 4096;   // from 64 x 64 on, label propagation
+// From 32 x 32 on, label propagation (five launches per step, 30-40 us); below
+// it, breadth-first growth in one workgroup (at 32 x 32, beta 0.40, clusters of
+// 600 of 1024 sites took 385 us per step that way).
+constexpr std::size_t kSlmcLabelMinSites = 1024;
 
 // --------------------------------------------------- plain Metropolis --
 struct FpiArgs {
