@@ -285,9 +285,8 @@ void cells_observe(System& s, ucmc_observables& o)
 std::uint64_t cells_bench(System& s, const ucmc_metropolis_params& p, int repeats, float& ms)
 {
     check_flip(p);
-    hipEvent_t e0, e1;
-    UCMC_HIP(hipEventCreate(&e0));
-    UCMC_HIP(hipEventCreate(&e1));
+    EventPair ev;
+    hipEvent_t e0 = ev.e0, e1 = ev.e1;
     UCMC_HIP(hipStreamSynchronize(s.stream));
     UCMC_HIP(hipEventRecord(e0, s.stream));
     for (int r = 0; r < repeats; ++r)
@@ -296,8 +295,6 @@ std::uint64_t cells_bench(System& s, const ucmc_metropolis_params& p, int repeat
     UCMC_HIP(hipEventRecord(e1, s.stream));
     UCMC_HIP(hipEventSynchronize(e1));
     UCMC_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     return static_cast<std::uint64_t>(repeats) * p.nsweeps * s.ncolours;
 }
 

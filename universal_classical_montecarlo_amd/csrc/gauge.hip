@@ -755,9 +755,8 @@ void gauge_reunitarize(System& s)
 std::uint64_t gauge_bench(System& s, const ucmc_metropolis_params& p, int repeats, float& ms)
 {
     check_proposal(s, p);
-    hipEvent_t e0, e1;
-    UCMC_HIP(hipEventCreate(&e0));
-    UCMC_HIP(hipEventCreate(&e1));
+    EventPair ev;
+    hipEvent_t e0 = ev.e0, e1 = ev.e1;
     UCMC_HIP(hipStreamSynchronize(s.stream));
     UCMC_HIP(hipEventRecord(e0, s.stream));
     std::uint64_t launches = 0;
@@ -773,8 +772,6 @@ std::uint64_t gauge_bench(System& s, const ucmc_metropolis_params& p, int repeat
     UCMC_HIP(hipEventRecord(e1, s.stream));
     UCMC_HIP(hipEventSynchronize(e1));
     UCMC_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     return launches;
 }
 

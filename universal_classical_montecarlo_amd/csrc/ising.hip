@@ -783,9 +783,8 @@ std::uint64_t ising_bench(System& s, const ucmc_metropolis_params& p, int repeat
     if ((simple || !fused_ok(s)) && !s.offsets.p) square_csr(s);
     const IsingTables t = make_tables(s, p.beta);
     upload_csr_table(s, t);
-    hipEvent_t e0, e1;
-    UCMC_HIP(hipEventCreate(&e0));
-    UCMC_HIP(hipEventCreate(&e1));
+    EventPair ev;
+    hipEvent_t e0 = ev.e0, e1 = ev.e1;
     UCMC_HIP(hipStreamSynchronize(s.stream));
     UCMC_HIP(hipEventRecord(e0, s.stream));
     std::uint64_t launches = 0;
@@ -798,8 +797,6 @@ std::uint64_t ising_bench(System& s, const ucmc_metropolis_params& p, int repeat
     UCMC_HIP(hipEventRecord(e1, s.stream));
     UCMC_HIP(hipEventSynchronize(e1));
     UCMC_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     return launches;
 }
 

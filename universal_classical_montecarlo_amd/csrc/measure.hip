@@ -30,20 +30,17 @@ extern "C" int ucmc_measure_copy(size_t bytes, int repeats, double* gbps)
         a.alloc(n * 16);
         b.alloc(n * 16);
         UCMC_HIP(hipMemset(a.p, 1, n * 16));
-        hipEvent_t e0, e1;
-        UCMC_HIP(hipEventCreate(&e0));
-        UCMC_HIP(hipEventCreate(&e1));
+        EventPair ev;
         const dim3 grid(8192), block(256);
         hipLaunchKernelGGL(ucmc_copy_kernel, grid, block, 0, nullptr, a.as<uint4>(), b.as<uint4>(), n);
-        UCMC_HIP(hipEventRecord(e0, nullptr));
+        UCMC_HIP(hipGetLastError());
+        UCMC_HIP(hipEventRecord(ev.e0, nullptr));
         for (int r = 0; r < repeats; ++r)
             hipLaunchKernelGGL(ucmc_copy_kernel, grid, block, 0, nullptr, a.as<uint4>(), b.as<uint4>(), n);
-        UCMC_HIP(hipEventRecord(e1, nullptr));
-        UCMC_HIP(hipEventSynchronize(e1));
+        UCMC_HIP(hipEventRecord(ev.e1, nullptr));
+        UCMC_HIP(hipEventSynchronize(ev.e1));
         float ms = 0.f;
-        UCMC_HIP(hipEventElapsedTime(&ms, e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+        UCMC_HIP(hipEventElapsedTime(&ms, ev.e0, ev.e1));
         *gbps = 2.0 * static_cast<double>(n * 16) * repeats / (ms * 1e-3) / 1e9;
         return UCMC_OK;
     } catch (const Error& e) {

@@ -800,9 +800,8 @@ std::uint64_t fpi_bench(System& s, const ucmc_metropolis_params& p, int repeats,
     std::vector<std::uint32_t> t;
     fpi_tables(s, p.beta, t);
     UCMC_HIP(hipMemcpyAsync(s.scratch2.p, t.data(), 100, hipMemcpyHostToDevice, s.stream));
-    hipEvent_t e0, e1;
-    UCMC_HIP(hipEventCreate(&e0));
-    UCMC_HIP(hipEventCreate(&e1));
+    EventPair ev;
+    hipEvent_t e0 = ev.e0, e1 = ev.e1;
     UCMC_HIP(hipStreamSynchronize(s.stream));
     UCMC_HIP(hipEventRecord(e0, s.stream));
     for (int r = 0; r < repeats; ++r)
@@ -812,8 +811,6 @@ std::uint64_t fpi_bench(System& s, const ucmc_metropolis_params& p, int repeats,
     UCMC_HIP(hipEventRecord(e1, s.stream));
     UCMC_HIP(hipEventSynchronize(e1));
     UCMC_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     return static_cast<std::uint64_t>(repeats) * p.nsweeps * 4;
 }
 

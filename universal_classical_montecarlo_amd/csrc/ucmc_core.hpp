@@ -27,6 +27,29 @@ inline void hip_check(hipError_t e, const char* what)
 }
 #define UCMC_HIP(x) ::ucmc::hip_check((x), #x)
 
+// Two HIP events for timing a region of a stream, destroyed on every path out
+// (a throw between their creation and the end of the timing included).
+struct EventPair {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    EventPair()
+    {
+        UCMC_HIP(hipEventCreate(&e0));
+        const hipError_t r = hipEventCreate(&e1);
+        if (r != hipSuccess) {
+            (void)hipEventDestroy(e0);
+            e0 = nullptr;
+            UCMC_HIP(r);
+        }
+    }
+    EventPair(const EventPair&) = delete;
+    EventPair& operator=(const EventPair&) = delete;
+    ~EventPair()
+    {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+};
+
 // Owning device buffer.
 struct DevBuf {
     void* p = nullptr;

@@ -524,9 +524,8 @@ void vector_observe(System& s, ucmc_observables& o)
 std::uint64_t vector_bench(System& s, const ucmc_metropolis_params& p, int repeats, float& ms)
 {
     check_proposal(p);
-    hipEvent_t e0, e1;
-    UCMC_HIP(hipEventCreate(&e0));
-    UCMC_HIP(hipEventCreate(&e1));
+    EventPair ev;
+    hipEvent_t e0 = ev.e0, e1 = ev.e1;
     UCMC_HIP(hipStreamSynchronize(s.stream));
     UCMC_HIP(hipEventRecord(e0, s.stream));
     std::uint64_t launches = 0;
@@ -538,8 +537,6 @@ std::uint64_t vector_bench(System& s, const ucmc_metropolis_params& p, int repea
     UCMC_HIP(hipEventRecord(e1, s.stream));
     UCMC_HIP(hipEventSynchronize(e1));
     UCMC_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     return launches;
 }
 

@@ -442,9 +442,8 @@ void xy_observe(System& s, ucmc_observables& o)
 std::uint64_t xy_bench(System& s, const ucmc_metropolis_params& p, int repeats, float& ms)
 {
     if (p.proposal != UCMC_PROP_U1_SHIFT) fail(UCMC_E_UNSUPPORTED, "XY angles take U1Shift");
-    hipEvent_t e0, e1;
-    UCMC_HIP(hipEventCreate(&e0));
-    UCMC_HIP(hipEventCreate(&e1));
+    EventPair ev;
+    hipEvent_t e0 = ev.e0, e1 = ev.e1;
     UCMC_HIP(hipStreamSynchronize(s.stream));
     UCMC_HIP(hipEventRecord(e0, s.stream));
     std::uint64_t n = 0;
@@ -456,8 +455,6 @@ std::uint64_t xy_bench(System& s, const ucmc_metropolis_params& p, int repeats, 
     UCMC_HIP(hipEventRecord(e1, s.stream));
     UCMC_HIP(hipEventSynchronize(e1));
     UCMC_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     return n;
 }
 
@@ -666,9 +663,8 @@ void pt_autotune(System& s, double width, std::uint32_t* tile, double* best_ms)
         p.nsweeps = 4;
         p.tile = t;
         pt_run(s, p, nullptr, nullptr); // warm-up
-        hipEvent_t e0, e1;
-        UCMC_HIP(hipEventCreate(&e0));
-        UCMC_HIP(hipEventCreate(&e1));
+        EventPair ev;
+        hipEvent_t e0 = ev.e0, e1 = ev.e1;
         p.nsweeps = 20;
         UCMC_HIP(hipEventRecord(e0, s.stream));
         pt_run(s, p, nullptr, nullptr);
@@ -676,8 +672,6 @@ void pt_autotune(System& s, double width, std::uint32_t* tile, double* best_ms)
         UCMC_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
         UCMC_HIP(hipEventElapsedTime(&ms, e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
         if (ms / 20.0 < best) {
             best = ms / 20.0;
             best_t = t;
