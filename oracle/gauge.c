@@ -620,3 +620,27 @@ void orc_slmc_keyed(size_t L, int8_t* s, double J1, double K, const double* Jeff
     free(in);
     free(q);
 }
+
+/* Least-squares loss and gradient of the SLMC effective model over n
+ * configurations, from their correlation rows corr[c] = (C1, C2, C3, P, M)
+ * (orc_fpi_correlations) on N sites.  Each configuration's features are
+ * x = (1, -C1/N, -C2/N, -C3/N) and its target y = (-J1 C1 - K P) / N, each
+ * rounded to binary32 as the training rows are stored; then, in double,
+ * out[0] = sum_c (theta . x_c - y_c)^2 and out[1 + k] = sum_c 2 (theta . x_c -
+ * y_c) x_c[k].  No counterpart in the reference (it has no learned model). */
+void orc_slmc_loss_grad(size_t n, size_t N, const double* corr, double J1, double K,
+                        const double* theta, double* out)
+{
+    for (int k = 0; k < 5; ++k) out[k] = 0.0;
+    for (size_t c = 0; c < n; ++c) {
+        const double* q = corr + 5 * c;
+        const double Nd = (double)N;
+        const double x[4] = {1.0, (double)(float)(-q[0] / Nd), (double)(float)(-q[1] / Nd),
+                             (double)(float)(-q[2] / Nd)};
+        const double y = (double)(float)((-J1 * q[0] - K * q[3]) / Nd);
+        double r = -y;
+        for (int k = 0; k < 4; ++k) r += theta[k] * x[k];
+        out[0] += r * r;
+        for (int k = 0; k < 4; ++k) out[1 + k] += 2.0 * r * x[k];
+    }
+}

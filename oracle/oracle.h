@@ -268,6 +268,10 @@ void   orc_fpi_correlations(size_t L, const int8_t* s, double* out5);
 void   orc_slmc_keyed(size_t L, int8_t* s, double J1, double K, const double* Jeff, double beta,
                       uint64_t run_seed, uint64_t flip0, size_t nflips, uint64_t* size_out,
                       uint8_t* accepted_out);
+/* SLMC least-squares loss and gradient over correlation rows (n x 5): out5 =
+ * (loss, dloss/dtheta[0..3]). */
+void   orc_slmc_loss_grad(size_t n, size_t N, const double* corr, double J1, double K,
+                          const double* theta, double* out5);
 
 #ifdef __cplusplus
 }

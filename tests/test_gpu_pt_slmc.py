@@ -144,7 +144,8 @@ def test_plaquette_ising_bit_exact(gpu):
 
 def test_slmc_training_kernels(gpu):
     """Device reductions vs the oracle, the MFMA normal equations vs a float64
-    least-squares solve, and the forward / backward gradient vs numpy."""
+    least-squares solve, and the forward / backward loss and gradient vs the
+    oracle (orc_slmc_loss_grad)."""
     L = 32
     sysm = abi.System.plaquette_ising(L, 1.0, 0.2)
     corr = sysm.slmc_collect(0.42, 200, 400, 2, 5)
@@ -166,9 +167,10 @@ def test_slmc_training_kernels(gpu):
     # gradient of sum (X theta - y)^2 at an arbitrary point
     th = np.array([0.1, 0.9, 0.05, -0.02])
     loss, grad = sysm.slmc_loss_grad(th)
-    r = X32 @ th - y32
-    assert loss == pytest.approx((r ** 2).sum(), rel=1e-9)
-    assert np.allclose(grad, 2.0 * X32.T @ r, rtol=1e-9, atol=1e-9)
+    want = np.zeros(5)
+    oracle().orc_slmc_loss_grad(400, N, np.ascontiguousarray(corr, dtype=np.float64), 1.0, 0.2, th, want)
+    assert loss == pytest.approx(want[0], rel=1e-9)
+    assert np.allclose(grad, want[1:], rtol=1e-9, atol=1e-9)
     # Adam reaches the least-squares solution
     th_adam, mloss = sysm.slmc_fit_adam(3000, 0.01, np.zeros(4))
     assert mloss <= ((X32 @ theta_ref - y32) ** 2).mean() * 1.05 + 1e-9

@@ -649,3 +649,43 @@ def test_cells_oracle_plaquette_chain_samples_exact_distribution():
     from universal_classical_montecarlo_amd import stats
     mean, err, _ = stats.mean_error(np.array(es))
     assert abs(mean - exact) < 4 * err + 1e-3, (mean, err, exact)
+
+
+def test_slmc_loss_grad_oracle_pinned():
+    """orc_slmc_loss_grad: its gradient is the central difference of its loss
+    (exact for a quadratic), and its loss is minimised, with a zero gradient,
+    at the least-squares fit of the configurations' energies per site
+    (orc_fpi_energy) on the shell features from orc_fpi_correlations."""
+    o = oracle()
+    L, J1, K, n = 8, 1.0, 0.2, 60
+    N = L * L
+    rng = np.random.default_rng(17)
+    corr = np.zeros((n, 5))
+    y = np.zeros(n)
+    for c in range(n):
+        # configurations with a spread of order: +1 with probability p
+        s = np.where(rng.random(N) < rng.uniform(0.2, 1.0), 1, -1).astype(np.int8)
+        o.orc_fpi_correlations(L, s, corr[c])
+        y[c] = o.orc_fpi_energy(L, s, J1, K) / N
+    f32r = lambda v: np.float32(v).astype(np.float64)
+    X = np.stack([np.ones(n), f32r(-corr[:, 0] / N), f32r(-corr[:, 1] / N), f32r(-corr[:, 2] / N)], axis=1)
+
+    def lg(th):
+        out = np.zeros(5)
+        o.orc_slmc_loss_grad(n, N, np.ascontiguousarray(corr), J1, K, np.asarray(th, np.float64), out)
+        return out
+
+    th = np.array([0.3, 0.8, -0.1, 0.05])
+    g = lg(th)[1:]
+    h = 1e-3
+    for k in range(4):
+        e = np.zeros(4)
+        e[k] = h
+        fd = (lg(th + e)[0] - lg(th - e)[0]) / (2 * h)
+        assert fd == pytest.approx(g[k], rel=1e-6, abs=1e-9)
+    theta_ls, *_ = np.linalg.lstsq(X, f32r(y), rcond=None)
+    at = lg(theta_ls)
+    assert np.abs(at[1:]).max() <= 1e-9 * max(1.0, np.abs(X).sum())
+    assert at[0] == pytest.approx(((X @ theta_ls - f32r(y)) ** 2).sum(), rel=1e-9, abs=1e-12)
+    for _ in range(5):
+        assert lg(theta_ls + rng.normal(scale=0.05, size=4))[0] > at[0]
