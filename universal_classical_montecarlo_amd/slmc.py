@@ -144,3 +144,35 @@ def _rates(e, m, secs, unit):
         # a run shorter than ~50 tau under-estimates tau; say so
         r[f"tau_reliable_{name}"] = bool(len(x) > 50 * tau)
     return r
+
+
+def main(argv=None):
+    """`python -m universal_classical_montecarlo_amd.slmc --L 256 --betas 0.40,0.43`:
+    one JSON line per temperature with the fitted couplings, SLMC acceptance and
+    mean cluster, tau_int of E and |M| for both samplers and the speed-ups in
+    independent samples / s (the model is trained at each temperature)."""
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--L", type=int, default=256)
+    ap.add_argument("--K", type=float, default=0.2)
+    ap.add_argument("--betas", default="0.40,0.41,0.42,0.43,0.44,0.45,0.46")
+    ap.add_argument("--nconfigs", type=int, default=20000)
+    ap.add_argument("--plain-sweeps", type=int, default=200000)
+    ap.add_argument("--slmc-steps", type=int, default=100000)
+    args = ap.parse_args(argv)
+    for b in (float(x) for x in args.betas.split(",")):
+        r = compare(L=args.L, K=args.K, beta=b, nconfigs=args.nconfigs, plain_sweeps=args.plain_sweeps,
+                    slmc_steps=args.slmc_steps)
+        p, s = r["plain"], r["slmc"]
+        print(json.dumps({"L": args.L, "K": args.K, "beta": b, "J_eff": r["J_eff"],
+                          "acceptance": s["acceptance"], "mean_cluster": s["mean_cluster"],
+                          "plain_sweeps_per_s": p["per_s"], "slmc_steps_per_s": s["per_s"],
+                          "tau_plain": [p["tau_E"], p["tau_absM"]], "tau_slmc": [s["tau_E"], s["tau_absM"]],
+                          "tau_reliable": [p["tau_reliable_E"], p["tau_reliable_absM"],
+                                           s["tau_reliable_E"], s["tau_reliable_absM"]],
+                          "speedup_E": r["speedup_E"], "speedup_absM": r["speedup_absM"]}), flush=True)
+
+
+if __name__ == "__main__":
+    main()
